@@ -1,0 +1,274 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes front end of the CPU restatement (``liboracle.so``, built from
+rd_oracle*.c) and, where it was built, of the reference harness
+(``_ref/librdref.so``, the RUMDEED reference modules compiled from their own
+sources).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg import this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_int32, c_uint64
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "librdref.so")
+
+GEOM_PLANAR = 1
+GEOM_TIP = 2
+
+
+class Geom(ctypes.Structure):
+    """Mirror of ``rdo_geom`` (oracle/rd_oracle.h)."""
+
+    _fields_ = [
+        ("kind", c_int32), ("image_charge", c_int32), ("n_ic_max", c_int32), ("pad0", c_int32),
+        ("d", c_double), ("V", c_double), ("E_z", c_double),
+        ("a_foci", c_double), ("shift_z", c_double), ("pre_fac_E_tip", c_double),
+        ("pre_fac_E_tip_unit_voltage", c_double),
+        ("h_tip", c_double), ("r_tip", c_double), ("eta_1", c_double), ("max_xi", c_double),
+        ("d_tip", c_double), ("R_base", c_double), ("theta_tip", c_double),
+    ]
+
+
+class Consts(ctypes.Structure):
+    _fields_ = [(n, c_double) for n in (
+        "pi", "h", "k_b", "c", "mu_0", "epsilon_0", "h_bar", "m_0", "q_0", "q_02", "m_u",
+        "m_N2", "m_N2p", "div_fac_c", "length_scale", "time_scale", "eps_soft",
+        "a_FN", "b_FN", "l_const")]
+
+
+_dp = POINTER(c_double)
+_ip = POINTER(c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        _lib = ctypes.CDLL(ORACLE_SO)
+        L = _lib
+        L.rdo_log.restype = c_double
+        L.rdo_log.argtypes = [c_double]
+        L.rdo_exp.restype = c_double
+        L.rdo_exp.argtypes = [c_double]
+        L.rdo_sum_chunk.restype = c_int
+    return _lib
+
+
+def consts() -> Consts:
+    c = Consts()
+    lib().rdo_get_consts(ctypes.byref(c))
+    return c
+
+
+def geom_planar(d, V, image_charge=True, n_ic_max=0) -> Geom:
+    g = Geom()
+    lib().rdo_geom_planar(ctypes.byref(g), c_double(d), c_double(V), c_int(int(image_charge)),
+                          c_int(n_ic_max))
+    return g
+
+
+def geom_tip(V, d_tip, R_base, h_tip, image_charge=True) -> Geom:
+    g = Geom()
+    lib().rdo_geom_tip(ctypes.byref(g), c_double(V), c_double(d_tip), c_double(R_base),
+                       c_double(h_tip), c_int(int(image_charge)))
+    return g
+
+
+def accel(g: Geom, pos, q, m, nthreads=None, long_double=False):
+    """Accelerations (n,3) of particles pos (n,3), charge q, mass m."""
+    pos = _f64(pos)
+    n = pos.shape[0]
+    q = _f64(q)
+    m = _f64(m)
+    out = np.zeros((n, 3))
+    nt = nthreads or os.cpu_count() or 1
+    fn = lib().rdo_accel_ld if long_double else lib().rdo_accel
+    fn(ctypes.byref(g), c_int(n), _d(pos), _d(q), _d(m), _d(out), c_int(nt))
+    return out
+
+
+def field(g: Geom, pos, q, pts, nthreads=None, long_double=False):
+    """Field (M,3) at points pts (M,3) from particles pos (n,3) with charge q."""
+    pos = _f64(pos).reshape(-1, 3)
+    q = _f64(q)
+    pts = _f64(pts).reshape(-1, 3)
+    out = np.zeros((pts.shape[0], 3))
+    nt = nthreads or os.cpu_count() or 1
+    fn = lib().rdo_field_ld if long_double else lib().rdo_field
+    fn(ctypes.byref(g), c_int(pos.shape[0]), _d(pos), _d(q), c_int(pts.shape[0]), _d(pts),
+       _d(out), c_int(nt))
+    return out
+
+
+def vacuum_field(g: Geom, p):
+    p = _f64(p)
+    out = np.zeros(3)
+    lib().rdo_vacuum_field(ctypes.byref(g), _d(p), _d(out))
+    return out
+
+
+def log(x):
+    return lib().rdo_log(float(x))
+
+
+def exp(x):
+    return lib().rdo_exp(float(x))
+
+
+def philox4x64(ctr, key):
+    out = (c_uint64 * 4)()
+    lib().rdo_philox4x64((c_uint64 * 4)(*ctr), (c_uint64 * 2)(*key), out)
+    return [int(v) for v in out]
+
+
+# ---------------------------------------------------------------------------
+# Reference harness (the reference's own routines, built by `make -C oracle ref`)
+class Ref:
+    """Front end of oracle/_ref/librdref.so.  The reference keeps its state
+    in module variables, so there is one instance per process."""
+
+    _inst = None
+
+    def __init__(self, cap=1 << 16):
+        if not os.path.exists(REF_SO):
+            raise RuntimeError(f"reference harness not built: {REF_SO}")
+        self.L = ctypes.CDLL(REF_SO)
+        self.L.ref_jgtf.restype = c_double
+        self.L.ref_jgtf.argtypes = [c_double, c_double, c_double]
+        self.L.ref_init(c_int(cap))
+        self.cap = cap
+
+    @classmethod
+    def get(cls, cap=1 << 16):
+        if cls._inst is None:
+            cls._inst = Ref(cap)
+        return cls._inst
+
+    @staticmethod
+    def available():
+        return os.path.exists(REF_SO)
+
+    def threads(self, n):
+        self.L.ref_set_threads(c_int(n))
+
+    def consts(self):
+        out = np.zeros(12)
+        self.L.ref_consts(_d(out))
+        return out
+
+    def setup_planar(self, d, V, dt=0.25e-15, image_charge=True, n_ic_max=0):
+        self.L.ref_setup_planar(c_double(d), c_double(V), c_double(dt),
+                                c_int(int(image_charge)), c_int(n_ic_max))
+
+    def setup_tip(self, V, d_tip, R_base, h_tip, dt=0.25e-15, image_charge=True):
+        self.L.ref_setup_tip(c_double(V), c_double(d_tip), c_double(R_base), c_double(h_tip),
+                             c_double(dt), c_int(int(image_charge)))
+
+    def tip_params(self):
+        out = np.zeros(8)
+        self.L.ref_tip_params(_d(out))
+        return out
+
+    def clear(self):
+        self.L.ref_clear()
+
+    def put_snapshot(self, pos, q, m, species=None):
+        pos = _f64(pos)
+        n = pos.shape[0]
+        assert n <= self.cap
+        sp = np.ones(n, dtype=np.int32) if species is None else np.ascontiguousarray(species, np.int32)
+        self.L.ref_put_snapshot(c_int(n), _d(pos), _d(_f64(q)), _d(_f64(m)), _i(sp))
+
+    def add_particle(self, pos, vel, species=1, step=1, emit=1, sec=1):
+        self.L.ref_add_particle(_d(_f64(pos)), _d(_f64(vel)), c_int(species), c_int(step),
+                                c_int(emit), c_int(sec))
+
+    def accel(self, which, n):
+        """which: 0 = planar-specialized pair loop, 1 = generic, 2 = gather (ACC form)."""
+        out = np.zeros((n, 3))
+        self.L.ref_accel(c_int(which), c_int(n), _d(out))
+        return out
+
+    def field_at(self, pts):
+        pts = _f64(pts).reshape(-1, 3)
+        out = np.zeros_like(pts)
+        self.L.ref_field_at(c_int(pts.shape[0]), _d(pts), _d(out))
+        return out
+
+    def update_position(self, step):
+        self.L.ref_update_position(c_int(step))
+
+    def remove_particles(self, step):
+        self.L.ref_remove_particles(c_int(step))
+
+    def mark_remove(self, i1, reason):
+        self.L.ref_mark_remove(c_int(i1), c_int(reason))
+
+    def counts(self):
+        out = np.zeros(16, dtype=np.int32)
+        self.L.ref_counts(_i(out))
+        keys = ["nrPart", "nrElec", "nrIon", "nrPart_remove", "nrElec_remove", "nrIon_remove",
+                "nrPart_remove_top", "nrPart_remove_bot", "nrElec_remove_top",
+                "nrElec_remove_bot", "nrIon_remove_top", "nrIon_remove_bot", "nrID",
+                "charge_rev", "nrElec_remove_top_emit1", "nrPart_dropped"]
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def state(self, n):
+        arrs = [np.zeros((n, 3)) for _ in range(6)]
+        q = np.zeros(n)
+        m = np.zeros(n)
+        ints = np.zeros((n, 6), dtype=np.int32)
+        self.L.ref_get_state(c_int(n), *[_d(a) for a in arrs], _d(q), _d(m), _i(ints))
+        keys = ["pos", "prev_pos", "vel", "acc", "acc_prev", "acc_prev2"]
+        st = dict(zip(keys, arrs))
+        st.update(q=q, m=m, species=ints[:, 0], step=ints[:, 1], emitter=ints[:, 2],
+                  section=ints[:, 3], id=ints[:, 4], mask=ints[:, 5])
+        return st
+
+    def ramo(self):
+        out = np.zeros(4)
+        self.L.ref_ramo(_d(out))
+        return out
+
+    def life_time(self):
+        out = np.zeros((3, 1000), dtype=np.int32)
+        self.L.ref_life_time(_i(out))
+        return out.T
+
+    def jgtf(self, F, T, w):
+        return self.L.ref_jgtf(F, T, w)
+
+    def read_work(self, x0, y0, lx, ly):
+        self.L.ref_read_work(c_double(x0), c_double(y0), c_double(lx), c_double(ly))
+
+    def work_at(self, pts):
+        pts = _f64(pts).reshape(-1, 3)
+        w = np.zeros(pts.shape[0])
+        sec = np.zeros(pts.shape[0], dtype=np.int32)
+        self.L.ref_work_at(c_int(pts.shape[0]), _d(pts), _d(w), _i(sec))
+        return w, sec
