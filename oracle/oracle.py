@@ -64,6 +64,13 @@ def _f64(a, shape=None):
 _lib = None
 
 
+def default_threads() -> int:
+    """OpenMP threads for the oracle: OMP_NUM_THREADS if set, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -106,7 +113,7 @@ def accel(g: Geom, pos, q, m, nthreads=None, long_double=False):
     q = _f64(q)
     m = _f64(m)
     out = np.zeros((n, 3))
-    nt = nthreads or os.cpu_count() or 1
+    nt = nthreads or default_threads()
     fn = lib().rdo_accel_ld if long_double else lib().rdo_accel
     fn(ctypes.byref(g), c_int(n), _d(pos), _d(q), _d(m), _d(out), c_int(nt))
     return out
@@ -118,7 +125,7 @@ def field(g: Geom, pos, q, pts, nthreads=None, long_double=False):
     q = _f64(q)
     pts = _f64(pts).reshape(-1, 3)
     out = np.zeros((pts.shape[0], 3))
-    nt = nthreads or os.cpu_count() or 1
+    nt = nthreads or default_threads()
     fn = lib().rdo_field_ld if long_double else lib().rdo_field
     fn(ctypes.byref(g), c_int(pos.shape[0]), _d(pos), _d(q), c_int(pts.shape[0]), _d(pts),
        _d(out), c_int(nt))

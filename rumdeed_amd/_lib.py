@@ -1,0 +1,111 @@
+"""ctypes binding of librumdeed.so (the C ABI declared in include/rumdeed.h).
+
+There is no fallback: if the library is missing or a call fails, an
+exception is raised with the library's own error message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "librumdeed.so")
+
+RD_OK = 0
+RD_GEOM_PLANAR = 1
+RD_GEOM_TIP = 2
+RD_PREC_FP64 = 0
+RD_PREC_MIXED = 1
+RD_SPECIES_ELEC = 1
+RD_SPECIES_ION = 2
+
+
+class RumdeedError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"librumdeed error {code}: {msg}")
+        self.code = code
+
+
+class Geometry(ctypes.Structure):
+    """rd_geometry (include/rumdeed.h)."""
+
+    _fields_ = [("kind", c_int32), ("image_charge", c_int32), ("n_ic_max", c_int32),
+                ("reserved", c_int32), ("V", c_double), ("d", c_double),
+                ("d_tip", c_double), ("R_base", c_double), ("h_tip", c_double)]
+
+
+_dp = POINTER(c_double)
+_ip = POINTER(c_int32)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "rd_version": (c_char_p, []),
+    "rd_last_error": (c_char_p, []),
+    "rd_device_count": (c_int32, [_ip]),
+    "rd_ctx_create": (c_int32, [c_int32, c_int64, POINTER(c_void_p)]),
+    "rd_ctx_destroy": (c_int32, [c_void_p]),
+    "rd_set_geometry": (c_int32, [c_void_p, POINTER(Geometry)]),
+    "rd_set_time_step": (c_int32, [c_void_p, c_double]),
+    "rd_put_particles": (c_int32, [c_void_p, c_int32, _dp, _dp, _dp, _dp, _ip]),
+    "rd_particle_count": (c_int32, [c_void_p, _ip]),
+    "rd_get_particles": (c_int32, [c_void_p, _dp, _dp, _dp, _dp, _ip]),
+    "rd_accel": (c_int32, [c_void_p, c_int32]),
+    "rd_get_accel": (c_int32, [c_void_p, _dp]),
+    "rd_field_batch": (c_int32, [c_void_p, c_int32, _dp, _dp, c_int32]),
+    "rd_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double, c_int32,
+                                c_int32, c_double, c_int32, _dp]),
+    "rd_time_accel": (c_int32, [c_void_p, c_int32, c_int32, _dp, _dp]),
+    "rd_time_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double,
+                                     c_int32, c_int32, c_double, c_int32, c_int32, _dp, _dp]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load librumdeed.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m rumdeed_amd.build` "
+                "(the HIP library is required; there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(code: int) -> None:
+    if code != RD_OK:
+        msg = lib().rd_last_error()
+        raise RumdeedError(code, msg.decode() if msg else "")
+
+
+def dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a

@@ -1,0 +1,143 @@
+"""Host-side mirror of the RUMDEED hot-path interface over librumdeed.so.
+
+The method names follow the reference routines they stand in for
+(mod_verlet / mod_pair / the emission modules); every call goes through the
+C ABI of the HIP library.  Arrays are numpy (n, 3) float64 in SI units, which
+is the memory layout of the Fortran (1:3, 1:n) particle arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_double, c_int32, c_void_p
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import RD_GEOM_PLANAR, RD_GEOM_TIP, RD_PREC_FP64, RD_PREC_MIXED, check, dptr, f64, iptr
+
+PRECISIONS = {"fp64": RD_PREC_FP64, "mixed": RD_PREC_MIXED}
+
+
+def _prec(p):
+    if isinstance(p, str):
+        return PRECISIONS[p]
+    return int(p)
+
+
+class Diode:
+    """One simulation context: particle state resident in HBM plus geometry."""
+
+    def __init__(self, max_particles: int, device: int = 0):
+        self._lib = L.lib()
+        h = c_void_p()
+        check(self._lib.rd_ctx_create(device, int(max_particles), ctypes.byref(h)))
+        self._h = h
+        self.max_particles = int(max_particles)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.rd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- set-up (Read_Input_Variables, Init_Field_Emission_v2, Init_Emission_Tip)
+    def set_planar(self, V: float, d: float, image_charge: bool = True, n_ic_max: int = 0):
+        g = L.Geometry(kind=RD_GEOM_PLANAR, image_charge=int(bool(image_charge)),
+                       n_ic_max=int(n_ic_max), V=float(V), d=float(d))
+        check(self._lib.rd_set_geometry(self._h, ctypes.byref(g)))
+
+    def set_tip(self, V: float, d_tip: float, R_base: float, h_tip: float,
+                image_charge: bool = True):
+        g = L.Geometry(kind=RD_GEOM_TIP, image_charge=int(bool(image_charge)), V=float(V),
+                       d_tip=float(d_tip), R_base=float(R_base), h_tip=float(h_tip))
+        check(self._lib.rd_set_geometry(self._h, ctypes.byref(g)))
+
+    def set_time_step(self, dt: float):
+        check(self._lib.rd_set_time_step(self._h, c_double(dt)))
+
+    # ---- particle store (mod_pair)
+    def put_particles(self, pos, q, m, vel=None, species=None):
+        pos = f64(pos).reshape(-1, 3)
+        n = pos.shape[0]
+        q = f64(q).reshape(n)
+        m = f64(m).reshape(n)
+        vel = None if vel is None else f64(vel).reshape(n, 3)
+        sp = None if species is None else np.ascontiguousarray(species, dtype=np.int32)
+        check(self._lib.rd_put_particles(self._h, n, dptr(pos), dptr(vel), dptr(q), dptr(m),
+                                         iptr(sp)))
+
+    @property
+    def n(self) -> int:
+        v = c_int32()
+        check(self._lib.rd_particle_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    def get_particles(self):
+        n = self.n
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        q = np.zeros(n)
+        m = np.zeros(n)
+        sp = np.zeros(n, dtype=np.int32)
+        check(self._lib.rd_get_particles(self._h, dptr(pos), dptr(vel), dptr(q), dptr(m),
+                                         iptr(sp)))
+        return dict(pos=pos, vel=vel, q=q, m=m, species=sp)
+
+    # ---- forces and fields (mod_verlet)
+    def calculate_acceleration_particles(self, precision="fp64"):
+        """Calculate_Acceleration_Particles: accelerations (n,3)."""
+        check(self._lib.rd_accel(self._h, _prec(precision)))
+        out = np.zeros((self.n, 3))
+        check(self._lib.rd_get_accel(self._h, dptr(out)))
+        return out
+
+    def calc_field_at_batch(self, pts, precision="fp64"):
+        """Calc_Field_at_Batch: field (M,3) at points (M,3)."""
+        pts = f64(pts).reshape(-1, 3)
+        out = np.zeros_like(pts)
+        check(self._lib.rd_field_batch(self._h, pts.shape[0], dptr(pts), dptr(out),
+                                       _prec(precision)))
+        return out
+
+    def calc_field_at(self, pos, precision="fp64"):
+        """Calc_Field_at: field (3,) at one point."""
+        return self.calc_field_at_batch(np.asarray(pos, dtype=np.float64).reshape(1, 3),
+                                        precision)[0]
+
+    def field_grid(self, x0, y0, lx, ly, nx, ny, z=0.0, precision="fp64", copy=True):
+        out = np.zeros((nx * ny, 3)) if copy else None
+        check(self._lib.rd_field_grid(self._h, x0, y0, lx, ly, nx, ny, z, _prec(precision),
+                                      dptr(out)))
+        return out
+
+    # ---- measurement
+    def time_accel(self, precision="fp64", reps=10):
+        tot = c_double()
+        ker = c_double()
+        check(self._lib.rd_time_accel(self._h, _prec(precision), reps, ctypes.byref(tot),
+                                      ctypes.byref(ker)))
+        return tot.value, ker.value
+
+    def time_field_grid(self, x0, y0, lx, ly, nx, ny, z=0.0, precision="fp64", reps=10):
+        tot = c_double()
+        ker = c_double()
+        check(self._lib.rd_time_field_grid(self._h, x0, y0, lx, ly, nx, ny, z, _prec(precision),
+                                           reps, ctypes.byref(tot), ctypes.byref(ker)))
+        return tot.value, ker.value
+
+
+def device_count() -> int:
+    v = c_int32()
+    code = L.lib().rd_device_count(ctypes.byref(v))
+    return v.value if code == 0 else 0

@@ -1,0 +1,52 @@
+"""Synthetic inputs of the force/field snapshot workload.
+
+Snapshot: N electrons at positions uniform in a 200 x 200 x 1000 nm box above
+a planar cathode (x, y in [-100, 100) nm, z in (0, 1000] nm), zero velocity,
+charge -q_0, mass m_0; gap d = 1000 nm; the surface field is evaluated on the
+256 x 256 cell centres of the 200 x 200 nm patch at z = 0.  Positions come
+from numpy's Philox4x64-10 bit generator with the stated seed.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+Q_0 = 1.602176634e-19
+M_0 = 9.1093837015e-31
+NM = 1.0e-9
+
+SNAPSHOT_D = 1000.0 * NM
+SNAPSHOT_V = 2000.0          # the vacuum term only adds q E_z / m
+SNAPSHOT_BOX = (-100.0 * NM, -100.0 * NM, 200.0 * NM, 200.0 * NM)  # x0, y0, lx, ly
+SNAPSHOT_GRID = 256
+
+
+def snapshot(n: int, seed: int = 2):
+    """Positions (n,3) [m], charges, masses of the force/field snapshot."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    x = rng.uniform(-100.0, 100.0, n)
+    y = rng.uniform(-100.0, 100.0, n)
+    z = 1000.0 - rng.uniform(0.0, 1000.0, n)   # (0, 1000]
+    pos = np.stack([x, y, z], axis=1) * NM
+    q = np.full(n, -Q_0)
+    m = np.full(n, M_0)
+    return pos, q, m
+
+
+def grid_points(nx=SNAPSHOT_GRID, ny=SNAPSHOT_GRID, box=SNAPSHOT_BOX, z=0.0):
+    """Cell centres of the surface grid, x fastest (same order as rd_field_grid)."""
+    x0, y0, lx, ly = box
+    ix = np.arange(nx)
+    iy = np.arange(ny)
+    X = x0 + (ix + 0.5) * (lx / nx)
+    Y = y0 + (iy + 0.5) * (ly / ny)
+    gx, gy = np.meshgrid(X, Y)  # gy rows -> y slowest
+    return np.stack([gx.ravel(), gy.ravel(), np.full(nx * ny, z)], axis=1)
+
+
+def pair_flops(n_ic_max: int) -> int:
+    """Algorithmic flops of one planar pair interaction (FMA = 2, sqrt = 1,
+    divide = 1), counted from the reference op sequence
+    (src/mod_verlet.F90:1292-1330 and src/acc_ic_planar_series.inc):
+    Coulomb 21, each image partner 15, image accumulation 6."""
+    P = 1 + 4 * n_ic_max
+    return 21 + 15 * P + 6

@@ -1,0 +1,150 @@
+"""HIP force/field kernels against the oracle and the reference's golden
+vectors.  FP64 kernels must be bit-identical to the oracle (same summation
+order); mixed-precision kernels must stay within 1e-5 relative error of the
+long-double oracle."""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+NM = 1e-9
+MIXED_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def Diode():
+    from rumdeed_amd.diode import Diode
+
+    return Diode
+
+
+def _diode_planar(Diode, pos, q, m, d, V, ic, nic, species=None):
+    dd = Diode(max(len(pos), 1) + 16)
+    dd.set_planar(V, d, ic, nic)
+    dd.put_particles(pos, q, m, species=species)
+    return dd
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 1000, 3000])
+@pytest.mark.parametrize("ic,nic", [(0, 0), (1, 0), (1, 1), (1, 10)])
+def test_planar_accel_fp64_bit_exact(Diode, oracle_mod, n, ic, nic):
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(n, seed=11 + n)
+    print(f"planar fp64 n={n} ic={ic} nic={nic}", flush=True)
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, ic, nic)
+    a_gpu = dd.calculate_acceleration_particles("fp64")
+    g = oracle_mod.geom_planar(W.SNAPSHOT_D, W.SNAPSHOT_V, bool(ic), nic)
+    a_cpu = oracle_mod.accel(g, pos, q, m)
+    assert np.array_equal(a_gpu, a_cpu)
+
+
+@pytest.mark.parametrize("nic", [0, 1, 10])
+def test_planar_golden_snapshot(Diode, nic):
+    s = golden("snapshot512.npz")
+    dd = _diode_planar(Diode, s["pos"], s["q"], s["m"], float(s["d"]), float(s["V"]), 1, nic)
+    a = dd.calculate_acceleration_particles("fp64")
+    assert rel_err(a, s[f"acc_n{nic}"]).max() < 1e-12
+    f = dd.calc_field_at_batch(s["grid"], "fp64")
+    assert rel_err(f, s[f"field_n{nic}"]).max() < 1e-12
+    am = dd.calculate_acceleration_particles("mixed")
+    assert rel_err(am, s[f"acc_n{nic}"]).max() < MIXED_TOL
+    fm = dd.calc_field_at_batch(s["grid"], "mixed")
+    assert rel_err(fm, s[f"field_n{nic}"]).max() < MIXED_TOL
+
+
+def test_forty_and_three_body_golden(Diode):
+    f = golden("forty.npz")
+    for ic, nic in [(0, 0), (1, 0), (1, 2), (1, 10)]:
+        dd = _diode_planar(Diode, f["pos"], f["q"], f["m"], float(f["d"]), float(f["V"]), ic, nic,
+                           species=f["species"])
+        a = dd.calculate_acceleration_particles("fp64")
+        assert rel_err(a, f[f"acc_gather_ic{ic}_n{nic}"]).max() < 1e-12
+        am = dd.calculate_acceleration_particles("mixed")
+        assert rel_err(am, f[f"acc_gather_ic{ic}_n{nic}"]).max() < MIXED_TOL
+        fl = dd.calc_field_at_batch(f["pts"], "fp64")
+        assert rel_err(fl, f[f"field_ic{ic}_n{nic}"]).max() < 1e-12
+    t = golden("three_body.npz")
+    dd = _diode_planar(Diode, t["pos"], t["q"], t["m"], float(t["d"]), float(t["V"]), 0, 0,
+                       species=t["species"])
+    assert rel_err(dd.calculate_acceleration_particles("fp64"), t["acc"]).max() < 1e-12
+    assert rel_err(dd.calc_field_at(t["probe"]), t["field"]).max() < 1e-12
+
+
+def test_tip_fp64_bit_exact_and_golden(Diode, oracle_mod):
+    t = golden("tip64.npz")
+    dd = Diode(128)
+    dd.set_tip(float(t["V"]), float(t["d_tip"]), float(t["R_base"]), float(t["h_tip"]))
+    dd.put_particles(t["pos"], t["q"], t["m"])
+    a = dd.calculate_acceleration_particles("fp64")
+    g = oracle_mod.geom_tip(float(t["V"]), float(t["d_tip"]), float(t["R_base"]), float(t["h_tip"]))
+    assert np.array_equal(a, oracle_mod.accel(g, t["pos"], t["q"], t["m"]))
+    assert rel_err(a, t["acc_gather"]).max() < 1e-12
+    f = dd.calc_field_at_batch(t["pts"], "fp64")
+    assert np.array_equal(f, oracle_mod.field(g, t["pos"], t["q"], t["pts"]))
+    assert rel_err(f, t["field"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [0, 1, 300, 2500])
+@pytest.mark.parametrize("nic", [0, 10])
+def test_field_fp64_bit_exact(Diode, oracle_mod, n, nic):
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(max(n, 1), seed=5)
+    pos, q, m = pos[:n], q[:n], m[:n]
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, 1, nic)
+    pts = W.grid_points(24, 20)
+    f = dd.calc_field_at_batch(pts, "fp64")
+    g = oracle_mod.geom_planar(W.SNAPSHOT_D, W.SNAPSHOT_V, True, nic)
+    assert np.array_equal(f, oracle_mod.field(g, pos, q, pts))
+    if n == 0:
+        assert np.all(f[:, :2] == 0) and np.all(f[:, 2] == -W.SNAPSHOT_V / W.SNAPSHOT_D)
+    grid = dd.field_grid(*W.SNAPSHOT_BOX, 24, 20, 0.0, "fp64")
+    assert np.array_equal(grid, f)
+
+
+@pytest.mark.parametrize("n", [4096])
+@pytest.mark.parametrize("nic", [0, 1, 10])
+def test_mixed_error_vs_long_double(Diode, oracle_mod, n, nic):
+    """Mixed precision against the long-double oracle on the force/field
+    snapshot: per-particle relative error <= 1e-5."""
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(n, seed=2)
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, 1, nic)
+    g = oracle_mod.geom_planar(W.SNAPSHOT_D, W.SNAPSHOT_V, True, nic)
+    ref = oracle_mod.accel(g, pos, q, m, long_double=True)
+    am = dd.calculate_acceleration_particles("mixed")
+    e = rel_err(am, ref)
+    print(f"mixed accel n={n} nic={nic}: max {e.max():.2e} p99 {np.percentile(e, 99):.2e}")
+    assert e.max() < MIXED_TOL
+    a64 = dd.calculate_acceleration_particles("fp64")
+    assert rel_err(a64, ref).max() < 1e-12
+    pts = W.grid_points(32, 32)
+    fref = oracle_mod.field(g, pos, q, pts, long_double=True)
+    fm = dd.calc_field_at_batch(pts, "mixed")
+    ef = rel_err(fm, fref)
+    print(f"mixed field n={n} nic={nic}: max {ef.max():.2e}")
+    assert ef.max() < MIXED_TOL
+
+
+def test_absorbed_particles_have_no_effect(Diode, oracle_mod):
+    """Particles with charge 0 (marked for removal, src/mod_pair.F90:206-207)
+    contribute nothing but still receive their vacuum-field acceleration."""
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(600, seed=3)
+    q = q.copy()
+    q[::3] = 0.0
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, 1, 1)
+    a = dd.calculate_acceleration_particles("fp64")
+    g = oracle_mod.geom_planar(W.SNAPSHOT_D, W.SNAPSHOT_V, True, 1)
+    assert np.array_equal(a, oracle_mod.accel(g, pos, q, m))
+    assert np.all(a[::3] == 0.0)  # q_i = 0 -> no force at all
+    keep = np.ones(600, bool)
+    keep[::3] = False
+    am = dd.calculate_acceleration_particles("mixed")
+    ref = oracle_mod.accel(g, pos[keep], q[keep], m[keep], long_double=True)
+    assert rel_err(am[keep], ref).max() < MIXED_TOL
