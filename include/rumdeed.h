@@ -58,7 +58,40 @@ typedef struct {
   double V;             /* V_s (V) */
   double d;             /* planar gap (m); ignored for the tip */
   double d_tip, R_base, h_tip; /* tip (m); ignored for planar */
+  double box_top;       /* removal height box_dim(3) (m); 0 = the gap (d, or d_tip + h_tip) */
 } rd_geometry;
+
+/* Emission models (EMISSION_MODE, src/mod_global.F90:292-307). */
+#define RD_EMIT_NONE 0
+#define RD_EMIT_TIP_FE 3      /* mod_emission_tip, EMITTERS_TYPE = 1 */
+#define RD_EMIT_PLANAR_GTF 9  /* mod_field_thermo_emission */
+#define RD_EMIT_PLANAR_FE 10  /* mod_field_emission_v2, rectangle */
+
+/* One emitter (emitters_pos/dim/delay, the `work` file, T_temp). */
+typedef struct {
+  int32_t mode;           /* RD_EMIT_* */
+  int32_t delay;          /* emits on steps with delay < step */
+  double x0, y0;          /* emitters_pos(1:2,1) (m) */
+  double lx, ly;          /* emitters_dim(1:2,1) (m) */
+  double T;               /* T_temp (K) of the cathode, thermal-field emission */
+  double w_tip;           /* tip work function (eV) */
+  int32_t wf_nx, wf_ny;   /* checkerboard work function; 1 x 1 = constant */
+  const double* wf_table; /* wf_ny rows of wf_nx values, first row = top, as in `work` */
+  int32_t quad_n;         /* supply-integral points per side, planar (0 = 32) */
+  int32_t ndim;           /* Metropolis-Hastings jumps per chain (0 = 200 FE, 25 GTF, 80 tip) */
+} rd_emitter;
+
+/* Per-step record of one simulation (the figures of emitted.dt, absorbed.dt,
+ * ramo_current.dt, field.dt: src/mod_pair.F90:790-837, mod_field_emission_v2.F90:389). */
+typedef struct {
+  int32_t step, n, emitted, n_round;
+  int32_t absorbed, absorbed_top, absorbed_bot, dropped;
+  int32_t start_rounds, reserved0, reserved1, reserved2;
+  double n_sup, ramo_current, mh_std, a_rate, df_avg;
+  double f_avg[3];
+  double ramo[3];
+  double avg_elec_vel[3];
+} rd_step_record;
 
 /* ---- library and context ------------------------------------------------ */
 const char* rd_version(void);
@@ -102,6 +135,66 @@ int rd_field_batch(rd_ctx* ctx, int32_t m, const double* pts, double* field, int
  * when it is not NULL.  The surface grid of the force/field snapshot. */
 int rd_field_grid(rd_ctx* ctx, double x0, double y0, double lx, double ly, int32_t nx,
                   int32_t ny, double z, int32_t precision, double* field);
+
+/* ---- time step (the main loop, src/main.F90:175-219) ---------------------- */
+/* Emitter of the current simulation (Init_Field_Emission_v2 / Init_Emission_Tip /
+ * Init_Field_Thermo_Emission with Read_work_function). */
+int rd_set_emitter(rd_ctx* ctx, const rd_emitter* em);
+/* Seed of the counter-based generator: every random draw of the simulation
+ * is a function of (seed, stream, step, purpose, chain), so results do not
+ * depend on what else runs on the device. */
+int rd_set_seed(rd_ctx* ctx, uint64_t seed, uint64_t stream);
+
+/* Add_Particle (src/mod_pair.F90:29-159) for n particles at once: appends
+ * them in order with the Beeman history seeded from the vacuum field.
+ * pos, vel: (3,n); species: (n) or NULL for electrons; sec: (n) or NULL (1). */
+int rd_add_particles(rd_ctx* ctx, int32_t n, const double* pos, const double* vel,
+                     const int32_t* species, int32_t step, int32_t emit, const int32_t* sec);
+
+/* Update_Position (src/mod_verlet.F90:115-162): Beeman position update with
+ * boundary marking (Update_ElecIon_Position, :197-232), accelerations, Beeman
+ * velocity update and Ramo current (Update_ElecIon_Velocity, :449-509). */
+int rd_update_position(rd_ctx* ctx, int32_t step, int32_t precision, rd_step_record* rec);
+/* Remove_Particles (src/mod_pair.F90:352-562): order-preserving compaction of
+ * the particles marked in the last position update, lifetime histogram. */
+int rd_remove_particles(rd_ctx* ctx, int32_t step, rd_step_record* rec);
+/* ptr_Do_Emission: surface supply integral and Metropolis-Hastings site
+ * sampling of the current emitter, then the escape test and Add_Particle
+ * (Do_Field_Emission_Planar_rectangle, src/mod_field_emission_v2.F90:261-395;
+ * Do_Field_Thermo_Emission_Planar_simple, src/mod_field_thermo_emission.F90:136-193;
+ * Do_Field_Emission_Tip_OLDCODE, src/mod_emission_tip.f90:417-534). */
+int rd_do_emission(rd_ctx* ctx, int32_t step, int32_t precision, rd_step_record* rec);
+/* nsteps iterations of the main loop (emission, update, removal) after the
+ * last completed step; records: nsteps entries (may be NULL). */
+int rd_run(rd_ctx* ctx, int32_t nsteps, int32_t precision, rd_step_record* records);
+
+/* Full particle state of the current simulation (any pointer may be NULL).
+ * vec3 arrays (3,n); ints (5,n): species, birth step, emitter, section, id. */
+int rd_get_state(rd_ctx* ctx, double* pos, double* prev_pos, double* vel, double* acc,
+                 double* acc_prev, double* acc_prev2, int32_t* ints);
+/* life_time histogram (src/mod_pair.F90:1135-1160): hist (1000, 2) for
+ * electrons then ions, lifetime in steps 1..1000. */
+int rd_get_life_time(rd_ctx* ctx, int32_t* hist);
+
+/* ---- many simulations on one device (batched serving) --------------------- */
+/* A pool: one particle store of total_particles shared by up to max_slots
+ * independent simulations; every step of rd_pool_step advances all open
+ * simulations with one launch per kernel.  rd_ctx_create is a pool of one
+ * slot that spans the whole store. */
+int rd_pool_create(int32_t device, int64_t total_particles, int32_t max_slots, rd_ctx** out);
+/* HBM bytes a simulation of `capacity` particles holds in a pool. */
+int rd_slot_footprint(int64_t capacity, int64_t* bytes);
+/* Open a simulation with room for `capacity` particles if a contiguous range
+ * of the store is free (RD_ERR_CAPACITY otherwise); it becomes current. */
+int rd_slot_open(rd_ctx* ctx, int64_t capacity, int32_t* slot);
+int rd_slot_close(rd_ctx* ctx, int32_t slot);
+/* Make `slot` the simulation the single-simulation entry points act on. */
+int rd_slot_select(rd_ctx* ctx, int32_t slot);
+/* Last completed step of a simulation. */
+int rd_slot_step(rd_ctx* ctx, int32_t slot, int32_t* step);
+/* One main-loop step of every open simulation; records: one per slot index
+ * (max_slots entries, may be NULL); slots that are not open are left untouched. */
+int rd_pool_step(rd_ctx* ctx, int32_t precision, rd_step_record* records);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* Average device time (ms) of the accelerations / grid field over `reps`

@@ -279,3 +279,170 @@ class Ref:
         sec = np.zeros(pts.shape[0], dtype=np.int32)
         self.L.ref_work_at(c_int(pts.shape[0]), _d(pts), _d(w), _i(sec))
         return w, sec
+
+
+# ---------------------------------------------------------------------------
+# One simulation of the CPU restatement (rd_oracle_emit.c)
+class Record(ctypes.Structure):
+    """Per-step record; same layout as rd_step_record (include/rumdeed.h)."""
+
+    _fields_ = [("step", c_int32), ("n", c_int32), ("emitted", c_int32), ("n_round", c_int32),
+                ("absorbed", c_int32), ("absorbed_top", c_int32), ("absorbed_bot", c_int32),
+                ("dropped", c_int32), ("start_rounds", c_int32), ("r0", c_int32),
+                ("r1", c_int32), ("r2", c_int32),
+                ("n_sup", c_double), ("ramo_current", c_double), ("mh_std", c_double),
+                ("a_rate", c_double), ("df_avg", c_double), ("f_avg", c_double * 3),
+                ("ramo", c_double * 3), ("avg_elec_vel", c_double * 3)]
+
+
+def record_dict(r):
+    d = {k: getattr(r, k) for k, _ in Record._fields_ if not k.startswith("r") or k in ("ramo", "ramo_current")}
+    for k in ("f_avg", "ramo", "avg_elec_vel"):
+        d[k] = list(getattr(r, k))
+    return d
+
+
+class Sim:
+    """CPU restatement of one simulation (emission, Beeman step, removal)."""
+
+    def __init__(self, cap, nthreads=None):
+        L = lib()
+        L.rdo_sim_create.restype = ctypes.c_void_p
+        L.rdo_sim_create.argtypes = [c_int, c_int]
+        for name in ("rdo_sim_destroy", "rdo_sim_set_geometry", "rdo_sim_set_time_step",
+                     "rdo_sim_set_seed", "rdo_sim_set_emitter", "rdo_sim_add", "rdo_sim_put",
+                     "rdo_sim_step", "rdo_sim_get", "rdo_sim_life", "rdo_sim_count"):
+            getattr(L, name).restype = None
+        L.rdo_sim_count.restype = c_int
+        L.rdo_sim_set_time_step.argtypes = [ctypes.c_void_p, c_double]
+        L.rdo_sim_set_seed.argtypes = [ctypes.c_void_p, c_uint64, c_uint64]
+        L.rdo_sim_set_geometry.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_double]
+        L.rdo_sim_set_emitter.argtypes = [ctypes.c_void_p, c_int, c_int, c_double, c_double,
+                                          c_double, c_double, c_double, c_double, c_int, c_int,
+                                          _dp, c_int, c_int]
+        L.rdo_sim_step.argtypes = [ctypes.c_void_p, c_int, c_int, ctypes.c_void_p]
+        L.rdo_sim_count.argtypes = [ctypes.c_void_p]
+        self.L = L
+        self.h = L.rdo_sim_create(int(cap), int(nthreads or default_threads()))
+        self.cap = cap
+        self.step_done = 0
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.rdo_sim_destroy(ctypes.c_void_p(self.h))
+                self.h = None
+        except Exception:
+            pass
+
+    def set_geometry(self, g: Geom, box_top=0.0):
+        self.L.rdo_sim_set_geometry(ctypes.c_void_p(self.h), ctypes.byref(g), c_double(box_top))
+
+    def set_time_step(self, dt):
+        self.L.rdo_sim_set_time_step(ctypes.c_void_p(self.h), dt)
+
+    def set_seed(self, seed, stream=0):
+        self.L.rdo_sim_set_seed(ctypes.c_void_p(self.h), int(seed), int(stream))
+
+    def set_emitter(self, mode, delay=0, x0=0.0, y0=0.0, lx=0.0, ly=0.0, T=300.0, w_tip=4.7,
+                    wf=None, quad=0, ndim=0):
+        wf = np.array([[4.7]]) if wf is None else np.atleast_2d(np.asarray(wf, dtype=np.float64))
+        ny, nx = wf.shape
+        tab = np.ascontiguousarray(wf.ravel())
+        self._wf = tab
+        self.L.rdo_sim_set_emitter(ctypes.c_void_p(self.h), mode, delay, x0, y0, lx, ly, T, w_tip,
+                                   nx, ny, _d(tab), quad, ndim)
+
+    def put(self, pos, q, m, vel=None, species=None):
+        pos = _f64(pos).reshape(-1, 3)
+        n = pos.shape[0]
+        sp = None if species is None else np.ascontiguousarray(species, np.int32)
+        v = None if vel is None else _f64(vel).reshape(n, 3)
+        self.L.rdo_sim_put(ctypes.c_void_p(self.h), c_int(n), _d(pos),
+                           None if v is None else _d(v), _d(_f64(q)), _d(_f64(m)),
+                           None if sp is None else _i(sp))
+
+    def add(self, pos, vel=None, species=None, step=1, emit=1, sec=None):
+        pos = _f64(pos).reshape(-1, 3)
+        n = pos.shape[0]
+        v = None if vel is None else _f64(vel).reshape(n, 3)
+        sp = None if species is None else np.ascontiguousarray(species, np.int32)
+        sc = None if sec is None else np.ascontiguousarray(sec, np.int32)
+        self.L.rdo_sim_add(ctypes.c_void_p(self.h), c_int(n), _d(pos),
+                           None if v is None else _d(v), None if sp is None else _i(sp),
+                           c_int(step), c_int(emit), None if sc is None else _i(sc))
+
+    def step(self, step=None, phases=7):
+        if step is None:
+            step = self.step_done + 1
+        rec = Record()
+        self.L.rdo_sim_step(ctypes.c_void_p(self.h), c_int(step), c_int(phases), ctypes.byref(rec))
+        if phases & 4 or phases == 7:
+            self.step_done = step
+        return rec
+
+    @property
+    def n(self):
+        return self.L.rdo_sim_count(ctypes.c_void_p(self.h))
+
+    def state(self):
+        n = self.n
+        arrs = [np.zeros((n, 3)) for _ in range(6)]
+        ints = np.zeros((n, 5), dtype=np.int32)
+        q = np.zeros(n)
+        self.L.rdo_sim_get(ctypes.c_void_p(self.h), *[_d(a) for a in arrs], _i(ints), _d(q))
+        keys = ["pos", "prev_pos", "vel", "acc", "acc_prev", "acc_prev2"]
+        st = dict(zip(keys, arrs))
+        st.update(species=ints[:, 0], step=ints[:, 1], emitter=ints[:, 2], section=ints[:, 3],
+                  id=ints[:, 4], q=q)
+        return st
+
+    def life_time(self):
+        h = np.zeros(2000, dtype=np.int32)
+        self.L.rdo_sim_life(ctypes.c_void_p(self.h), _i(h))
+        return h.reshape(2, 1000).T
+
+
+def gtf(F, T, w):
+    L = lib()
+    L.rdo_gtf.restype = c_double
+    L.rdo_gtf.argtypes = [c_double, c_double, c_double]
+    return L.rdo_gtf(F, T, w)
+
+
+def fn_functions(ic, F, w):
+    """(v_y, t_y, Escape_Prob_log, Elec_Supply_log) of the planar FN model."""
+    L = lib()
+    for name in ("rdo_fn_v", "rdo_fn_t", "rdo_escape_log", "rdo_supply_log"):
+        getattr(L, name).restype = c_double
+        getattr(L, name).argtypes = [c_int, c_double, c_double]
+    return (L.rdo_fn_v(ic, F, w), L.rdo_fn_t(ic, F, w), L.rdo_escape_log(ic, F, w),
+            L.rdo_supply_log(ic, F, w))
+
+
+def work_checker(table, x0, y0, lx, ly, pts):
+    L = lib()
+    L.rdo_work_checker.restype = c_double
+    L.rdo_work_checker.argtypes = [_dp, c_int, c_int, c_double, c_double, c_double, c_double,
+                                   c_double, c_double, POINTER(c_int)]
+    tab = np.ascontiguousarray(np.atleast_2d(table), dtype=np.float64)
+    ny, nx = tab.shape
+    pts = _f64(pts).reshape(-1, 3)
+    w = np.zeros(len(pts))
+    sec = np.zeros(len(pts), dtype=np.int32)
+    for k, p in enumerate(pts):
+        s = c_int()
+        w[k] = L.rdo_work_checker(_d(tab.ravel()), nx, ny, x0, y0, lx, ly, p[0], p[1],
+                                  ctypes.byref(s))
+        sec[k] = s.value
+    return w, sec
+
+
+def sincos(x):
+    L = lib()
+    s = c_double()
+    c = c_double()
+    L.rdo_sincos.restype = None
+    L.rdo_sincos.argtypes = [c_double, POINTER(c_double), POINTER(c_double)]
+    L.rdo_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value

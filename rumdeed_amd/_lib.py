@@ -29,16 +29,53 @@ class RumdeedError(RuntimeError):
         self.code = code
 
 
+RD_EMIT_NONE = 0
+RD_EMIT_TIP_FE = 3
+RD_EMIT_PLANAR_GTF = 9
+RD_EMIT_PLANAR_FE = 10
+
+
 class Geometry(ctypes.Structure):
     """rd_geometry (include/rumdeed.h)."""
 
     _fields_ = [("kind", c_int32), ("image_charge", c_int32), ("n_ic_max", c_int32),
                 ("reserved", c_int32), ("V", c_double), ("d", c_double),
-                ("d_tip", c_double), ("R_base", c_double), ("h_tip", c_double)]
+                ("d_tip", c_double), ("R_base", c_double), ("h_tip", c_double),
+                ("box_top", c_double)]
 
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)
+
+
+class Emitter(ctypes.Structure):
+    """rd_emitter (include/rumdeed.h)."""
+
+    _fields_ = [("mode", c_int32), ("delay", c_int32), ("x0", c_double), ("y0", c_double),
+                ("lx", c_double), ("ly", c_double), ("T", c_double), ("w_tip", c_double),
+                ("wf_nx", c_int32), ("wf_ny", c_int32), ("wf_table", _dp),
+                ("quad_n", c_int32), ("ndim", c_int32)]
+
+
+class StepRecord(ctypes.Structure):
+    """rd_step_record (include/rumdeed.h)."""
+
+    _fields_ = [("step", c_int32), ("n", c_int32), ("emitted", c_int32), ("n_round", c_int32),
+                ("absorbed", c_int32), ("absorbed_top", c_int32), ("absorbed_bot", c_int32),
+                ("dropped", c_int32), ("start_rounds", c_int32), ("reserved0", c_int32),
+                ("reserved1", c_int32), ("reserved2", c_int32),
+                ("n_sup", c_double), ("ramo_current", c_double), ("mh_std", c_double),
+                ("a_rate", c_double), ("df_avg", c_double), ("f_avg", c_double * 3),
+                ("ramo", c_double * 3), ("avg_elec_vel", c_double * 3)]
+
+    def as_dict(self):
+        d = {}
+        for k, _ in self._fields_:
+            if k.startswith("reserved"):
+                continue
+            v = getattr(self, k)
+            d[k] = list(v) if k in ("f_avg", "ramo", "avg_elec_vel") else v
+        return d
 
 # name -> (restype, argtypes)
 _SIGS = {
@@ -57,6 +94,22 @@ _SIGS = {
     "rd_field_batch": (c_int32, [c_void_p, c_int32, _dp, _dp, c_int32]),
     "rd_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double, c_int32,
                                 c_int32, c_double, c_int32, _dp]),
+    "rd_set_emitter": (c_int32, [c_void_p, POINTER(Emitter)]),
+    "rd_set_seed": (c_int32, [c_void_p, c_uint64, c_uint64]),
+    "rd_add_particles": (c_int32, [c_void_p, c_int32, _dp, _dp, _ip, c_int32, c_int32, _ip]),
+    "rd_update_position": (c_int32, [c_void_p, c_int32, c_int32, POINTER(StepRecord)]),
+    "rd_remove_particles": (c_int32, [c_void_p, c_int32, POINTER(StepRecord)]),
+    "rd_do_emission": (c_int32, [c_void_p, c_int32, c_int32, POINTER(StepRecord)]),
+    "rd_run": (c_int32, [c_void_p, c_int32, c_int32, POINTER(StepRecord)]),
+    "rd_get_state": (c_int32, [c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "rd_get_life_time": (c_int32, [c_void_p, _ip]),
+    "rd_pool_create": (c_int32, [c_int32, c_int64, c_int32, POINTER(c_void_p)]),
+    "rd_slot_footprint": (c_int32, [c_int64, POINTER(c_int64)]),
+    "rd_slot_open": (c_int32, [c_void_p, c_int64, _ip]),
+    "rd_slot_close": (c_int32, [c_void_p, c_int32]),
+    "rd_slot_select": (c_int32, [c_void_p, c_int32]),
+    "rd_slot_step": (c_int32, [c_void_p, c_int32, _ip]),
+    "rd_pool_step": (c_int32, [c_void_p, c_int32, POINTER(StepRecord)]),
     "rd_time_accel": (c_int32, [c_void_p, c_int32, c_int32, _dp, _dp]),
     "rd_time_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double,
                                      c_int32, c_int32, c_double, c_int32, c_int32, _dp, _dp]),

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "librumdeed.so")
 
-SOURCES = ["rd_api.hip", "rd_pair.hip"]
+SOURCES = ["rd_api.hip", "rd_pair.hip", "rd_engine.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -1,7 +1,9 @@
-// C ABI of librumdeed.so: contexts, particle store, force and field entry
-// points, timing.  See include/rumdeed.h for the contract of each function.
+// C ABI of librumdeed.so: pools and simulation slots, particle store, force
+// and field entry points, the time step, timing.  See include/rumdeed.h for
+// the contract of each function.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -77,8 +79,12 @@ __global__ void k_fill_i(int n, int32_t v, int32_t* __restrict__ a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
+__global__ void k_iota(int n, int32_t v0, int32_t* __restrict__ a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v0 + i;
+}
 
-int upload_vec3(rd_ctx* c, int n, const double* host, double* x, double* y, double* z) {
+static int upload_vec3(rd_ctx* c, int n, const double* host, double* x, double* y, double* z) {
   if (n == 0) return RD_OK;
   int r = ensure(c, c->stage, sizeof(double) * 3 * (size_t)n);
   if (r) return r;
@@ -86,11 +92,12 @@ int upload_vec3(rd_ctx* c, int n, const double* host, double* x, double* y, doub
                         c->stream));
   k_aos_to_soa<<<(n + 255) / 256, 256, 0, c->stream>>>(n, (const double*)c->stage.p, x, y, z);
   RD_HIP(hipGetLastError());
+  RD_HIP(hipStreamSynchronize(c->stream));
   return RD_OK;
 }
 
-int download_vec3(rd_ctx* c, int n, const double* x, const double* y, const double* z,
-                  double* host) {
+static int download_vec3(rd_ctx* c, int n, const double* x, const double* y, const double* z,
+                         double* host) {
   if (n == 0) return RD_OK;
   int r = ensure(c, c->stage, sizeof(double) * 3 * (size_t)n);
   if (r) return r;
@@ -148,17 +155,116 @@ int build_geom(const rd_geometry* in, DevGeom* g) {
   } else {
     return fail(RD_ERR_ARG, "unknown geometry kind");
   }
-  g->d_nm = (float)(g->d * 1e9);
+  g->box_top = in->box_top > 0.0 ? in->box_top : g->d;
   return RD_OK;
 }
+
+static int put_desc(rd_ctx* c, int s) {
+  RD_HIP(hipMemcpyAsync(c->d_desc + s, &c->slots[s].desc, sizeof(SimDesc), hipMemcpyHostToDevice,
+                        c->stream));
+  RD_HIP(hipStreamSynchronize(c->stream));
+  return RD_OK;
+}
+
+// Write the host copy of one slot's state to the device (after a sync).
+static int put_state(rd_ctx* c, int s) {
+  RD_HIP(hipMemcpyAsync(c->d_state + s, c->h_state + s, sizeof(SimState), hipMemcpyHostToDevice,
+                        c->stream));
+  RD_HIP(hipStreamSynchronize(c->stream));
+  return RD_OK;
+}
+
+static void default_state(SimState* S) {
+  std::memset(S, 0, sizeof(*S));
+  S->MH_std = 0.0125;  // mod_field_emission_v2.F90:56
+  S->a_rate = 1.0;
+}
+
+static int slot_range_free(rd_ctx* c, int64_t cap, int* base) {
+  // first fit over the ranges of the open slots, sorted by base
+  std::vector<std::pair<int, int>> used;
+  for (const Slot& s : c->slots)
+    if (s.active) used.push_back({s.desc.base, s.desc.cap});
+  std::sort(used.begin(), used.end());
+  int64_t pos = 0;
+  for (auto& u : used) {
+    if (u.first - pos >= cap) break;
+    pos = std::max<int64_t>(pos, (int64_t)u.first + u.second);
+  }
+  if (pos + cap > c->cap) return RD_ERR_CAPACITY;
+  *base = (int)pos;
+  return RD_OK;
+}
+
+static int open_slot(rd_ctx* c, int64_t cap, int32_t* out) {
+  if (cap < 1) return fail(RD_ERR_ARG, "slot capacity must be >= 1");
+  int idx = -1;
+  for (int s = 0; s < c->nslots_max; ++s)
+    if (!c->slots[s].active) {
+      idx = s;
+      break;
+    }
+  if (idx < 0) return fail(RD_ERR_CAPACITY, "all simulation slots are in use");
+  int base = 0;
+  if (slot_range_free(c, cap, &base))
+    return fail(RD_ERR_CAPACITY, "no contiguous free range of the particle store");
+  int r = sync_state(c);
+  if (r) return r;
+  Slot& sl = c->slots[idx];
+  sl = Slot();
+  sl.active = true;
+  std::memset(&sl.desc, 0, sizeof(SimDesc));
+  sl.desc.base = base;
+  sl.desc.cap = (int)cap;
+  sl.desc.quad = 32;
+  sl.desc.ndim = 200;
+  sl.desc.w_tip = 4.7;
+  sl.desc.wf_nx = 0;
+  sl.n = 0;
+  sl.step = 0;
+  default_state(&c->h_state[idx]);
+  r = put_state(c, idx);
+  if (!r) r = put_desc(c, idx);
+  if (r) return r;
+  // zero the lifetime histogram of the slot
+  int32_t* life = c->d_life + (size_t)idx * 2 * kLife;
+  k_fill_i<<<(2 * kLife + 255) / 256, 256, 0, c->stream>>>(2 * kLife, 0, life);
+  RD_HIP(hipStreamSynchronize(c->stream));
+  c->cur = idx;
+  *out = idx;
+  return RD_OK;
+}
+
+static rd::Slot* cur_slot(rd_ctx* c) {
+  if (c->cur < 0 || c->cur >= c->nslots_max || !c->slots[c->cur].active) return nullptr;
+  return &c->slots[c->cur];
+}
+
+static Ent slot_ent(rd_ctx* c, int s) {
+  Ent e{};
+  e.slot = s;
+  e.base = c->slots[s].desc.base;
+  e.n_ub = c->slots[s].n;
+  e.step = c->slots[s].step;
+  return e;
+}
+
+int engine_phases(rd_ctx* c, const std::vector<int>& slots, int precision, int phases,
+                  const int* steps, rd_step_record* recs);
 
 }  // namespace rd
 
 using namespace rd;
 
+#define RD_CUR(c)                                                        \
+  if (!(c)) return fail(RD_ERR_ARG, "null context");                     \
+  rd::Slot* sl = cur_slot(c);                                            \
+  if (!sl) return fail(RD_ERR_STATE, "no open simulation slot selected"); \
+  RD_HIP(hipSetDevice((c)->device))
+
 extern "C" {
 
-const char* rd_version(void) { return "rumdeed-mi355x 0.1 (gfx950)"; }
+const char* rd_version(void) { return "rumdeed-mi355x 0.2 (gfx950)"; }
 const char* rd_last_error(void) { return g_err.c_str(); }
 
 int rd_device_count(int32_t* n) {
@@ -173,11 +279,21 @@ int rd_device_count(int32_t* n) {
   return RD_OK;
 }
 
-int rd_ctx_create(int32_t device, int64_t max_particles, rd_ctx** out) {
+int rd_slot_footprint(int64_t capacity, int64_t* bytes) {
+  if (!bytes || capacity < 0) return fail(RD_ERR_ARG, "bad argument");
+  // 20 FP64 + 6 int32 per particle, packed copies (32 B), tip image data
+  // (32 B), and the FP64 chunk partials 3 * ceil(n/256) * n * 8 B.
+  int64_t per = 20 * 8 + 6 * 4 + 32 + 32;
+  int64_t nch = (capacity + kChunk - 1) / kChunk;
+  *bytes = capacity * per + 3 * nch * capacity * 8 + (int64_t)sizeof(SimState) + 2 * kLife * 4;
+  return RD_OK;
+}
+
+int rd_pool_create(int32_t device, int64_t total, int32_t max_slots, rd_ctx** out) {
   if (!out) return fail(RD_ERR_ARG, "null output pointer");
   *out = nullptr;
-  if (max_particles < 1 || max_particles > (int64_t)1 << 30)
-    return fail(RD_ERR_ARG, "max_particles outside 1..2^30");
+  if (total < 1 || total > (int64_t)1 << 30) return fail(RD_ERR_ARG, "max_particles outside 1..2^30");
+  if (max_slots < 1 || max_slots > kMaxSlots) return fail(RD_ERR_ARG, "max_slots outside 1..4096");
   int nd = 0;
   hipError_t e = hipGetDeviceCount(&nd);
   if (e != hipSuccess || nd == 0) return fail(RD_ERR_HIP, "no HIP device available");
@@ -185,17 +301,19 @@ int rd_ctx_create(int32_t device, int64_t max_particles, rd_ctx** out) {
   RD_HIP(hipSetDevice(device));
   rd_ctx* c = new rd_ctx();
   c->device = device;
-  c->cap = max_particles;
+  c->cap = total;
+  c->nslots_max = max_slots;
+  c->slots.assign(max_slots, Slot());
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "hipStreamCreate");
   }
-  double** d_arrays[] = {&c->x, &c->y, &c->z, &c->px, &c->py, &c->pz, &c->vx, &c->vy,
-                         &c->vz, &c->ax, &c->ay, &c->az, &c->a1x, &c->a1y, &c->a1z, &c->a2x,
-                         &c->a2y, &c->a2z, &c->q, &c->m};
-  int32_t** i_arrays[] = {&c->species, &c->step, &c->emitter, &c->section, &c->id};
-  size_t cap = (size_t)max_particles;
+  double** d_arrays[] = {&c->x,   &c->y,   &c->z,   &c->px,  &c->py,  &c->pz,  &c->vx,
+                         &c->vy,  &c->vz,  &c->ax,  &c->ay,  &c->az,  &c->a1x, &c->a1y,
+                         &c->a1z, &c->a2x, &c->a2y, &c->a2z, &c->q,   &c->m};
+  int32_t** i_arrays[] = {&c->species, &c->pstep, &c->emitter, &c->section, &c->id, &c->alive};
+  size_t cap = (size_t)total;
   for (double** p : d_arrays) {
     e = hipMalloc((void**)p, sizeof(double) * cap);
     if (e != hipSuccess) {
@@ -210,27 +328,55 @@ int rd_ctx_create(int32_t device, int64_t max_particles, rd_ctx** out) {
       return hip_fail(e, "hipMalloc(particle store)");
     }
   }
+  e = hipMalloc((void**)&c->d_desc, sizeof(SimDesc) * max_slots);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_state, sizeof(SimState) * max_slots);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_state, sizeof(SimState) * max_slots);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_life, sizeof(int32_t) * 2 * kLife * max_slots);
+  if (e != hipSuccess) {
+    rd_ctx_destroy(c);
+    return hip_fail(e, "hipMalloc(slot state)");
+  }
+  std::memset(c->h_state, 0, sizeof(SimState) * max_slots);
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
   hipEventCreate(&c->ev2);
+  c->cur = -1;
   *out = c;
   return RD_OK;
+}
+
+int rd_ctx_create(int32_t device, int64_t max_particles, rd_ctx** out) {
+  int r = rd_pool_create(device, max_particles, 1, out);
+  if (r) return r;
+  int32_t s;
+  r = open_slot(*out, max_particles, &s);
+  if (r) {
+    rd_ctx_destroy(*out);
+    *out = nullptr;
+  }
+  return r;
 }
 
 int rd_ctx_destroy(rd_ctx* c) {
   if (!c) return RD_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  double* d_arrays[] = {c->x,  c->y,  c->z,  c->px,  c->py,  c->pz,  c->vx,
-                        c->vy, c->vz, c->ax, c->ay,  c->az,  c->a1x, c->a1y,
-                        c->a1z, c->a2x, c->a2y, c->a2z, c->q, c->m};
+  double* d_arrays[] = {c->x,   c->y,   c->z,   c->px,  c->py,  c->pz,  c->vx,
+                        c->vy,  c->vz,  c->ax,  c->ay,  c->az,  c->a1x, c->a1y,
+                        c->a1z, c->a2x, c->a2y, c->a2z, c->q,   c->m};
   for (double* p : d_arrays)
     if (p) hipFree(p);
-  int32_t* i_arrays[] = {c->species, c->step, c->emitter, c->section, c->id};
+  int32_t* i_arrays[] = {c->species, c->pstep, c->emitter, c->section, c->id, c->alive};
   for (int32_t* p : i_arrays)
     if (p) hipFree(p);
-  Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pack_im, &c->pts, &c->fld,
-                 &c->pts_hi, &c->pts_lo, &c->stage, &c->img64};
+  if (c->d_desc) hipFree(c->d_desc);
+  if (c->d_state) hipFree(c->d_state);
+  if (c->d_life) hipFree(c->d_life);
+  if (c->d_wf) hipFree(c->d_wf);
+  if (c->h_state) hipHostFree(c->h_state);
+  if (c->h_tab) hipHostFree(c->h_tab);
+  Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
+                 &c->stage, &c->img64, &c->tab, &c->red, &c->chains};
   for (Buf* b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -240,118 +386,287 @@ int rd_ctx_destroy(rd_ctx* c) {
   return RD_OK;
 }
 
-int rd_set_geometry(rd_ctx* c, const rd_geometry* geom) {
-  if (!c || !geom) return fail(RD_ERR_ARG, "null pointer");
-  DevGeom g;
-  int r = build_geom(geom, &g);
-  if (r) return r;
-  c->g = g;
-  c->geom_in = *geom;
-  c->geom_set = true;
+int rd_slot_open(rd_ctx* c, int64_t capacity, int32_t* slot) {
+  if (!c || !slot) return fail(RD_ERR_ARG, "null pointer");
+  RD_HIP(hipSetDevice(c->device));
+  return open_slot(c, capacity, slot);
+}
+
+int rd_slot_close(rd_ctx* c, int32_t s) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  if (s < 0 || s >= c->nslots_max || !c->slots[s].active) return fail(RD_ERR_ARG, "slot not open");
+  RD_HIP(hipSetDevice(c->device));
+  RD_HIP(hipStreamSynchronize(c->stream));
+  c->slots[s].active = false;
+  if (c->cur == s) c->cur = -1;
   return RD_OK;
 }
 
-int rd_set_time_step(rd_ctx* c, double dt) {
-  if (!c) return fail(RD_ERR_ARG, "null pointer");
-  if (!(dt > 0.0)) return fail(RD_ERR_ARG, "time step must be > 0");
-  c->dt = dt;
+int rd_slot_select(rd_ctx* c, int32_t s) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  if (s < 0 || s >= c->nslots_max || !c->slots[s].active) return fail(RD_ERR_ARG, "slot not open");
+  c->cur = s;
   return RD_OK;
+}
+
+int rd_slot_step(rd_ctx* c, int32_t s, int32_t* step) {
+  if (!c || !step) return fail(RD_ERR_ARG, "null pointer");
+  if (s < 0 || s >= c->nslots_max || !c->slots[s].active) return fail(RD_ERR_ARG, "slot not open");
+  *step = c->slots[s].step;
+  return RD_OK;
+}
+
+int rd_set_geometry(rd_ctx* c, const rd_geometry* geom) {
+  RD_CUR(c);
+  if (!geom) return fail(RD_ERR_ARG, "null geometry");
+  DevGeom g;
+  int r = build_geom(geom, &g);
+  if (r) return r;
+  sl->desc.g = g;
+  sl->geom = *geom;
+  sl->geom_set = true;
+  return put_desc(c, c->cur);
+}
+
+int rd_set_time_step(rd_ctx* c, double dt) {
+  RD_CUR(c);
+  if (!(dt > 0.0)) return fail(RD_ERR_ARG, "time step must be > 0");
+  sl->desc.dt = dt;
+  sl->desc.dt2 = dt * dt;
+  sl->desc.dt_q0 = dt / Consts::q_0;
+  return put_desc(c, c->cur);
+}
+
+int rd_set_seed(rd_ctx* c, uint64_t seed, uint64_t stream) {
+  RD_CUR(c);
+  sl->desc.seed = seed;
+  sl->desc.stream = stream;
+  return put_desc(c, c->cur);
+}
+
+int rd_set_emitter(rd_ctx* c, const rd_emitter* em) {
+  RD_CUR(c);
+  if (!em) return fail(RD_ERR_ARG, "null emitter");
+  SimDesc& d = sl->desc;
+  int mode = em->mode;
+  if (mode != RD_EMIT_NONE && mode != RD_EMIT_PLANAR_FE && mode != RD_EMIT_PLANAR_GTF &&
+      mode != RD_EMIT_TIP_FE)
+    return fail(RD_ERR_ARG, "unknown emission mode");
+  if (mode != RD_EMIT_NONE && !sl->geom_set) return fail(RD_ERR_STATE, "set the geometry first");
+  if (mode == RD_EMIT_TIP_FE && d.g.kind != RD_GEOM_TIP)
+    return fail(RD_ERR_ARG, "tip emission needs the tip geometry");
+  if ((mode == RD_EMIT_PLANAR_FE || mode == RD_EMIT_PLANAR_GTF) && d.g.kind != RD_GEOM_PLANAR)
+    return fail(RD_ERR_ARG, "planar emission needs the planar geometry");
+  if (mode == RD_EMIT_PLANAR_FE || mode == RD_EMIT_PLANAR_GTF) {
+    if (!(em->lx > 0.0 && em->ly > 0.0)) return fail(RD_ERR_ARG, "emitter size must be > 0");
+    int nx = em->wf_nx, ny = em->wf_ny;
+    if (nx < 1 || ny < 1 || nx * ny > 96 * 96 || !em->wf_table)
+      return fail(RD_ERR_ARG, "work-function table missing or larger than 96 x 96");
+    // append the table to the work-function store
+    int need = c->wf_used + nx * ny;
+    if (need > c->wf_cap) {
+      int ncap = std::max(need, 2 * c->wf_cap + 256);
+      double* nd = nullptr;
+      RD_HIP(hipMalloc((void**)&nd, sizeof(double) * ncap));
+      if (c->d_wf) {
+        RD_HIP(hipMemcpy(nd, c->d_wf, sizeof(double) * c->wf_used, hipMemcpyDeviceToDevice));
+        hipFree(c->d_wf);
+      }
+      c->d_wf = nd;
+      c->wf_cap = ncap;
+    }
+    RD_HIP(hipMemcpy(c->d_wf + c->wf_used, em->wf_table, sizeof(double) * nx * ny,
+                     hipMemcpyHostToDevice));
+    d.wf_off = c->wf_used;
+    d.wf_nx = nx;
+    d.wf_ny = ny;
+    c->wf_used = need;
+  }
+  if (mode == RD_EMIT_TIP_FE && !(em->w_tip > 0.0))
+    return fail(RD_ERR_ARG, "tip work function must be > 0");
+  if (em->quad_n < 0 || em->quad_n > 1024) return fail(RD_ERR_ARG, "quad_n outside 0..1024");
+  if (em->ndim < 0 || em->ndim > 100000) return fail(RD_ERR_ARG, "ndim outside 0..100000");
+  d.mode = mode;
+  d.delay = em->delay;
+  d.ex0 = em->x0;
+  d.ey0 = em->y0;
+  d.elx = em->lx;
+  d.ely = em->ly;
+  d.T = em->T;
+  d.w_tip = em->w_tip;
+  d.quad = em->quad_n > 0 ? em->quad_n : 32;
+  int ndef = mode == RD_EMIT_PLANAR_GTF ? 25 : (mode == RD_EMIT_TIP_FE ? 80 : 200);
+  d.ndim = em->ndim > 0 ? em->ndim : ndef;
+  int r = sync_state(c);
+  if (r) return r;
+  SimState& S = c->h_state[c->cur];
+  // module defaults of the samplers (src/mod_field_emission_v2.F90:55-56,
+  // src/mod_field_thermo_emission.F90:32-33, src/mod_emission_tip.f90:50)
+  if (mode == RD_EMIT_PLANAR_GTF) {
+    S.MH_std = 0.125;
+    S.a_rate = 1.0;
+  } else if (mode == RD_EMIT_TIP_FE) {
+    S.MH_std = 1.0;
+    S.a_rate = 0.5;
+  } else {
+    S.MH_std = 0.0125;
+    S.a_rate = 1.0;
+  }
+  S.residual = 0.0;
+  r = put_state(c, c->cur);
+  if (!r) r = put_desc(c, c->cur);
+  return r;
 }
 
 int rd_put_particles(rd_ctx* c, int32_t n, const double* pos, const double* vel, const double* q,
                      const double* m, const int32_t* species) {
-  if (!c) return fail(RD_ERR_ARG, "null context");
+  RD_CUR(c);
   if (n < 0) return fail(RD_ERR_ARG, "negative particle count");
-  if (n > c->cap) return fail(RD_ERR_CAPACITY, "particle count exceeds context capacity");
+  if (n > sl->desc.cap) return fail(RD_ERR_CAPACITY, "particle count exceeds the slot capacity");
   if (n > 0 && (!pos || !q || !m)) return fail(RD_ERR_ARG, "pos, q and m are required");
-  RD_HIP(hipSetDevice(c->device));
-  int r = upload_vec3(c, n, pos, c->x, c->y, c->z);
+  const int b = sl->desc.base;
+  int r = upload_vec3(c, n, pos, c->x + b, c->y + b, c->z + b);
   if (r) return r;
-  RD_HIP(hipStreamSynchronize(c->stream));
   if (vel) {
-    r = upload_vec3(c, n, vel, c->vx, c->vy, c->vz);
+    r = upload_vec3(c, n, vel, c->vx + b, c->vy + b, c->vz + b);
     if (r) return r;
-    RD_HIP(hipStreamSynchronize(c->stream));
   } else {
-    fill(c, n, 0.0, c->vx);
-    fill(c, n, 0.0, c->vy);
-    fill(c, n, 0.0, c->vz);
+    fill(c, n, 0.0, c->vx + b);
+    fill(c, n, 0.0, c->vy + b);
+    fill(c, n, 0.0, c->vz + b);
   }
-  double* zero[] = {c->px, c->py, c->pz, c->ax, c->ay, c->az, c->a1x,
-                    c->a1y, c->a1z, c->a2x, c->a2y, c->a2z};
-  for (double* p : zero) fill(c, n, 0.0, p);
+  double* zero[] = {c->px, c->py, c->pz, c->ax, c->ay, c->az, c->a1x, c->a1y, c->a1z, c->a2x,
+                    c->a2y, c->a2z};
+  for (double* p : zero) fill(c, n, 0.0, p + b);
   if (n > 0) {
-    RD_HIP(hipMemcpyAsync(c->q, q, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    RD_HIP(hipMemcpyAsync(c->m, m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    RD_HIP(hipMemcpyAsync(c->q + b, q, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    RD_HIP(hipMemcpyAsync(c->m + b, m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     if (species)
-      RD_HIP(hipMemcpyAsync(c->species, species, sizeof(int32_t) * n, hipMemcpyHostToDevice,
+      RD_HIP(hipMemcpyAsync(c->species + b, species, sizeof(int32_t) * n, hipMemcpyHostToDevice,
                             c->stream));
     else
-      k_fill_i<<<(n + 255) / 256, 256, 0, c->stream>>>(n, RD_SPECIES_ELEC, c->species);
-    int32_t* izero[] = {c->step, c->emitter, c->section};
-    for (int32_t* p : izero) k_fill_i<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 0, p);
+      k_fill_i<<<(n + 255) / 256, 256, 0, c->stream>>>(n, RD_SPECIES_ELEC, c->species + b);
+    int32_t* izero[] = {c->pstep, c->emitter};
+    for (int32_t* p : izero) k_fill_i<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 0, p + b);
+    k_fill_i<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 1, c->section + b);
+    k_fill_i<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 1, c->alive + b);
+    k_iota<<<(n + 255) / 256, 256, 0, c->stream>>>(n, 0, c->id + b);
     RD_HIP(hipGetLastError());
   }
-  RD_HIP(hipStreamSynchronize(c->stream));
-  c->n = n;
+  r = sync_state(c);
+  if (r) return r;
+  SimState& S = c->h_state[c->cur];
+  S.n = n;
+  S.nrID = n;
+  r = put_state(c, c->cur);
+  if (r) return r;
+  sl->n = n;
   return RD_OK;
 }
 
 int rd_particle_count(rd_ctx* c, int32_t* n) {
-  if (!c || !n) return fail(RD_ERR_ARG, "null pointer");
-  *n = c->n;
+  RD_CUR(c);
+  if (!n) return fail(RD_ERR_ARG, "null pointer");
+  *n = sl->n;
   return RD_OK;
 }
 
 int rd_get_particles(rd_ctx* c, double* pos, double* vel, double* q, double* m,
                      int32_t* species) {
-  if (!c) return fail(RD_ERR_ARG, "null context");
-  RD_HIP(hipSetDevice(c->device));
-  int n = c->n;
+  RD_CUR(c);
+  const int n = sl->n, b = sl->desc.base;
   int r = RD_OK;
-  if (pos) r = download_vec3(c, n, c->x, c->y, c->z, pos);
-  if (!r && vel) r = download_vec3(c, n, c->vx, c->vy, c->vz, vel);
+  if (pos) r = download_vec3(c, n, c->x + b, c->y + b, c->z + b, pos);
+  if (!r && vel) r = download_vec3(c, n, c->vx + b, c->vy + b, c->vz + b, vel);
   if (r) return r;
   if (n > 0) {
-    if (q) RD_HIP(hipMemcpyAsync(q, c->q, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    if (m) RD_HIP(hipMemcpyAsync(m, c->m, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (q)
+      RD_HIP(hipMemcpyAsync(q, c->q + b, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (m)
+      RD_HIP(hipMemcpyAsync(m, c->m + b, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     if (species)
-      RD_HIP(hipMemcpyAsync(species, c->species, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+      RD_HIP(hipMemcpyAsync(species, c->species + b, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
                             c->stream));
   }
   RD_HIP(hipStreamSynchronize(c->stream));
   return RD_OK;
 }
 
+int rd_get_state(rd_ctx* c, double* pos, double* prev, double* vel, double* acc, double* a1,
+                 double* a2, int32_t* ints) {
+  RD_CUR(c);
+  const int n = sl->n, b = sl->desc.base;
+  int r = RD_OK;
+  if (pos) r = download_vec3(c, n, c->x + b, c->y + b, c->z + b, pos);
+  if (!r && prev) r = download_vec3(c, n, c->px + b, c->py + b, c->pz + b, prev);
+  if (!r && vel) r = download_vec3(c, n, c->vx + b, c->vy + b, c->vz + b, vel);
+  if (!r && acc) r = download_vec3(c, n, c->ax + b, c->ay + b, c->az + b, acc);
+  if (!r && a1) r = download_vec3(c, n, c->a1x + b, c->a1y + b, c->a1z + b, a1);
+  if (!r && a2) r = download_vec3(c, n, c->a2x + b, c->a2y + b, c->a2z + b, a2);
+  if (r) return r;
+  if (ints && n > 0) {
+    std::vector<int32_t> tmp((size_t)n);
+    int32_t* cols[] = {c->species, c->pstep, c->emitter, c->section, c->id};
+    for (int k = 0; k < 5; ++k) {
+      RD_HIP(hipMemcpy(tmp.data(), cols[k] + b, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) ints[5 * (size_t)i + k] = tmp[i];
+    }
+  }
+  return RD_OK;
+}
+
+int rd_get_life_time(rd_ctx* c, int32_t* hist) {
+  RD_CUR(c);
+  if (!hist) return fail(RD_ERR_ARG, "null pointer");
+  std::vector<int32_t> tmp(2 * kLife);
+  RD_HIP(hipStreamSynchronize(c->stream));
+  RD_HIP(hipMemcpy(tmp.data(), c->d_life + (size_t)c->cur * 2 * kLife, sizeof(int32_t) * 2 * kLife,
+                   hipMemcpyDeviceToHost));
+  // (1000, 2) column-major like life_time(1:MAX_LIFE_TIME, 1:nrSpecies)
+  for (int s = 0; s < 2; ++s)
+    for (int l = 0; l < kLife; ++l) hist[(size_t)s * kLife + l] = tmp[(size_t)s * kLife + l];
+  return RD_OK;
+}
+
 int rd_accel(rd_ctx* c, int32_t precision) {
-  if (!c) return fail(RD_ERR_ARG, "null context");
-  if (!c->geom_set) return fail(RD_ERR_STATE, "geometry not set");
-  RD_HIP(hipSetDevice(c->device));
-  int r = launch_accel(c, precision, nullptr, nullptr);
+  RD_CUR(c);
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  std::vector<Ent> ents{slot_ent(c, c->cur)};
+  int r = launch_accel(c, ents, precision, nullptr, nullptr);
   if (r) return r;
   RD_HIP(hipStreamSynchronize(c->stream));
   return RD_OK;
 }
 
 int rd_get_accel(rd_ctx* c, double* acc) {
-  if (!c || !acc) return fail(RD_ERR_ARG, "null pointer");
-  RD_HIP(hipSetDevice(c->device));
-  return download_vec3(c, c->n, c->ax, c->ay, c->az, acc);
+  RD_CUR(c);
+  if (!acc) return fail(RD_ERR_ARG, "null pointer");
+  const int b = sl->desc.base;
+  return download_vec3(c, sl->n, c->ax + b, c->ay + b, c->az + b, acc);
+}
+
+static int field_points(rd_ctx* c, rd::Slot* sl, int m, int precision, hipEvent_t k0,
+                        hipEvent_t k1) {
+  Ent e = slot_ent(c, c->cur);
+  e.pbase = 0;
+  e.m = m;
+  (void)sl;
+  std::vector<Ent> ents{e};
+  return launch_field(c, ents, m, precision, k0, k1);
 }
 
 int rd_field_batch(rd_ctx* c, int32_t m, const double* pts, double* field, int32_t precision) {
-  if (!c) return fail(RD_ERR_ARG, "null context");
-  if (!c->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  RD_CUR(c);
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
   if (m < 0) return fail(RD_ERR_ARG, "negative point count");
   if (m == 0) return RD_OK;
   if (!pts || !field) return fail(RD_ERR_ARG, "null point/field buffer");
-  RD_HIP(hipSetDevice(c->device));
   int r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)m);
   if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)m);
   if (r) return r;
   double* ptx = (double*)c->pts.p;
   r = upload_vec3(c, m, pts, ptx, ptx + m, ptx + 2 * m);
-  if (!r) r = launch_field(c, m, precision, nullptr, nullptr);
+  if (!r) r = field_points(c, sl, m, precision, nullptr, nullptr);
   if (r) return r;
   double* f = (double*)c->fld.p;
   return download_vec3(c, m, f, f + m, f + 2 * m, field);
@@ -359,13 +674,12 @@ int rd_field_batch(rd_ctx* c, int32_t m, const double* pts, double* field, int32
 
 int rd_field_grid(rd_ctx* c, double x0, double y0, double lx, double ly, int32_t nx, int32_t ny,
                   double zz, int32_t precision, double* field) {
-  if (!c) return fail(RD_ERR_ARG, "null context");
-  if (!c->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  RD_CUR(c);
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
   if (nx < 1 || ny < 1 || (int64_t)nx * ny > (1 << 26)) return fail(RD_ERR_ARG, "bad grid size");
-  RD_HIP(hipSetDevice(c->device));
   int m = nx * ny;
   int r = launch_grid_points(c, x0, y0, lx, ly, nx, ny, zz);
-  if (!r) r = launch_field(c, m, precision, nullptr, nullptr);
+  if (!r) r = field_points(c, sl, m, precision, nullptr, nullptr);
   if (r) return r;
   if (field) {
     double* f = (double*)c->fld.p;
@@ -375,6 +689,93 @@ int rd_field_grid(rd_ctx* c, double x0, double y0, double lx, double ly, int32_t
   return RD_OK;
 }
 
+// ---- time step -------------------------------------------------------------
+static int check_step_ready(rd_ctx* c, rd::Slot* sl) {
+  (void)c;
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  if (!(sl->desc.dt > 0.0)) return fail(RD_ERR_STATE, "time step not set");
+  return RD_OK;
+}
+
+int rd_add_particles(rd_ctx* c, int32_t n, const double* pos, const double* vel,
+                     const int32_t* species, int32_t step, int32_t emit, const int32_t* sec) {
+  RD_CUR(c);
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  if (n < 0) return fail(RD_ERR_ARG, "negative particle count");
+  if (n > 0 && !pos) return fail(RD_ERR_ARG, "null positions");
+  if (species)
+    for (int k = 0; k < n; ++k)
+      if (species[k] != RD_SPECIES_ELEC && species[k] != RD_SPECIES_ION)
+        return fail(RD_ERR_ARG, "unknown particle species");
+  return engine_add(c, c->cur, n, pos, vel, species, step, emit, sec);
+}
+
+static int run_phases(rd_ctx* c, int32_t step, int32_t precision, int phases, rd_step_record* rec) {
+  std::vector<int> slots{c->cur};
+  int steps[1] = {step};
+  return engine_phases(c, slots, precision, phases, steps, rec);
+}
+
+int rd_do_emission(rd_ctx* c, int32_t step, int32_t precision, rd_step_record* rec) {
+  RD_CUR(c);
+  int r = check_step_ready(c, sl);
+  if (r) return r;
+  return run_phases(c, step, precision, 1, rec);
+}
+
+int rd_update_position(rd_ctx* c, int32_t step, int32_t precision, rd_step_record* rec) {
+  RD_CUR(c);
+  int r = check_step_ready(c, sl);
+  if (r) return r;
+  return run_phases(c, step, precision, 2, rec);
+}
+
+int rd_remove_particles(rd_ctx* c, int32_t step, rd_step_record* rec) {
+  RD_CUR(c);
+  int r = run_phases(c, step, RD_PREC_FP64, 4, rec);
+  if (r) return r;
+  sl->step = step;
+  return RD_OK;
+}
+
+int rd_run(rd_ctx* c, int32_t nsteps, int32_t precision, rd_step_record* records) {
+  RD_CUR(c);
+  int r = check_step_ready(c, sl);
+  if (r) return r;
+  if (nsteps < 0) return fail(RD_ERR_ARG, "negative step count");
+  for (int k = 0; k < nsteps; ++k) {
+    int step = sl->step + 1;
+    r = run_phases(c, step, precision, 7, records ? &records[k] : nullptr);
+    if (r) return r;
+    sl->step = step;
+  }
+  return RD_OK;
+}
+
+int rd_pool_step(rd_ctx* c, int32_t precision, rd_step_record* records) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  RD_HIP(hipSetDevice(c->device));
+  std::vector<int> slots;
+  std::vector<int> steps;
+  for (int s = 0; s < c->nslots_max; ++s) {
+    Slot& sl = c->slots[s];
+    if (!sl.active) continue;
+    if (!sl.geom_set || !(sl.desc.dt > 0.0)) return fail(RD_ERR_STATE, "slot not configured");
+    slots.push_back(s);
+    steps.push_back(sl.step + 1);
+  }
+  if (slots.empty()) return RD_OK;
+  std::vector<rd_step_record> recs(slots.size());
+  int r = engine_phases(c, slots, precision, 7, steps.data(), recs.data());
+  if (r) return r;
+  for (size_t k = 0; k < slots.size(); ++k) {
+    c->slots[slots[k]].step = steps[k];
+    if (records) records[slots[k]] = recs[k];
+  }
+  return RD_OK;
+}
+
+// ---- measurement -------------------------------------------------------------
 static int time_loop(rd_ctx* c, int reps, double* total_ms, double* kernel_ms,
                      int (*body)(rd_ctx*, void*, hipEvent_t, hipEvent_t), void* arg) {
   if (reps < 1) return fail(RD_ERR_ARG, "reps must be >= 1");
@@ -407,15 +808,15 @@ struct AccelArg {
   int precision;
 };
 static int accel_body(rd_ctx* c, void* a, hipEvent_t k0, hipEvent_t k1) {
-  return launch_accel(c, ((AccelArg*)a)->precision, k0, k1);
+  std::vector<Ent> ents{slot_ent(c, c->cur)};
+  return launch_accel(c, ents, ((AccelArg*)a)->precision, k0, k1);
 }
 
 int rd_time_accel(rd_ctx* c, int32_t precision, int32_t reps, double* total_ms,
                   double* kernel_ms) {
-  if (!c) return fail(RD_ERR_ARG, "null context");
-  if (!c->geom_set) return fail(RD_ERR_STATE, "geometry not set");
-  if (c->n < 1) return fail(RD_ERR_STATE, "no particles");
-  RD_HIP(hipSetDevice(c->device));
+  RD_CUR(c);
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  if (sl->n < 1) return fail(RD_ERR_STATE, "no particles");
   AccelArg a{precision};
   return time_loop(c, reps, total_ms, kernel_ms, accel_body, &a);
 }
@@ -425,17 +826,16 @@ struct GridArg {
 };
 static int grid_body(rd_ctx* c, void* a, hipEvent_t k0, hipEvent_t k1) {
   GridArg* g = (GridArg*)a;
-  return launch_field(c, g->m, g->precision, k0, k1);
+  return field_points(c, &c->slots[c->cur], g->m, g->precision, k0, k1);
 }
 
 int rd_time_field_grid(rd_ctx* c, double x0, double y0, double lx, double ly, int32_t nx,
                        int32_t ny, double zz, int32_t precision, int32_t reps, double* total_ms,
                        double* kernel_ms) {
-  if (!c) return fail(RD_ERR_ARG, "null context");
-  if (!c->geom_set) return fail(RD_ERR_STATE, "geometry not set");
-  if (c->n < 1) return fail(RD_ERR_STATE, "no particles");
+  RD_CUR(c);
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  if (sl->n < 1) return fail(RD_ERR_STATE, "no particles");
   if (nx < 1 || ny < 1 || (int64_t)nx * ny > (1 << 26)) return fail(RD_ERR_ARG, "bad grid size");
-  RD_HIP(hipSetDevice(c->device));
   int r = launch_grid_points(c, x0, y0, lx, ly, nx, ny, zz);
   if (r) return r;
   GridArg a{nx * ny, precision};

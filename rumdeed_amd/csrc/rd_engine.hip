@@ -1,0 +1,1397 @@
+// The device-resident time step: emission, Beeman update with boundary
+// marking and Ramo current, order-preserving compaction, batched Add_Particle.
+//
+// Reference (RUMDEED tree):
+//   main loop ................. src/main.F90:175-219
+//   Beeman position ........... Update_ElecIon_Position, src/mod_verlet.F90:197-232
+//   boundaries ................ Check_Boundary_Planar :325-338, Check_Boundary_Tip
+//                               src/mod_emission_tip.f90:1627-1647
+//   Beeman velocity, Ramo ..... Update_ElecIon_Velocity, src/mod_verlet.F90:449-509
+//   marking / compaction ...... Mark_Particles_Remove, Remove_Particles, record_lifetime,
+//                               src/mod_pair.F90:169-562, :1135-1223
+//   Add_Particle .............. src/mod_pair.F90:29-159
+//   planar FE emission ........ Do_Field_Emission_Planar_rectangle,
+//                               Metropolis_Hastings_rectangle_J_batch,
+//                               src/mod_field_emission_v2.F90:261-395, :1284-1516
+//   thermal-field emission .... src/mod_field_thermo_emission.F90:136-446
+//   tip emission .............. Do_Field_Emission_Tip_OLDCODE, Metro_algo_tip_v3,
+//                               src/mod_emission_tip.f90:417-534, :1213-1390
+//
+// Every kernel takes a launch table: one entry per simulation slot, so a
+// pool of independent simulations advances with one launch per kernel.
+//
+// Emission semantics on the device (all simulations, all modes): the
+// Metropolis-Hastings chains of a step run in lockstep against the particle
+// configuration of the start of the step (the reference's mh_batch
+// semantics, src/mod_global.F90:316-328).  The planar FE sampler updates the
+// proposal width once per jump from the batch acceptance rate, as the
+// reference's batch sampler does; the thermal-field and tip samplers, whose
+// reference updates the width once per chain, apply the per-chain updates in
+// chain order after the step's chains (all chains of a step use the width of
+// the start of the step).  The supply integral replaces the Cuba adaptive
+// integrator (not available) with a midpoint rule on a fixed grid summed in
+// point order.  Random numbers come from Philox4x64-10 streams keyed by the
+// simulation seed and indexed by (step, purpose, chain).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "rd_internal.h"
+#include "rd_physics.h"
+
+namespace rd {
+
+static constexpr double kTiny = 2.2250738585072014e-308;  // tiny(1.0d0)
+static constexpr double kHuge = 1.7976931348623157e308;   // huge(1.0d0)
+
+// ------------------------------------------------------------------------
+// Chain arrays (structure of arrays over all chains of a step).
+struct Chains {
+  double *cx, *cy;    // current position (x, y) planar, (xi, phi) tip
+  double *px, *py;    // proposal
+  double *F, *sup, *D;
+  double *ox, *oy, *oz;  // final emission site (tip: xyz on the surface)
+  int *ok, *acc, *rej, *pv;  // start found; per-chain counts; proposal valid
+};
+
+static Chains chain_view(rd_ctx* c, int C) {
+  Chains h;
+  double* d = (double*)c->chains.p;
+  h.cx = d;
+  h.cy = d + C;
+  h.px = d + 2 * (size_t)C;
+  h.py = d + 3 * (size_t)C;
+  h.F = d + 4 * (size_t)C;
+  h.sup = d + 5 * (size_t)C;
+  h.D = d + 6 * (size_t)C;
+  h.ox = d + 7 * (size_t)C;
+  h.oy = d + 8 * (size_t)C;
+  h.oz = d + 9 * (size_t)C;
+  int* ip = (int*)(d + 10 * (size_t)C);
+  h.ok = ip;
+  h.acc = ip + C;
+  h.rej = ip + 2 * (size_t)C;
+  h.pv = ip + 3 * (size_t)C;
+  return h;
+}
+static size_t chain_bytes(int C) { return (size_t)C * (10 * sizeof(double) + 4 * sizeof(int)); }
+
+// Work function at a surface point of a planar emitter.
+__device__ __forceinline__ double work_at(const SimDesc& s, const double* wf, double x, double y,
+                                          int& sec) {
+  return work_checker(wf + s.wf_off, s.wf_nx, s.wf_ny, s.ex0, s.ey0, s.elx, s.ely, x, y, sec);
+}
+
+__device__ __forceinline__ TipSurf tip_surf(const DevGeom& g) {
+  TipSurf t;
+  t.a = g.a_foci;
+  t.eta1 = g.eta_1;
+  t.shift_z = g.shift_z;
+  t.max_xi = g.max_xi;
+  return t;
+}
+
+// Normal component of the field on the tip surface (Field_normal).
+__device__ __forceinline__ double field_normal(const DevGeom& g, double x, double y, double ex,
+                                              double ey, double ez) {
+  double nx, ny, nz;
+  tip_normal(tip_surf(g), x, y, nx, ny, nz);
+  return nx * ex + ny * ey + nz * ez;
+}
+
+// MH chain target at a surface point with field component F (< 0).
+__device__ __forceinline__ double chain_target(const SimDesc& s, const double* wf,
+                                               const EmitConsts& ec, double F, double a, double b,
+                                               double xs, double ys, double zs) {
+  if (s.mode == RD_EMIT_PLANAR_FE) {
+    int sec;
+    double w = work_at(s, wf, a, b, sec);
+    return fn_supply_log(ec, s.g.ic, F, w);  // Elec_Supply_log
+  }
+  if (s.mode == RD_EMIT_PLANAR_GTF) {
+    int sec;
+    double w = work_at(s, wf, a, b, sec);
+    double J = gtf_current(F, s.T, w);
+    return rd_log(J > kTiny ? J : kTiny);
+  }
+  // tip: Tip_fe_target_log(eta_f, xi, pos); a = xi
+  double t = fn_t(s.g.ic, ec.l_const, F, s.w_tip);
+  double sup = s.dt_q0 * ec.a_FN / (t * t * s.w_tip) * (F * F);
+  (void)xs;
+  (void)ys;
+  (void)zs;
+  return rd_log(sup > kTiny ? sup : kTiny) + 0.5 * rd_log(a * a - s.g.eta_1 * s.g.eta_1);
+}
+
+// ------------------------------------------------------------------------
+__global__ void k_reset_counters(const Ent* tab, SimState* st) {
+  const Ent e = tab[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  SimState& s = st[e.slot];
+  s.emitted = 0;
+  s.n_round = 0;
+  s.rem_total = s.rem_top = s.rem_bot = s.rem_elec = s.rem_ion = s.rem_top_emit = 0;
+  s.pending = 0;
+  s.start_rounds = 0;
+  s.N_sup = 0.0;
+  s.df_avg = 0.0;
+  for (int k = 0; k < 3; ++k) s.F_avg[k] = 0.0;
+}
+
+// Quadrature points of the supply integral.
+__global__ void k_quad_points(const Ent* tab, const SimDesc* desc, double* __restrict__ ptx,
+                              double* __restrict__ pty, double* __restrict__ ptz) {
+  const Ent e = tab[blockIdx.z];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= e.m) return;
+  const SimDesc& s = desc[e.slot];
+  double x, y, z;
+  if (s.mode == RD_EMIT_TIP_FE) {
+    const int nr = 100;
+    int i = k / nr + 1, j = k % nr + 1;
+    double len_xi = (s.g.max_xi - 1.0) / nr;
+    double len_phi = 2.0 * 3.141592653589793238462643383279502884197 / nr;
+    double xi_c = 1.0 + (i - 0.5) * len_xi;
+    double phi_c = (j - 0.5) * len_phi;
+    tip_xyz(tip_surf(s.g), xi_c, s.g.eta_1, phi_c, x, y, z);
+  } else {
+    const int G = s.quad;
+    int ix = k % G, iy = k / G;
+    double u = (ix + 0.5) / G, v = (iy + 0.5) / G;
+    x = s.ex0 + u * s.elx;
+    y = s.ey0 + v * s.ely;
+    z = 0.0;
+  }
+  ptx[e.pbase + k] = x;
+  pty[e.pbase + k] = y;
+  ptz[e.pbase + k] = z;
+}
+
+// Integrand per point, then the point-ordered sum and the number of chains.
+__global__ __launch_bounds__(256) void k_quad_reduce(const Ent* tab, const SimDesc* desc,
+                                                     SimState* st, const double* wf,
+                                                     const double* __restrict__ ptx,
+                                                     const double* __restrict__ pty,
+                                                     const double* __restrict__ ptz,
+                                                     const double* __restrict__ fx,
+                                                     const double* __restrict__ fy,
+                                                     const double* __restrict__ fz,
+                                                     double* __restrict__ qv) {
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  const EmitConsts ec = emit_consts();
+  for (int k = threadIdx.x; k < e.m; k += blockDim.x) {
+    const int p = e.pbase + k;
+    double val = 0.0, F;
+    if (s.mode == RD_EMIT_TIP_FE) {
+      F = field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]);
+      if (F >= 0.0) {
+        val = 0.0;
+      } else {
+        const int nr = 100;
+        int i = k / nr + 1, j = k % nr + 1;
+        double len_xi = (s.g.max_xi - 1.0) / nr;
+        double len_phi = 2.0 * 3.141592653589793238462643383279502884197 / nr;
+        double xi_1 = 1.0 + (i - 1.0) * len_xi, phi_1 = (j - 1.0) * len_phi;
+        double xi_2 = 1.0 + (i + 0.0) * len_xi, phi_2 = (j + 0.0) * len_phi;
+        double A_f = tip_area(tip_surf(s.g), xi_1, xi_2, phi_1, phi_2);
+        double t = fn_t(s.g.ic, ec.l_const, F, s.w_tip);
+        // Elec_Supply (src/mod_emission_tip.f90:1711-1718)
+        val = A_f * ec.a_FN * (F * F) * s.dt / (ec.q0 * s.w_tip * (t * t));
+      }
+    } else {
+      F = fz[p];
+      if (F < 0.0) {
+        int sec;
+        double w = work_at(s, wf, ptx[p], pty[p], sec);
+        if (s.mode == RD_EMIT_PLANAR_FE)
+          val = fn_supply(ec, s.g.ic, s.dt_q0, F, w);
+        else
+          val = gtf_current(F, s.T, w) * s.dt_q0;
+      }
+      val = (s.elx * s.ely) * val;
+    }
+    qv[p] = val;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  SimState& S = st[e.slot];
+  double sum = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int k = 0; k < e.m; ++k) {
+    const int p = e.pbase + k;
+    sum = sum + qv[p];
+    if (s.mode == RD_EMIT_TIP_FE) {
+      f2 = f2 + field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]);
+    } else {
+      f0 = f0 + fx[p];
+      f1 = f1 + fy[p];
+      f2 = f2 + fz[p];
+    }
+  }
+  int nr;
+  if (s.mode == RD_EMIT_TIP_FE) {
+    S.N_sup = sum;
+    nr = nint(sum);
+    if (nr < 0) nr = 0;
+    S.F_avg[0] = 0.0;
+    S.F_avg[1] = 0.0;
+    S.F_avg[2] = f2 / e.m;
+  } else {
+    double G2 = (double)s.quad * (double)s.quad;
+    double nsup = sum / G2;
+    S.N_sup = nsup;
+    if (s.mode == RD_EMIT_PLANAR_FE) {
+      nr = nint(nsup + S.residual);
+      S.residual = nsup - nr;
+      if (nr < 0) nr = 0;
+    } else {
+      nr = poisson(s.seed, s.stream, (uint64_t)e.step, nsup);
+    }
+    S.F_avg[0] = f0 / e.m;
+    S.F_avg[1] = f1 / e.m;
+    S.F_avg[2] = f2 / e.m;
+  }
+  S.n_round = nr;
+}
+
+// ---- Metropolis-Hastings ------------------------------------------------
+__global__ void k_mh_init(const Ent* tab, const SimDesc* desc, SimState* st, Chains ch) {
+  const Ent e = tab[blockIdx.z];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    SimState& S = st[e.slot];
+    S.pending = e.nch;
+    if (desc[e.slot].mode == RD_EMIT_TIP_FE) {
+      // Metro_algo_tip_v3 brings the shared width into its limits first
+      if (S.MH_std > 0.125)
+        S.MH_std = 0.125;
+      else if (S.MH_std < 0.0005)
+        S.MH_std = 0.0005;
+    }
+  }
+  if (k >= e.nch) return;
+  const int c = e.cbase + k;
+  ch.ok[c] = 0;
+  ch.acc[c] = 0;
+  ch.rej[c] = 0;
+  ch.pv[c] = 1;
+}
+
+// Uniform start proposals for the chains without a start site.
+__global__ void k_mh_start(const Ent* tab, const SimDesc* desc, Chains ch, int round,
+                           double* __restrict__ ptx, double* __restrict__ pty,
+                           double* __restrict__ ptz) {
+  const Ent e = tab[blockIdx.z];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= e.nch) return;
+  const int c = e.cbase + k;
+  if (ch.ok[c]) return;
+  const SimDesc& s = desc[e.slot];
+  u64x4 r = philox4x64((uint64_t)e.step, tag(kTagStart, (uint64_t)round), (uint64_t)k, 0, s.seed,
+                       s.stream);
+  double u1 = u01(r.v[0]), u2 = u01(r.v[1]);
+  double a, b, x, y, z;
+  if (s.mode == RD_EMIT_TIP_FE) {
+    a = 1.0 + (s.g.max_xi - 1.0) * u1;
+    b = 2.0 * 3.141592653589793238462643383279502884197 * u2;
+    tip_xyz(tip_surf(s.g), a, s.g.eta_1, b, x, y, z);
+  } else {
+    a = u1 * s.elx + s.ex0;
+    b = u2 * s.ely + s.ey0;
+    x = a;
+    y = b;
+    z = 0.0;
+  }
+  ch.px[c] = a;
+  ch.py[c] = b;
+  ptx[e.pbase + k] = x;
+  pty[e.pbase + k] = y;
+  ptz[e.pbase + k] = z;
+}
+
+__global__ __launch_bounds__(256) void k_mh_start_check(const Ent* tab, const SimDesc* desc,
+                                                        SimState* st, const double* wf,
+                                                        Chains ch, int round, int last,
+                                                        const double* __restrict__ ptx,
+                                                        const double* __restrict__ pty,
+                                                        const double* __restrict__ ptz,
+                                                        const double* __restrict__ fx,
+                                                        const double* __restrict__ fy,
+                                                        const double* __restrict__ fz) {
+  __shared__ int red[256];
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  const EmitConsts ec = emit_consts();
+  int pend = 0;
+  for (int k = threadIdx.x; k < e.nch; k += blockDim.x) {
+    const int c = e.cbase + k;
+    if (ch.ok[c]) continue;
+    const int p = e.pbase + k;
+    double F = (s.mode == RD_EMIT_TIP_FE)
+                   ? field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p])
+                   : fz[p];
+    if (F < 0.0) {
+      ch.ok[c] = 1;
+      ch.cx[c] = ch.px[c];
+      ch.cy[c] = ch.py[c];
+      ch.F[c] = F;
+      ch.sup[c] = chain_target(s, wf, ec, F, ch.px[c], ch.py[c], ptx[p], pty[p], ptz[p]);
+      ch.ox[c] = ptx[p];
+      ch.oy[c] = pty[p];
+      ch.oz[c] = ptz[p];
+    } else if (last) {
+      // failed chain: unfavourable field, no emission
+      ch.ok[c] = -1;
+      ch.F[c] = 1.0;
+      ch.sup[c] = -kHuge;
+      if (s.mode == RD_EMIT_TIP_FE) {
+        double x, y, z;
+        tip_xyz(tip_surf(s.g), 1.0, s.g.eta_1, 0.0, x, y, z);
+        ch.cx[c] = 1.0;
+        ch.cy[c] = 0.0;
+        ch.ox[c] = x;
+        ch.oy[c] = y;
+        ch.oz[c] = z;
+      } else {
+        ch.cx[c] = s.ex0;
+        ch.cy[c] = s.ey0;
+        ch.ox[c] = s.ex0;
+        ch.oy[c] = s.ey0;
+        ch.oz[c] = 0.0;
+      }
+    } else {
+      ++pend;
+    }
+  }
+  red[threadIdx.x] = pend;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    st[e.slot].pending = red[0];
+    st[e.slot].start_rounds = round;
+  }
+}
+
+// Gaussian proposal of jump `it` (1-based) for every started chain.
+__global__ void k_mh_propose(const Ent* tab, const SimDesc* desc, const SimState* st, Chains ch,
+                             int it, double* __restrict__ ptx, double* __restrict__ pty,
+                             double* __restrict__ ptz) {
+  const Ent e = tab[blockIdx.z];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= e.nch) return;
+  const SimDesc& s = desc[e.slot];
+  if (it > s.ndim) return;
+  const int c = e.cbase + k;
+  const int p = e.pbase + k;
+  if (ch.ok[c] != 1) {
+    ptx[p] = ch.ox[c];
+    pty[p] = ch.oy[c];
+    ptz[p] = ch.oz[c];
+    return;
+  }
+  const double MH = st[e.slot].MH_std;
+  const uint64_t tg = tag(kTagJump, (uint64_t)it);
+  const double PI2 = 2.0 * 3.141592653589793238462643383279502884197;
+  if (s.mode == RD_EMIT_TIP_FE) {
+    const int first = nint(s.ndim * 0.25);
+    double s1, s2;
+    if (it > first) {
+      s1 = (s.g.max_xi - 1.0) * MH;
+      s2 = PI2 * MH;
+    } else {
+      s1 = (s.g.max_xi - 1.0) * 0.10;
+      s2 = PI2 * 0.10;
+    }
+    double d1, d2;
+    box_muller(s.seed, s.stream, (uint64_t)e.step, tg, (uint64_t)k, 0.0, 0.0, s1, s2, d1, d2);
+    double nxi = ch.cx[c] + d1;
+    double nphi = fmodulo(ch.cy[c] + d2, PI2);
+    if (nxi > s.g.max_xi) nxi = 2.0 * s.g.max_xi - nxi;
+    if (nxi < 1.0) nxi = 2.0 - nxi;
+    int valid = !((nxi < 1.0) || (nxi > s.g.max_xi));
+    ch.pv[c] = valid;
+    ch.px[c] = nxi;
+    ch.py[c] = nphi;
+    double x, y, z;
+    if (valid) {
+      tip_xyz(tip_surf(s.g), nxi, s.g.eta_1, nphi, x, y, z);
+    } else {
+      x = ch.ox[c];
+      y = ch.oy[c];
+      z = ch.oz[c];
+    }
+    ptx[p] = x;
+    pty[p] = y;
+    ptz[p] = z;
+    return;
+  }
+  double sx, sy;
+  if (s.mode == RD_EMIT_PLANAR_FE) {
+    const int first = nint(s.ndim * 0.25);
+    double f = (it > first) ? MH : 0.10;
+    sx = s.elx * f;
+    sy = s.ely * f;
+  } else {
+    sx = s.elx * MH;
+    sy = s.ely * MH;
+  }
+  double nx, ny;
+  box_muller(s.seed, s.stream, (uint64_t)e.step, tg, (uint64_t)k, ch.cx[c], ch.cy[c], sx, sy, nx,
+             ny);
+  if (s.mode == RD_EMIT_PLANAR_FE) {
+    // check_limits_metro_rec (src/mod_field_emission_v2.F90:1466-1516): reflect
+    double x_max = s.ex0 + s.elx, x_min = s.ex0;
+    double y_max = s.ey0 + s.ely, y_min = s.ey0;
+    if (nx > x_max)
+      nx = x_max - (nx - x_max);
+    else if (nx < x_min)
+      nx = (x_min - nx) + x_min;
+    if (ny > y_max)
+      ny = y_max - (ny - y_max);
+    else if (ny < y_min)
+      ny = (y_min - ny) + y_min;
+  } else {
+    // check_limits_metro_rec (src/mod_field_thermo_emission.F90:369-389): wrap
+    double sxs = (nx - s.ex0) / s.elx, sys = (ny - s.ey0) / s.ely;
+    if ((sxs > 1.0) || (sxs < 0.0)) sxs = 1.0 - (sxs - __builtin_floor(sxs));
+    if ((sys > 1.0) || (sys < 0.0)) sys = 1.0 - (sys - __builtin_floor(sys));
+    nx = sxs * s.elx + s.ex0;
+    ny = sys * s.ely + s.ey0;
+  }
+  ch.px[c] = nx;
+  ch.py[c] = ny;
+  ptx[p] = nx;
+  pty[p] = ny;
+  ptz[p] = 0.0;
+}
+
+// Accept/reject of jump `it`; one block per simulation.
+__global__ __launch_bounds__(256) void k_mh_accept(const Ent* tab, const SimDesc* desc,
+                                                   SimState* st, const double* wf, Chains ch,
+                                                   int it, const double* __restrict__ ptx,
+                                                   const double* __restrict__ pty,
+                                                   const double* __restrict__ ptz,
+                                                   const double* __restrict__ fx,
+                                                   const double* __restrict__ fy,
+                                                   const double* __restrict__ fz) {
+  __shared__ int ra[256], rr[256];
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  if (it > s.ndim) return;
+  const EmitConsts ec = emit_consts();
+  const int first = nint(s.ndim * 0.25);
+  const bool tip = s.mode == RD_EMIT_TIP_FE;
+  const bool counted = (s.mode == RD_EMIT_PLANAR_GTF) || (it > first);
+  int na = 0, nrj = 0;
+  for (int k = threadIdx.x; k < e.nch; k += blockDim.x) {
+    const int c = e.cbase + k;
+    if (ch.ok[c] != 1) continue;
+    const int p = e.pbase + k;
+    bool acc = false;
+    bool rejected = false;
+    double F = 0.0;
+    if (tip && !ch.pv[c]) {
+      rejected = true;
+    } else {
+      F = tip ? field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]) : fz[p];
+      bool bad = (s.mode == RD_EMIT_PLANAR_GTF) ? (F > 0.0) : (F >= 0.0);
+      if (bad) {
+        rejected = true;
+      } else {
+        double sup_new = chain_target(s, wf, ec, F, ch.px[c], ch.py[c], ptx[p], pty[p], ptz[p]);
+        double alpha = sup_new - ch.sup[c];
+        if (sup_new >= ch.sup[c]) {
+          acc = true;
+        } else {
+          double rnd = uniform(s.seed, s.stream, (uint64_t)e.step, tag(kTagAccept, (uint64_t)it),
+                               (uint64_t)k, 0);
+          acc = rd_log(rnd) <= alpha;
+        }
+        if (acc) {
+          ch.cx[c] = ch.px[c];
+          ch.cy[c] = ch.py[c];
+          ch.sup[c] = sup_new;
+          ch.F[c] = F;
+          ch.ox[c] = ptx[p];
+          ch.oy[c] = pty[p];
+          ch.oz[c] = ptz[p];
+        } else {
+          rejected = true;
+        }
+      }
+    }
+    if (acc) ++na;
+    if (rejected) ++nrj;
+    if (counted && !tip && s.mode == RD_EMIT_PLANAR_GTF) {
+      if (acc) ch.acc[c] += 1;
+      if (rejected) ch.rej[c] += 1;
+    }
+    if (tip && it > first) {
+      if (acc) ch.acc[c] += 1;
+      if (rejected) ch.rej[c] += 1;
+    }
+  }
+  if (s.mode != RD_EMIT_PLANAR_FE) return;  // per-chain schedules: no batch update
+  ra[threadIdx.x] = na;
+  rr[threadIdx.x] = nrj;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ra[threadIdx.x] += ra[threadIdx.x + w];
+      rr[threadIdx.x] += rr[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    SimState& S = st[e.slot];
+    S.it_a = ra[0];
+    S.it_r = rr[0];
+    if ((it > first) && (ra[0] + rr[0] > 0)) {
+      // MH_std_update (src/mod_field_emission_v2.F90:603-612)
+      S.a_rate = (double)ra[0] / (double)(ra[0] + rr[0]);
+      S.MH_std = S.MH_std * rd_exp(0.025 * (S.a_rate - 0.35));
+      if (S.MH_std > 0.1250)
+        S.MH_std = 0.1250;
+      else if (S.MH_std < 0.00005)
+        S.MH_std = 0.00005;
+    }
+  }
+}
+
+// Per-chain width updates (thermal-field, tip) in chain order and the escape
+// quantity of every chain.
+__global__ __launch_bounds__(256) void k_mh_finish(const Ent* tab, const SimDesc* desc,
+                                                   SimState* st, const double* wf, Chains ch) {
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  const EmitConsts ec = emit_consts();
+  if (threadIdx.x == 0 && s.mode != RD_EMIT_PLANAR_FE) {
+    SimState& S = st[e.slot];
+    const double lo = (s.mode == RD_EMIT_TIP_FE) ? 0.0005 : 0.005;
+    for (int k = 0; k < e.nch; ++k) {
+      const int c = e.cbase + k;
+      if (ch.ok[c] != 1) continue;
+      int a = ch.acc[c], r = ch.rej[c];
+      if (a + r > 0) {
+        S.a_rate = (double)a / (double)(r + a);
+        S.MH_std = S.MH_std * rd_exp(0.025 * (S.a_rate - 0.35));
+        if (S.MH_std > 0.1250)
+          S.MH_std = 0.1250;
+        else if (S.MH_std < lo)
+          S.MH_std = lo;
+      }
+    }
+  }
+  for (int k = threadIdx.x; k < e.nch; k += blockDim.x) {
+    const int c = e.cbase + k;
+    double D;
+    if (s.mode == RD_EMIT_PLANAR_FE) {
+      if (ch.ok[c] == 1) {
+        int sec;
+        double w = work_at(s, wf, ch.cx[c], ch.cy[c], sec);
+        D = fn_escape_log(ec, s.g.ic, ch.F[c], w);
+      } else {
+        D = -kHuge;
+      }
+    } else if (s.mode == RD_EMIT_TIP_FE) {
+      // Escape_Prob_Tip: exp(b_FN w^(3/2) v_y / |F|); failed chains 0
+      if (ch.ok[c] == 1) {
+        double sw = __builtin_sqrt(s.w_tip);
+        D = rd_exp(ec.b_FN * (sw * sw * sw) * fn_v(s.g.ic, ec.l_const, ch.F[c], s.w_tip) /
+                   __builtin_fabs(ch.F[c]));
+      } else {
+        D = 0.0;
+      }
+    } else {
+      D = 0.0;
+    }
+    ch.D[c] = D;
+  }
+}
+
+// Escape test and append (Add_Particle for the emitted electrons, in chain
+// order).  One block of 256 per simulation.
+__global__ __launch_bounds__(256) void k_emit(const Ent* tab, const SimDesc* desc, SimState* st,
+                                              const double* wf, Chains ch, double* X, double* Y,
+                                              double* Z, double* PX, double* PY, double* PZ,
+                                              double* VX, double* VY, double* VZ, double* AX,
+                                              double* AY, double* AZ, double* A1X, double* A1Y,
+                                              double* A1Z, double* A2X, double* A2Y, double* A2Z,
+                                              double* Q, double* M, int32_t* SP, int32_t* STEP,
+                                              int32_t* EMIT, int32_t* SEC, int32_t* ID,
+                                              int32_t* ALIVE) {
+  __shared__ int scan[256];
+  __shared__ double dsum[256];
+  __shared__ int s_off, s_drop;
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  SimState& S = st[e.slot];
+  const EmitConsts ec = emit_consts();
+  const int n0 = S.n, id0 = S.nrID;
+  const double qe = -1.0 * ec.q0, me = 1.0 * ec.m0;
+  if (threadIdx.x == 0) {
+    s_off = 0;
+    s_drop = 0;
+  }
+  double dacc = 0.0;
+  __syncthreads();
+  for (int kb = 0; kb < e.nch; kb += 256) {
+    const int k = kb + threadIdx.x;
+    int flag = 0;
+    double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
+    int sec = 1;
+    if (k < e.nch) {
+      const int c = e.cbase + k;
+      if (s.mode == RD_EMIT_PLANAR_FE) {
+        double Df = ch.D[c];
+        if (ch.F[c] >= 0.0) Df = -kHuge;
+        dacc = dacc + rd_exp(Df);
+        double rnd = uniform(s.seed, s.stream, (uint64_t)e.step, tag(kTagEmit, 0), (uint64_t)k, 0);
+        if (rd_log(rnd) <= Df) {
+          flag = 1;
+          x = ch.cx[c];
+          y = ch.cy[c];
+          z = 1.0 * Consts::length_scale;
+          work_at(s, wf, x, y, sec);
+        }
+      } else if (s.mode == RD_EMIT_PLANAR_GTF) {
+        if (ch.ok[c] == 1) {
+          flag = 1;
+          x = ch.cx[c];
+          y = ch.cy[c];
+          z = 1.0 * Consts::length_scale;
+          // Get_MB_Velocity (src/mod_velocity.f90:53-68)
+          double sd = __builtin_sqrt(ec.k_b * s.T / ec.m0);
+          double b1, b2, b3, b4;
+          box_muller(s.seed, s.stream, (uint64_t)e.step, tag(kTagVel, 0), (uint64_t)k, 0.0, 0.0,
+                     sd, sd, b1, b2);
+          box_muller(s.seed, s.stream, (uint64_t)e.step, tag(kTagVel, 1), (uint64_t)k, 0.0, 0.0,
+                     sd, sd, b3, b4);
+          vx = b1;
+          vy = b3;
+          vz = __builtin_fabs(b4);
+          work_at(s, wf, x, y, sec);
+        }
+      } else {
+        double rnd = uniform(s.seed, s.stream, (uint64_t)e.step, tag(kTagEmit, 0), (uint64_t)k, 0);
+        if ((ch.F[c] < 0.0) && (rnd <= ch.D[c])) {
+          flag = 1;
+          double nx, ny, nz;
+          tip_normal(tip_surf(s.g), ch.ox[c], ch.oy[c], nx, ny, nz);
+          x = ch.ox[c] + nx * Consts::length_scale;
+          y = ch.oy[c] + ny * Consts::length_scale;
+          z = ch.oz[c] + nz * Consts::length_scale;
+          sec = 1;
+        }
+      }
+    }
+    scan[threadIdx.x] = flag;
+    __syncthreads();
+    // inclusive Hillis-Steele scan
+    for (int w = 1; w < 256; w <<= 1) {
+      int v = (int)threadIdx.x >= w ? scan[threadIdx.x - w] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int total = scan[255];
+    const int rank = scan[threadIdx.x] - flag;
+    if (flag) {
+      const int slot_idx = n0 + s_off + rank;  // index within the simulation
+      if (slot_idx >= s.cap) {
+        atomicAdd(&s_drop, 1);
+      } else {
+        const int gi = s.base + slot_idx;
+        X[gi] = x;
+        Y[gi] = y;
+        Z[gi] = z;
+        PX[gi] = -1.0 * Consts::length_scale;
+        PY[gi] = -1.0 * Consts::length_scale;
+        PZ[gi] = -1.0 * Consts::length_scale;
+        VX[gi] = vx;
+        VY[gi] = vy;
+        VZ[gi] = vz;
+        double ex, ey, ez;
+        if (s.g.kind == RD_GEOM_TIP) {
+          // field_E_Hyperboloid at the emission point
+          double af = s.g.a_foci, sz = s.g.shift_z;
+          double r_p = __builtin_sqrt(x * x + y * y + (z + af - sz) * (z + af - sz));
+          double r_m = __builtin_sqrt(x * x + y * y + (z - af - sz) * (z - af - sz));
+          double xi = (r_p + r_m) / (2.0 * af);
+          double eta = (r_p - r_m) / (2.0 * af);
+          double cc, ss;
+          if ((__builtin_fabs(x) < 1.0e-18) && (__builtin_fabs(y) < 1.0e-18)) {
+            cc = 1.0;
+            ss = 0.0;
+          } else {
+            double rho = __builtin_sqrt(x * x + y * y);
+            cc = x / rho;
+            ss = y / rho;
+          }
+          double pre = s.g.pre_E * 1.0 / (xi * xi - eta * eta);
+          double fac = (__builtin_fabs(xi - 1.0) < 1.0e-6)
+                           ? 0.0
+                           : eta * __builtin_sqrt((xi * xi - 1.0) / (1.0 - eta * eta));
+          ex = -1.0 * pre * fac * cc;
+          ey = -1.0 * pre * fac * ss;
+          ez = pre * xi;
+        } else {
+          ex = 0.0;
+          ey = 0.0;
+          ez = s.g.Ez;
+        }
+        const double qm = qe / me;
+        AX[gi] = A1X[gi] = A2X[gi] = qm * ex;
+        AY[gi] = A1Y[gi] = A2Y[gi] = qm * ey;
+        AZ[gi] = A1Z[gi] = A2Z[gi] = qm * ez;
+        Q[gi] = qe;
+        M[gi] = me;
+        SP[gi] = RD_SPECIES_ELEC;
+        STEP[gi] = e.step;
+        EMIT[gi] = 1;
+        SEC[gi] = sec;
+        ID[gi] = id0 + s_off + rank;
+        ALIVE[gi] = 1;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_off += total;
+    __syncthreads();
+  }
+  dsum[threadIdx.x] = dacc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) dsum[threadIdx.x] = dsum[threadIdx.x] + dsum[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int added = s_off - s_drop;
+    S.n = n0 + added;
+    S.nrID = id0 + added;
+    S.emitted = added;
+    S.dropped += s_drop;
+    if (s.mode == RD_EMIT_PLANAR_FE) S.df_avg = dsum[0] / S.N_sup;
+  }
+}
+
+// ---- Add_Particle from the host ------------------------------------------
+__global__ void k_add(int slot, const SimDesc* desc, SimState* st, int n, const double* __restrict__ in,
+                      int step, int emit, double* X, double* Y, double* Z, double* PX, double* PY,
+                      double* PZ, double* VX, double* VY, double* VZ, double* AX, double* AY,
+                      double* AZ, double* A1X, double* A1Y, double* A1Z, double* A2X, double* A2Y,
+                      double* A2Z, double* Q, double* M, int32_t* SP, int32_t* STEP, int32_t* EMIT,
+                      int32_t* SEC, int32_t* ID, int32_t* ALIVE) {
+  // in: n rows of (x, y, z, vx, vy, vz, species, section)
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const SimDesc& s = desc[slot];
+  const SimState& S = st[slot];
+  if (k >= n) return;
+  const int idx = S.n + k;
+  if (idx >= s.cap) return;
+  const int gi = s.base + idx;
+  const double* r = in + 8 * (size_t)k;
+  double x = r[0], y = r[1], z = r[2];
+  int spc = (int)r[6];
+  double q = (spc == RD_SPECIES_ION) ? 1.0 * Consts::q_0 : -1.0 * Consts::q_0;
+  double m = (spc == RD_SPECIES_ION) ? (28.0134 * Consts::m_u - Consts::m_0) : 1.0 * Consts::m_0;
+  X[gi] = x;
+  Y[gi] = y;
+  Z[gi] = z;
+  PX[gi] = -1.0 * Consts::length_scale;
+  PY[gi] = -1.0 * Consts::length_scale;
+  PZ[gi] = -1.0 * Consts::length_scale;
+  VX[gi] = r[3];
+  VY[gi] = r[4];
+  VZ[gi] = r[5];
+  double ex, ey, ez;
+  if (s.g.kind == RD_GEOM_TIP) {
+    double af = s.g.a_foci, sz = s.g.shift_z;
+    double r_p = __builtin_sqrt(x * x + y * y + (z + af - sz) * (z + af - sz));
+    double r_m = __builtin_sqrt(x * x + y * y + (z - af - sz) * (z - af - sz));
+    double xi = (r_p + r_m) / (2.0 * af);
+    double eta = (r_p - r_m) / (2.0 * af);
+    double cc, ss;
+    if ((__builtin_fabs(x) < 1.0e-18) && (__builtin_fabs(y) < 1.0e-18)) {
+      cc = 1.0;
+      ss = 0.0;
+    } else {
+      double rho = __builtin_sqrt(x * x + y * y);
+      cc = x / rho;
+      ss = y / rho;
+    }
+    double pre = s.g.pre_E * 1.0 / (xi * xi - eta * eta);
+    double fac = (__builtin_fabs(xi - 1.0) < 1.0e-6)
+                     ? 0.0
+                     : eta * __builtin_sqrt((xi * xi - 1.0) / (1.0 - eta * eta));
+    ex = -1.0 * pre * fac * cc;
+    ey = -1.0 * pre * fac * ss;
+    ez = pre * xi;
+  } else {
+    ex = 0.0;
+    ey = 0.0;
+    ez = s.g.Ez;
+  }
+  const double qm = q / m;
+  AX[gi] = A1X[gi] = A2X[gi] = qm * ex;
+  AY[gi] = A1Y[gi] = A2Y[gi] = qm * ey;
+  AZ[gi] = A1Z[gi] = A2Z[gi] = qm * ez;
+  Q[gi] = q;
+  M[gi] = m;
+  SP[gi] = spc;
+  STEP[gi] = step;
+  EMIT[gi] = emit;
+  SEC[gi] = (int)r[7];
+  ID[gi] = S.nrID + k;
+  ALIVE[gi] = 1;
+}
+
+__global__ void k_add_commit(int slot, const SimDesc* desc, SimState* st, int n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  SimState& S = st[slot];
+  int room = desc[slot].cap - S.n;
+  int added = n < room ? n : room;
+  S.n += added;
+  S.nrID += added;
+  S.dropped += n - added;
+}
+
+// ---- Beeman position update + boundary marking ---------------------------
+__device__ __forceinline__ void mark_remove(SimState& S, int32_t* ALIVE, double* Q,
+                                            const int32_t* SP, const int32_t* EMIT, int gi,
+                                            int reason) {
+  if (!ALIVE[gi]) return;
+  ALIVE[gi] = 0;
+  Q[gi] = 0.0;
+  atomicAdd(&S.rem_total, 1);
+  const int spc = SP[gi];
+  if (spc == RD_SPECIES_ELEC) {
+    atomicAdd(&S.rem_elec, 1);
+    if (reason == 1) {
+      atomicAdd(&S.rem_top, 1);
+      if (EMIT[gi] == 1) atomicAdd(&S.rem_top_emit, 1);
+    } else {
+      atomicAdd(&S.rem_bot, 1);
+    }
+  } else {
+    atomicAdd(&S.rem_ion, 1);
+    if (reason == 1)
+      atomicAdd(&S.rem_top, 1);
+    else
+      atomicAdd(&S.rem_bot, 1);
+  }
+}
+
+__global__ void k_position(const Ent* tab, const SimDesc* desc, SimState* st, double* X,
+                           double* Y, double* Z, double* PX, double* PY, double* PZ,
+                           const double* VX, const double* VY, const double* VZ, double* AX,
+                           double* AY, double* AZ, double* A1X, double* A1Y, double* A1Z,
+                           double* A2X, double* A2Y, double* A2Z, double* Q, const int32_t* SP,
+                           const int32_t* EMIT, int32_t* ALIVE) {
+  const Ent e = tab[blockIdx.z];
+  SimState& S = st[e.slot];
+  const int n = S.n;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SimDesc& s = desc[e.slot];
+  const int gi = e.base + i;
+  const double dt = s.dt, dt2 = s.dt2;
+  const double c16 = 1.0 / 6.0;
+  double x = X[gi], y = Y[gi], z = Z[gi];
+  PX[gi] = x;
+  PY[gi] = y;
+  PZ[gi] = z;
+  double ax = AX[gi], ay = AY[gi], az = AZ[gi];
+  double a1x = A1X[gi], a1y = A1Y[gi], a1z = A1Z[gi];
+  x = x + VX[gi] * dt + c16 * (4.0 * ax - a1x) * dt2;
+  y = y + VY[gi] * dt + c16 * (4.0 * ay - a1y) * dt2;
+  z = z + VZ[gi] * dt + c16 * (4.0 * az - a1z) * dt2;
+  X[gi] = x;
+  Y[gi] = y;
+  Z[gi] = z;
+  A2X[gi] = a1x;
+  A2Y[gi] = a1y;
+  A2Z[gi] = a1z;
+  A1X[gi] = ax;
+  A1Y[gi] = ay;
+  A1Z[gi] = az;
+  AX[gi] = 0.0;
+  AY[gi] = 0.0;
+  AZ[gi] = 0.0;
+  // ptr_Check_Boundary
+  if (z < 0.0)
+    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2);
+  else if (z > s.g.box_top)
+    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 1);
+  if (s.g.kind == RD_GEOM_TIP) {
+    double eta = tip_eta(tip_surf(s.g), x, y, z);
+    if (eta < s.g.eta_1) mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2);
+  }
+}
+
+// ---- Beeman velocity update + Ramo current --------------------------------
+// Per block: fixed-order tree sums of 13 values (Ramo per species 3, velocity
+// sums 9, electron count 1) into red[entry][block][16].
+__global__ __launch_bounds__(256) void k_velocity(const Ent* tab, const SimDesc* desc,
+                                                  const SimState* st, const double* X,
+                                                  const double* Y, const double* Z, double* VX,
+                                                  double* VY, double* VZ, const double* AX,
+                                                  const double* AY, const double* AZ,
+                                                  const double* A1X, const double* A1Y,
+                                                  const double* A1Z, const double* A2X,
+                                                  const double* A2Y, const double* A2Z,
+                                                  const double* Q, const int32_t* SP,
+                                                  double* red) {
+  __shared__ double sm[13][256];
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
+  if ((int)(blockIdx.x * blockDim.x) >= max(n, 1) && blockIdx.x > 0) return;
+  const SimDesc& s = desc[e.slot];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[13];
+  for (int k = 0; k < 13; ++k) v[k] = 0.0;
+  if (i < n) {
+    const int gi = e.base + i;
+    const double dt = s.dt;
+    const double c16 = 1.0 / 6.0;
+    double vx = VX[gi] + c16 * (2.0 * AX[gi] + 5.0 * A1X[gi] - A2X[gi]) * dt;
+    double vy = VY[gi] + c16 * (2.0 * AY[gi] + 5.0 * A1Y[gi] - A2Y[gi]) * dt;
+    double vz = VZ[gi] + c16 * (2.0 * AZ[gi] + 5.0 * A1Z[gi] - A2Z[gi]) * dt;
+    VX[gi] = vx;
+    VY[gi] = vy;
+    VZ[gi] = vz;
+    double ux, uy, uz;
+    if (s.g.kind == RD_GEOM_TIP) {
+      double ex, ey, ez;
+      double x = X[gi], y = Y[gi], z = Z[gi];
+      double af = s.g.a_foci, sz = s.g.shift_z;
+      double r_p = __builtin_sqrt(x * x + y * y + (z + af - sz) * (z + af - sz));
+      double r_m = __builtin_sqrt(x * x + y * y + (z - af - sz) * (z - af - sz));
+      double xi = (r_p + r_m) / (2.0 * af);
+      double eta = (r_p - r_m) / (2.0 * af);
+      double cc, ss;
+      if ((__builtin_fabs(x) < 1.0e-18) && (__builtin_fabs(y) < 1.0e-18)) {
+        cc = 1.0;
+        ss = 0.0;
+      } else {
+        double rho = __builtin_sqrt(x * x + y * y);
+        cc = x / rho;
+        ss = y / rho;
+      }
+      double pre = s.g.pre_E * 1.0 / (xi * xi - eta * eta);
+      double fac = (__builtin_fabs(xi - 1.0) < 1.0e-6)
+                       ? 0.0
+                       : eta * __builtin_sqrt((xi * xi - 1.0) / (1.0 - eta * eta));
+      ex = -1.0 * pre * fac * cc;
+      ey = -1.0 * pre * fac * ss;
+      ez = pre * xi;
+      ux = ex * s.g.pre_E_unit / s.g.pre_E;
+      uy = ey * s.g.pre_E_unit / s.g.pre_E;
+      uz = ez * s.g.pre_E_unit / s.g.pre_E;
+    } else {
+      ux = 0.0;
+      uy = 0.0;
+      uz = -1.0 / s.g.d;
+    }
+    double EzV = vx * ux + vy * uy + vz * uz;
+    const int spc = SP[gi];
+    const double cur = Q[gi] * EzV;
+    if (spc >= 1 && spc <= 3) v[spc - 1] = cur;
+    v[3] = vx;
+    v[4] = vy;
+    v[5] = vz;
+    if (spc == RD_SPECIES_ELEC) {
+      v[6] = vx;
+      v[7] = vy;
+      v[8] = vz;
+      v[12] = 1.0;
+    } else if (spc == RD_SPECIES_ION) {
+      v[9] = vx;
+      v[10] = vy;
+      v[11] = vz;
+    }
+  }
+  for (int k = 0; k < 13; ++k) sm[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 13; ++k) sm[k][threadIdx.x] = sm[k][threadIdx.x] + sm[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 13)
+    red[((size_t)blockIdx.z * e.nblk + blockIdx.x) * 16 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void k_velocity_finish(const Ent* tab, SimState* st, const double* red) {
+  const Ent e = tab[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  SimState& S = st[e.slot];
+  const int nb = max(1, (S.n + 255) / 256);
+  double acc[13];
+  for (int k = 0; k < 13; ++k) acc[k] = 0.0;
+  for (int b = 0; b < nb && b < e.nblk; ++b)
+    for (int k = 0; k < 13; ++k) acc[k] = acc[k] + red[((size_t)blockIdx.x * e.nblk + b) * 16 + k];
+  for (int k = 0; k < 3; ++k) S.ramo[k] = acc[k];
+  for (int k = 0; k < 9; ++k) S.vsum[k] = acc[3 + k];
+  S.n_elec = (int)acc[12];  // electrons updated this step
+}
+
+// ---- order-preserving compaction (one block of 1024 per simulation) ------
+__global__ __launch_bounds__(1024) void k_compact(
+    const Ent* tab, const SimDesc* desc, SimState* st, int32_t* life, double* X, double* Y,
+    double* Z, double* PX, double* PY, double* PZ, double* VX, double* VY, double* VZ, double* AX,
+    double* AY, double* AZ, double* A1X, double* A1Y, double* A1Z, double* A2X, double* A2Y,
+    double* A2Z, double* Q, double* M, int32_t* SP, int32_t* STEP, int32_t* EMIT, int32_t* SEC,
+    int32_t* ID, int32_t* ALIVE) {
+  __shared__ int scan[1024];
+  __shared__ int s_off;
+  const Ent e = tab[blockIdx.x];
+  SimState& S = st[e.slot];
+  if (S.rem_total == 0) return;
+  const int n = S.n;
+  const int base = e.base;
+  if (threadIdx.x == 0) s_off = 0;
+  __syncthreads();
+  for (int cb = 0; cb < n; cb += 1024) {
+    const int i = cb + threadIdx.x;
+    int flag = 0;
+    double r[21];
+    int ri[5];
+    if (i < n) {
+      const int gi = base + i;
+      flag = ALIVE[gi] ? 1 : 0;
+      if (flag) {
+        r[0] = X[gi]; r[1] = Y[gi]; r[2] = Z[gi];
+        r[3] = PX[gi]; r[4] = PY[gi]; r[5] = PZ[gi];
+        r[6] = VX[gi]; r[7] = VY[gi]; r[8] = VZ[gi];
+        r[9] = AX[gi]; r[10] = AY[gi]; r[11] = AZ[gi];
+        r[12] = A1X[gi]; r[13] = A1Y[gi]; r[14] = A1Z[gi];
+        r[15] = A2X[gi]; r[16] = A2Y[gi]; r[17] = A2Z[gi];
+        r[18] = Q[gi]; r[19] = M[gi];
+        ri[0] = SP[gi]; ri[1] = STEP[gi]; ri[2] = EMIT[gi]; ri[3] = SEC[gi]; ri[4] = ID[gi];
+      } else {
+        // record_lifetime (src/mod_pair.F90:1135-1160)
+        int lt = e.step - STEP[gi];
+        if (lt <= 0) lt = 1;
+        if (lt > kLife) lt = kLife;
+        int spc = SP[gi];
+        if (spc >= 1 && spc <= 2) atomicAdd(&life[((size_t)e.slot * 2 + (spc - 1)) * kLife + (lt - 1)], 1);
+      }
+    }
+    scan[threadIdx.x] = flag;
+    __syncthreads();
+    for (int w = 1; w < 1024; w <<= 1) {
+      int v = (int)threadIdx.x >= w ? scan[threadIdx.x - w] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int total = scan[1023];
+    const int rank = scan[threadIdx.x] - flag;
+    const int off = s_off;
+    __syncthreads();  // every read of this chunk is done before any write
+    if (flag) {
+      const int d = base + off + rank;
+      X[d] = r[0]; Y[d] = r[1]; Z[d] = r[2];
+      PX[d] = r[3]; PY[d] = r[4]; PZ[d] = r[5];
+      VX[d] = r[6]; VY[d] = r[7]; VZ[d] = r[8];
+      AX[d] = r[9]; AY[d] = r[10]; AZ[d] = r[11];
+      A1X[d] = r[12]; A1Y[d] = r[13]; A1Z[d] = r[14];
+      A2X[d] = r[15]; A2Y[d] = r[16]; A2Z[d] = r[17];
+      Q[d] = r[18]; M[d] = r[19];
+      SP[d] = ri[0]; STEP[d] = ri[1]; EMIT[d] = ri[2]; SEC[d] = ri[3]; ID[d] = ri[4];
+      ALIVE[d] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_off = off + total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S.n = s_off;
+}
+
+// ------------------------------------------------------------------------
+// Host orchestration.
+#define RD_ARRAYS                                                                          \
+  c->x, c->y, c->z, c->px, c->py, c->pz, c->vx, c->vy, c->vz, c->ax, c->ay, c->az, c->a1x, \
+      c->a1y, c->a1z, c->a2x, c->a2y, c->a2z, c->q, c->m, c->species, c->pstep, c->emitter, \
+      c->section, c->id, c->alive
+
+int sync_state(rd_ctx* c) {
+  int hi = 0;
+  for (int s = 0; s < c->nslots_max; ++s)
+    if (c->slots[s].active) hi = s + 1;
+  if (hi > 0)
+    RD_HIP(hipMemcpyAsync(c->h_state, c->d_state, sizeof(SimState) * hi, hipMemcpyDeviceToHost,
+                          c->stream));
+  RD_HIP(hipStreamSynchronize(c->stream));
+  c->tab_used = 0;  // every table copy has completed
+  for (int s = 0; s < c->nslots_max; ++s)
+    if (c->slots[s].active) c->slots[s].n = c->h_state[s].n;
+  return RD_OK;
+}
+
+int engine_add(rd_ctx* c, int slot, int n, const double* pos, const double* vel,
+               const int32_t* species, int step, int emit, const int32_t* sec) {
+  if (n <= 0) return RD_OK;
+  std::vector<double> rows((size_t)n * 8);
+  for (int k = 0; k < n; ++k) {
+    double* r = &rows[(size_t)k * 8];
+    r[0] = pos[3 * k];
+    r[1] = pos[3 * k + 1];
+    r[2] = pos[3 * k + 2];
+    r[3] = vel ? vel[3 * k] : 0.0;
+    r[4] = vel ? vel[3 * k + 1] : 0.0;
+    r[5] = vel ? vel[3 * k + 2] : 0.0;
+    r[6] = species ? species[k] : RD_SPECIES_ELEC;
+    r[7] = sec ? sec[k] : 1;
+  }
+  int r = ensure(c, c->stage, sizeof(double) * rows.size());
+  if (r) return r;
+  RD_HIP(hipMemcpyAsync(c->stage.p, rows.data(), sizeof(double) * rows.size(),
+                        hipMemcpyHostToDevice, c->stream));
+  k_add<<<(n + 255) / 256, 256, 0, c->stream>>>(slot, c->d_desc, c->d_state, n,
+                                                 (const double*)c->stage.p, step, emit, RD_ARRAYS);
+  k_add_commit<<<1, 1, 0, c->stream>>>(slot, c->d_desc, c->d_state, n);
+  RD_HIP(hipGetLastError());
+  return sync_state(c);
+}
+
+static int max_n_ub(const std::vector<Ent>& ents) {
+  int m = 0;
+  for (const Ent& e : ents) m = std::max(m, e.n_ub);
+  return m;
+}
+
+int engine_position(rd_ctx* c, const std::vector<Ent>& ents) {
+  if (ents.empty()) return RD_OK;
+  int mx = max_n_ub(ents);
+  if (mx == 0) return RD_OK;
+  Ent* dt;
+  int r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  k_position<<<dim3((mx + 255) / 256, 1, ents.size()), 256, 0, c->stream>>>(
+      dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
+      c->ax, c->ay, c->az, c->a1x, c->a1y, c->a1z, c->a2x, c->a2y, c->a2z, c->q, c->species,
+      c->emitter, c->alive);
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
+int engine_velocity(rd_ctx* c, const std::vector<Ent>& in) {
+  if (in.empty()) return RD_OK;
+  std::vector<Ent> ents = in;
+  int mx = std::max(1, max_n_ub(ents));
+  int nb = (mx + 255) / 256;
+  for (Ent& e : ents) e.nblk = nb;
+  int r = ensure(c, c->red, sizeof(double) * 16 * (size_t)nb * ents.size());
+  if (r) return r;
+  Ent* dt;
+  r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  k_velocity<<<dim3(nb, 1, ents.size()), 256, 0, c->stream>>>(
+      dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az,
+      c->a1x, c->a1y, c->a1z, c->a2x, c->a2y, c->a2z, c->q, c->species, (double*)c->red.p);
+  k_velocity_finish<<<ents.size(), 1, 0, c->stream>>>(dt, c->d_state, (const double*)c->red.p);
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
+int engine_compact(rd_ctx* c, const std::vector<Ent>& ents) {
+  if (ents.empty()) return RD_OK;
+  Ent* dt;
+  int r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  k_compact<<<ents.size(), 1024, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_life, RD_ARRAYS);
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
+static int reset_counters(rd_ctx* c, const std::vector<Ent>& ents) {
+  Ent* dt;
+  int r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  k_reset_counters<<<ents.size(), 64, 0, c->stream>>>(dt, c->d_state);
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
+// Emission for the entries whose simulation emits at this step.  On return
+// the device state holds the new particle counts; n_round[k] receives the
+// number of chains of entry k (an upper bound of the electrons added).
+static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision,
+                           std::vector<int>& n_round) {
+  n_round.assign(all.size(), 0);
+  std::vector<Ent> q;
+  std::vector<int> qidx;
+  int mtot = 0;
+  for (size_t k = 0; k < all.size(); ++k) {
+    const Ent& e0 = all[k];
+    const SimDesc& d = c->slots[e0.slot].desc;
+    if (d.mode == RD_EMIT_NONE || !(d.delay < e0.step)) continue;
+    Ent e = e0;
+    e.pbase = mtot;
+    e.m = (d.mode == RD_EMIT_TIP_FE) ? 100 * 100 : d.quad * d.quad;
+    mtot += e.m;
+    q.push_back(e);
+    qidx.push_back((int)k);
+  }
+  if (q.empty()) return RD_OK;
+  int r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)mtot);
+  if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)mtot);
+  if (!r) r = ensure(c, c->stage, sizeof(double) * (size_t)mtot);
+  if (r) return r;
+  double* ptx = (double*)c->pts.p;
+  Ent* dq;
+  r = upload_tab(c, q, &dq);
+  if (r) return r;
+  int maxm = 0;
+  for (const Ent& e : q) maxm = std::max(maxm, e.m);
+  k_quad_points<<<dim3((maxm + 255) / 256, 1, q.size()), 256, 0, c->stream>>>(
+      dq, c->d_desc, ptx, ptx + mtot, ptx + 2 * mtot);
+  r = launch_field(c, q, mtot, precision, nullptr, nullptr);
+  if (r) return r;
+  double* f = (double*)c->fld.p;
+  k_quad_reduce<<<q.size(), 256, 0, c->stream>>>(dq, c->d_desc, c->d_state, c->d_wf, ptx,
+                                                  ptx + mtot, ptx + 2 * mtot, f, f + mtot,
+                                                  f + 2 * mtot, (double*)c->stage.p);
+  RD_HIP(hipGetLastError());
+  r = sync_state(c);
+  if (r) return r;
+  // chains
+  std::vector<Ent> ch;
+  int C = 0, maxc = 0, maxndim = 0;
+  for (size_t k = 0; k < q.size(); ++k) {
+    Ent e = q[k];
+    int nr = c->h_state[e.slot].n_round;
+    n_round[qidx[k]] = nr;
+    if (nr <= 0) continue;
+    e.cbase = C;
+    e.nch = nr;
+    e.pbase = C;
+    e.m = nr;
+    C += nr;
+    maxc = std::max(maxc, nr);
+    maxndim = std::max(maxndim, c->slots[e.slot].desc.ndim);
+    ch.push_back(e);
+  }
+  if (C == 0) return RD_OK;
+  r = ensure(c, c->chains, chain_bytes(C));
+  if (!r) r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)C);
+  if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)C);
+  if (r) return r;
+  ptx = (double*)c->pts.p;
+  f = (double*)c->fld.p;
+  double *pty = ptx + C, *ptz = ptx + 2 * (size_t)C;
+  double *fx = f, *fy = f + C, *fz = f + 2 * (size_t)C;
+  Chains V = chain_view(c, C);
+  Ent* dc;
+  r = upload_tab(c, ch, &dc);
+  if (r) return r;
+  dim3 cg((maxc + 255) / 256, 1, ch.size());
+  k_mh_init<<<cg, 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, V);
+  // start sites: rounds until every chain has F < 0 (at most 10000)
+  for (int round = 1;; ++round) {
+    const int last = round >= 10001;
+    k_mh_start<<<cg, 256, 0, c->stream>>>(dc, c->d_desc, V, round, ptx, pty, ptz);
+    r = launch_field(c, ch, C, precision, nullptr, nullptr);
+    if (r) return r;
+    k_mh_start_check<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V,
+                                                        round, last, ptx, pty, ptz, fx, fy, fz);
+    RD_HIP(hipGetLastError());
+    r = sync_state(c);
+    if (r) return r;
+    int pend = 0;
+    for (const Ent& e : ch) pend += c->h_state[e.slot].pending;
+    if (pend == 0 || last) break;
+  }
+  for (int it = 1; it <= maxndim; ++it) {
+    k_mh_propose<<<cg, 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, V, it, ptx, pty, ptz);
+    r = launch_field(c, ch, C, precision, nullptr, nullptr);
+    if (r) return r;
+    k_mh_accept<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V, it,
+                                                   ptx, pty, ptz, fx, fy, fz);
+  }
+  k_mh_finish<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V);
+  k_emit<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V, RD_ARRAYS);
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
+static void fill_record(rd_ctx* c, int slot, int step, rd_step_record* rec) {
+  const SimState& S = c->h_state[slot];
+  std::memset(rec, 0, sizeof(*rec));
+  rec->step = step;
+  rec->n = S.n;
+  rec->emitted = S.emitted;
+  rec->n_round = S.n_round;
+  rec->absorbed = S.rem_total;
+  rec->absorbed_top = S.rem_top;
+  rec->absorbed_bot = S.rem_bot;
+  rec->dropped = S.dropped;
+  rec->start_rounds = S.start_rounds;
+  rec->n_sup = S.N_sup;
+  rec->ramo_current = S.ramo[0] + S.ramo[1] + S.ramo[2];
+  rec->mh_std = S.MH_std;
+  rec->a_rate = S.a_rate;
+  rec->df_avg = S.df_avg;
+  for (int k = 0; k < 3; ++k) {
+    rec->f_avg[k] = S.F_avg[k];
+    rec->ramo[k] = S.ramo[k];
+    rec->avg_elec_vel[k] = S.n_elec > 0 ? S.vsum[3 + k] / S.n_elec : 0.0;
+  }
+}
+
+// One main-loop step (emission, position/force/velocity, removal) of the
+// given slots.  phases: bit 0 emission, bit 1 update, bit 2 removal.
+int engine_phases(rd_ctx* c, const std::vector<int>& slots, int precision, int phases,
+                  const int* steps, rd_step_record* recs) {
+  std::vector<Ent> ents;
+  for (size_t k = 0; k < slots.size(); ++k) {
+    Ent e{};
+    e.slot = slots[k];
+    e.base = c->slots[e.slot].desc.base;
+    e.n_ub = c->slots[e.slot].n;
+    e.step = steps[k];
+    ents.push_back(e);
+  }
+  if (ents.empty()) return RD_OK;
+  int r;
+  if (phases & 1) {
+    r = reset_counters(c, ents);
+    if (r) return r;
+  }
+  std::vector<int> nr;
+  if (phases & 1) {
+    r = engine_emission(c, ents, precision, nr);
+    if (r) return r;
+    for (size_t k = 0; k < ents.size(); ++k) {
+      const SimDesc& d = c->slots[ents[k].slot].desc;
+      ents[k].n_ub = std::min(d.cap, ents[k].n_ub + nr[k]);
+    }
+  }
+  if (phases & 2) {
+    if (!(phases & 1)) {
+      // counters of the removal phase start at zero in a stand-alone update
+      r = reset_counters(c, ents);
+      if (r) return r;
+    }
+    r = engine_position(c, ents);
+    if (!r) r = launch_accel(c, ents, precision, nullptr, nullptr);
+    if (!r) r = engine_velocity(c, ents);
+    if (r) return r;
+  }
+  if (phases & 4) {
+    r = engine_compact(c, ents);
+    if (r) return r;
+  }
+  r = sync_state(c);
+  if (r) return r;
+  if (recs)
+    for (size_t k = 0; k < ents.size(); ++k) fill_record(c, ents[k].slot, ents[k].step, &recs[k]);
+  return RD_OK;
+}
+
+}  // namespace rd

@@ -1,11 +1,19 @@
-// Internal definitions of librumdeed.so: the per-context device state and
-// the kernel launch helpers shared by the translation units.
+// Internal definitions of librumdeed.so: simulation slots, device state and
+// the launch tables shared by the translation units.
+//
+// A context owns one set of particle arrays in HBM (structure of arrays,
+// capacity `cap`) and up to kMaxSlots independent simulations ("slots"),
+// each holding a contiguous index range [base, base + slot_cap).  Every
+// kernel of the step is launched once for all active slots: blockIdx.z (or
+// the block index) selects a launch-table entry, so many small simulations
+// share each launch.  A single simulation is a context with one slot.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/rumdeed.h"
 
@@ -34,20 +42,75 @@ double m_N2p();
 struct DevGeom {
   int kind, ic, nic, pad;
   double d, Ez, es, dfc, ktip;
+  double box_top;  // removal height (box_dim(3))
   // hyperboloid tip (Init_Emission_Tip, src/mod_emission_tip.f90:105-125)
   double a_foci, shift_z, pre_E, pre_E_unit, z0, R, eta_1, h_tip, d_tip, R_base, max_xi, theta;
-  // mixed-precision kernels work in nanometres
-  float d_nm, pad2;
 };
 
-// Canonical FP64 summation: chunks of kChunk sources summed sequentially,
-// chunk partials summed sequentially (the CPU restatement uses the same).
-constexpr int kChunk = 256;
+// Read-only description of one simulation slot.
+struct SimDesc {
+  DevGeom g;
+  double dt, dt2, dt_q0;
+  int mode;   // RD_EMIT_*
+  int delay;  // emitters_delay
+  double ex0, ey0, elx, ely;  // planar emitter rectangle
+  double T;                   // cathode temperature (GTF)
+  double w_tip;               // tip work function
+  int wf_nx, wf_ny, wf_off;   // checkerboard table in ctx->wf
+  int quad;                   // quadrature points per side (planar)
+  int base, cap;
+  int ndim;                   // Metropolis-Hastings jumps per chain
+  int pad;
+  uint64_t seed, stream;
+};
+
+// Mutable per-slot state, resident in HBM.
+struct SimState {
+  int n;          // live particles
+  int nrID;       // next particle id
+  int n_round;    // chains of the current step
+  int emitted;    // electrons emitted this step
+  int rem_total, rem_top, rem_bot, rem_elec, rem_ion, rem_top_emit;
+  int dropped;    // refused for capacity (cumulative)
+  int pending;    // chains still without a start position
+  int it_a, it_r; // accept / reject count of the current jump
+  int start_rounds;
+  int n_elec;     // electrons in the last velocity update
+  double residual;  // rounding residual of the supply (planar FE)
+  double MH_std, a_rate;
+  double N_sup, df_avg;
+  double F_avg[3];
+  double ramo[3];   // Ramo current per species (A)
+  double vsum[9];   // velocity sums: all, electrons, ions
+};
+
+// One launch-table entry (per active slot and launch).
+struct Ent {
+  int slot, base, n_ub, nchunks_ub;
+  int cps, segs;
+  long long part_off;
+  int pbase, m;   // field points [pbase, pbase + m)
+  int cbase, nch; // MH chains [cbase, cbase + nch)
+  int step, nblk; // time step; per-slot block count (reductions)
+};
+
+constexpr int kChunk = 256;  // canonical FP64 summation chunk
 constexpr int kBlock = 256;
+constexpr int kMaxSlots = 4096;
+constexpr int kLife = 1000;  // MAX_LIFE_TIME
 
 struct Buf {
   void* p = nullptr;
   size_t bytes = 0;
+};
+
+struct Slot {
+  bool active = false;
+  SimDesc desc{};
+  int n = 0;         // host mirror of state.n after the last sync
+  int step = 0;      // last completed step
+  rd_geometry geom{};
+  bool geom_set = false;
 };
 
 }  // namespace rd
@@ -56,11 +119,8 @@ struct rd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int64_t cap = 0;
-  int32_t n = 0;
-  bool geom_set = false;
-  rd_geometry geom_in{};
-  rd::DevGeom g{};
-  double dt = 0.0;
+  int nslots_max = 1;
+  std::vector<rd::Slot> slots;
 
   // particle state, structure of arrays in HBM
   double *x = nullptr, *y = nullptr, *z = nullptr;
@@ -70,19 +130,23 @@ struct rd_ctx {
   double *a1x = nullptr, *a1y = nullptr, *a1z = nullptr;  // previous
   double *a2x = nullptr, *a2y = nullptr, *a2z = nullptr;  // previous-previous
   double *q = nullptr, *m = nullptr;
-  int32_t *species = nullptr, *step = nullptr, *emitter = nullptr, *section = nullptr,
-          *id = nullptr;
+  int32_t *species = nullptr, *pstep = nullptr, *emitter = nullptr, *section = nullptr,
+          *id = nullptr, *alive = nullptr;
+
+  rd::SimDesc* d_desc = nullptr;
+  rd::SimState* d_state = nullptr;
+  rd::SimState* h_state = nullptr;  // pinned mirror
+  int32_t* d_life = nullptr;        // [slot][species 1..2][kLife]
+  double* d_wf = nullptr;           // work-function tables
+  int wf_used = 0, wf_cap = 0;
 
   // work buffers (grown on demand, never shrunk)
-  rd::Buf part;      // partial sums of the pair kernels
-  rd::Buf pack_hi;   // mixed-precision packed sources
-  rd::Buf pack_lo;
-  rd::Buf pack_im;   // tip image points (FP64 or packed)
-  rd::Buf pts;       // field points (SoA x,y,z)
-  rd::Buf fld;       // field results (SoA)
-  rd::Buf pts_hi, pts_lo;
-  rd::Buf stage;     // host<->device staging (AoS)
-  rd::Buf img64;     // tip image data FP64: dis, xim, yim, zim
+  rd::Buf part, pack_hi, pack_lo, pts, fld, pts_hi, pts_lo, stage, img64, tab, red, chains;
+  rd::Ent* h_tab = nullptr;  // pinned staging of launch tables (ring)
+  int tab_used = 0;          // ring entries used since the last stream synchronisation
+  int tab_last = -1;         // ring index of the last upload
+  size_t tab_last_n = 0;
+  int cur = 0;               // slot the single-simulation entry points act on
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
@@ -92,16 +156,33 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 int ensure(rd_ctx* c, Buf& b, size_t bytes);
+int build_geom(const rd_geometry* in, DevGeom* g);
 
-// Pair kernels (rd_pair.hip).  All launches go to c->stream.
-int launch_accel(rd_ctx* c, int precision, hipEvent_t k0, hipEvent_t k1);
-int launch_field(rd_ctx* c, int m, int precision, hipEvent_t k0, hipEvent_t k1);
+// Pair kernels (rd_pair.hip).  `tab` is a device launch table of `ne`
+// entries; all launches go to c->stream.
+// Accelerations of the particles of every entry (n from the slot state).
+int launch_accel(rd_ctx* c, const std::vector<Ent>& ents, int precision, hipEvent_t k0,
+                 hipEvent_t k1);
+// Field at the points [pbase, pbase + m) of every entry from its particles,
+// points and results SoA in c->pts / c->fld (stride `mtot`).
+int launch_field(rd_ctx* c, const std::vector<Ent>& ents, int mtot, int precision, hipEvent_t k0,
+                 hipEvent_t k1);
+int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab);
 int launch_grid_points(rd_ctx* c, double x0, double y0, double lx, double ly, int nx, int ny,
                        double zz);
+
+// Engine (rd_engine.hip).
+int engine_step(rd_ctx* c, int precision, rd_step_record* rec_per_slot);
+int engine_add(rd_ctx* c, int slot, int n, const double* pos, const double* vel,
+               const int32_t* species, int step, int emit, const int32_t* sec);
+int engine_position(rd_ctx* c, const std::vector<Ent>& ents);
+int engine_velocity(rd_ctx* c, const std::vector<Ent>& ents);
+int engine_compact(rd_ctx* c, const std::vector<Ent>& ents);
+int sync_state(rd_ctx* c);
 }  // namespace rd
 
-#define RD_HIP(call)                                    \
-  do {                                                  \
-    hipError_t _e = (call);                             \
+#define RD_HIP(call)                                      \
+  do {                                                    \
+    hipError_t _e = (call);                               \
     if (_e != hipSuccess) return rd::hip_fail(_e, #call); \
   } while (0)

@@ -1,4 +1,4 @@
-// All-pairs Coulomb force, image charges and surface field for gfx950.
+// All-pairs Coulomb force, image charges and field at points for gfx950.
 //
 // Reference: Calculate_Acceleration_Particles_ACC (src/mod_verlet.F90:1217-1429),
 // Calc_Field_at / Calc_Field_at_Batch (:1466-1529, :1635-1911), the planar image
@@ -7,11 +7,13 @@
 //
 // Shape: one lane per target (particle or field point), 256-lane workgroups;
 // sources are staged through LDS 256 at a time and read as wavefront-uniform
-// broadcasts.  The source range is split over gridDim.y segments so that a
-// launch has enough workgroups to fill 256 CUs; every (segment, target)
-// writes FP64 partial sums and a finishing kernel adds them in a fixed order.
-// The pair work is vector-ALU arithmetic (square roots, reciprocals, FMAs),
-// not a contraction, so nothing here uses MFMA.
+// broadcasts.  blockIdx.z selects a simulation (launch-table entry), so one
+// launch serves every simulation of a pool; the source range of a simulation
+// is split over gridDim.y segments so that a launch has enough workgroups to
+// fill 256 CUs.  Every (segment, target) writes FP64 partial sums and a
+// finishing kernel adds them in a fixed order.  The pair work is vector-ALU
+// arithmetic (square roots, reciprocals, FMAs), not a contraction, so
+// nothing here uses MFMA.
 //
 // FP64 kernels: operand-for-operand the reference expressions, with the
 //   partials formed per 256-source chunk and added chunk by chunk, so the
@@ -25,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "rd_internal.h"
 
@@ -106,13 +109,26 @@ __device__ __host__ __forceinline__ void tip_vacuum64(const DevGeom& g, double x
   fz = pre * xi;
 }
 
+// Vacuum field of either geometry.
+__device__ __forceinline__ void vacuum64(const DevGeom& g, double x, double y, double z,
+                                         double& fx, double& fy, double& fz) {
+  if (g.kind == RD_GEOM_TIP) {
+    tip_vacuum64(g, x, y, z, fx, fy, fz);
+  } else {
+    fx = 0.0;
+    fy = 0.0;
+    fz = g.Ez;
+  }
+}
+
 // Sphere image point of the particle at a (acc_tip_image_point.inc:14-18).
 __device__ __forceinline__ void tip_image_point64(double x_a, double y_a, double z_a, double z0,
                                                   double R, double& dis_a, double& x_im,
                                                   double& y_im, double& z_im) {
   double dz = z_a - z0;
   double da = __builtin_sqrt(x_a * x_a + y_a * y_a + dz * dz);
-  double zi = z0 + R * R / (__builtin_sqrt(1 + x_a * x_a / (dz * dz) + y_a * y_a / (dz * dz)) * da);
+  double zi =
+      z0 + R * R / (__builtin_sqrt(1 + x_a * x_a / (dz * dz) + y_a * y_a / (dz * dz)) * da);
   z_im = zi;
   x_im = (zi - z0) * x_a / dz;
   y_im = (zi - z0) * y_a / dz;
@@ -135,90 +151,67 @@ __device__ __forceinline__ void tip_ic_force64(double x_a, double y_a, double z_
   icz = kt * ((z_a - z_b) / tda - (R * (z_im - z_b)) / (dis_a * tdb));
 }
 
+// Per-block view of a launch-table entry.  Returns false (for the whole
+// block) when the block lies beyond the slot's live particles.
+struct View {
+  Ent e;
+  int n;
+  int nchunks;
+};
+__device__ __forceinline__ bool view(const Ent* tab, const SimState* st, View& v, int targets) {
+  v.e = tab[blockIdx.z];
+  v.n = st[v.e.slot].n;
+  v.nchunks = (v.n + kChunk - 1) / kChunk;
+  if ((int)(blockIdx.x * kBlock) >= targets) return false;
+  return true;
+}
+
 // Image points of every particle (they depend on the particle alone).
-__global__ void k_tip_images64(int n, const double* __restrict__ x, const double* __restrict__ y,
-                               const double* __restrict__ z, DevGeom g, double* __restrict__ img) {
+__global__ void k_tip_images64(const Ent* tab, const SimDesc* desc, const SimState* st,
+                               const double* __restrict__ X, const double* __restrict__ Y,
+                               const double* __restrict__ Z, double* __restrict__ img) {
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const DevGeom& g = desc[e.slot].g;
+  if (g.kind != RD_GEOM_TIP || !g.ic) return;
+  const int gi = e.base + i;
   double da, xi, yi, zi;
-  tip_image_point64(x[i], y[i], z[i], g.z0, g.R, da, xi, yi, zi);
-  img[i] = da;
-  img[n + i] = xi;
-  img[2 * n + i] = yi;
-  img[3 * n + i] = zi;
+  tip_image_point64(X[gi], Y[gi], Z[gi], g.z0, g.R, da, xi, yi, zi);
+  img[4 * (size_t)gi] = da;
+  img[4 * (size_t)gi + 1] = xi;
+  img[4 * (size_t)gi + 2] = yi;
+  img[4 * (size_t)gi + 3] = zi;
 }
 
 // ------------------------------------------------------------------------
-// FP64 accelerations, planar.  part layout: [3][nchunks][n].
-__global__ __launch_bounds__(kBlock) void k_accel_planar64(
-    int n, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const double* __restrict__ q, DevGeom g, int cps, int nchunks,
-    double* __restrict__ part) {
+// FP64 accelerations.  part layout per entry: [3][nchunks_ub][n_ub].
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, const SimDesc* desc,
+                                                    const SimState* st,
+                                                    const double* __restrict__ X,
+                                                    const double* __restrict__ Y,
+                                                    const double* __restrict__ Z,
+                                                    const double* __restrict__ Q,
+                                                    const double* __restrict__ img,
+                                                    double* __restrict__ part) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
-  const int t = threadIdx.x;
-  const int i = blockIdx.x * kBlock + t;
-  const bool act = i < n;
-  double x1 = 0, y1 = 0, z1 = 0, qd1 = 0;
-  if (act) {
-    x1 = x[i];
-    y1 = y[i];
-    z1 = z[i];
-    qd1 = q[i] * g.dfc;
-  }
-  const int c0 = blockIdx.y * cps;
-  const int c1 = min(nchunks, c0 + cps);
-  for (int c = c0; c < c1; ++c) {
-    const int jb = c * kChunk;
-    __syncthreads();
-    if (jb + t < n) {
-      sx[t] = x[jb + t];
-      sy[t] = y[jb + t];
-      sz[t] = z[jb + t];
-      sq[t] = q[jb + t];
-    }
-    __syncthreads();
-    const int cnt = min(kChunk, n - jb);
-    double px = 0.0, py = 0.0, pz = 0.0;
-    if (act) {
-      for (int jj = 0; jj < cnt; ++jj) {
-        const int j = jb + jj;
-        if (j == i) continue;
-        double x2 = sx[jj], y2 = sy[jj], z2 = sz[jj];
-        double pre = qd1 * sq[jj];
-        double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
-        double dxy2 = dx * dx + dy * dy;
-        double r = __builtin_sqrt(dxy2 + dz * dz) + g.es;
-        double inv_r3 = 1.0 / (r * r * r);
-        double axx = pre * inv_r3 * dx;
-        double ayy = pre * inv_r3 * dy;
-        double azz = pre * inv_r3 * dz;
-        if (g.ic) {
-          double za = j > i ? z1 : z2;
-          double zb = j > i ? z2 : z1;
-          double icx, icy, icz;
-          ic_planar64(za, zb, dx, dy, dxy2, g.nic, g.d, g.es, icx, icy, icz);
-          axx = axx + pre * icx;
-          ayy = ayy + pre * icy;
-          azz = azz + pre * icz;
-        }
-        px = px + axx;
-        py = py + ayy;
-        pz = pz + azz;
-      }
-      part[(size_t)(0 * nchunks + c) * n + i] = px;
-      part[(size_t)(1 * nchunks + c) * n + i] = py;
-      part[(size_t)(2 * nchunks + c) * n + i] = pz;
-    }
-  }
-}
-
-// FP64 accelerations, hyperboloid tip (image data from k_tip_images64).
-__global__ __launch_bounds__(kBlock) void k_accel_tip64(
-    int n, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const double* __restrict__ q, const double* __restrict__ img,
-    DevGeom g, int cps, int nchunks, double* __restrict__ part) {
-  __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
-  __shared__ double sd[kChunk], sxi[kChunk], syi[kChunk], szi[kChunk];
+  __shared__ double sd[KIND == RD_GEOM_TIP ? kChunk : 1], sxi[KIND == RD_GEOM_TIP ? kChunk : 1],
+      syi[KIND == RD_GEOM_TIP ? kChunk : 1], szi[KIND == RD_GEOM_TIP ? kChunk : 1];
+  View v;
+  if (!view(tab, st, v, st[tab[blockIdx.z].slot].n)) return;
+  const int c0 = blockIdx.y * v.e.cps;
+  if (c0 >= v.nchunks) return;
+  const int c1 = min(v.nchunks, c0 + v.e.cps);
+  const DevGeom g = desc[v.e.slot].g;
+  if (g.kind != KIND) return;
+  const int n = v.n, base = v.e.base;
+  const double* x = X + base;
+  const double* y = Y + base;
+  const double* z = Z + base;
+  const double* q = Q + base;
+  double* P = part + v.e.part_off;
   const int t = threadIdx.x;
   const int i = blockIdx.x * kBlock + t;
   const bool act = i < n;
@@ -228,15 +221,13 @@ __global__ __launch_bounds__(kBlock) void k_accel_tip64(
     y1 = y[i];
     z1 = z[i];
     qd1 = q[i] * g.dfc;
-    if (g.ic) {
-      d1 = img[i];
-      xi1 = img[n + i];
-      yi1 = img[2 * n + i];
-      zi1 = img[3 * n + i];
+    if (KIND == RD_GEOM_TIP && g.ic) {
+      d1 = img[4 * (size_t)(base + i)];
+      xi1 = img[4 * (size_t)(base + i) + 1];
+      yi1 = img[4 * (size_t)(base + i) + 2];
+      zi1 = img[4 * (size_t)(base + i) + 3];
     }
   }
-  const int c0 = blockIdx.y * cps;
-  const int c1 = min(nchunks, c0 + cps);
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
@@ -245,11 +236,12 @@ __global__ __launch_bounds__(kBlock) void k_accel_tip64(
       sy[t] = y[jb + t];
       sz[t] = z[jb + t];
       sq[t] = q[jb + t];
-      if (g.ic) {
-        sd[t] = img[jb + t];
-        sxi[t] = img[n + jb + t];
-        syi[t] = img[2 * n + jb + t];
-        szi[t] = img[3 * n + jb + t];
+      if (KIND == RD_GEOM_TIP && g.ic) {
+        const size_t gj = 4 * (size_t)(base + jb + t);
+        sd[t] = img[gj];
+        sxi[t] = img[gj + 1];
+        syi[t] = img[gj + 2];
+        szi[t] = img[gj + 3];
       }
     }
     __syncthreads();
@@ -262,86 +254,134 @@ __global__ __launch_bounds__(kBlock) void k_accel_tip64(
         double x2 = sx[jj], y2 = sy[jj], z2 = sz[jj];
         double pre = qd1 * sq[jj];
         double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
-        double r = __builtin_sqrt(dx * dx + dy * dy + dz * dz) + g.es;
-        double inv_r3 = 1.0 / (r * r * r);
-        double axx = pre * inv_r3 * dx;
-        double ayy = pre * inv_r3 * dy;
-        double azz = pre * inv_r3 * dz;
-        if (g.ic) {
-          double icx, icy, icz;
-          if (j > i) {
-            tip_ic_force64(x1, y1, z1, x2, y2, z2, d1, xi1, yi1, zi1, g.R, g.ktip, icx, icy, icz);
-            axx = axx + pre * 1.0 * icx;
-            ayy = ayy + pre * 1.0 * icy;
-          } else {
-            tip_ic_force64(x2, y2, z2, x1, y1, z1, sd[jj], sxi[jj], syi[jj], szi[jj], g.R,
-                           g.ktip, icx, icy, icz);
-            axx = axx + pre * -1.0 * icx;
-            ayy = ayy + pre * -1.0 * icy;
+        double axx, ayy, azz;
+        if (KIND == RD_GEOM_PLANAR) {
+          double dxy2 = dx * dx + dy * dy;
+          double r = __builtin_sqrt(dxy2 + dz * dz) + g.es;
+          double inv_r3 = 1.0 / (r * r * r);
+          axx = pre * inv_r3 * dx;
+          ayy = pre * inv_r3 * dy;
+          azz = pre * inv_r3 * dz;
+          if (g.ic) {
+            double za = j > i ? z1 : z2;
+            double zb = j > i ? z2 : z1;
+            double icx, icy, icz;
+            ic_planar64(za, zb, dx, dy, dxy2, g.nic, g.d, g.es, icx, icy, icz);
+            axx = axx + pre * icx;
+            ayy = ayy + pre * icy;
+            azz = azz + pre * icz;
           }
-          azz = azz + pre * icz;
+        } else {
+          double r = __builtin_sqrt(dx * dx + dy * dy + dz * dz) + g.es;
+          double inv_r3 = 1.0 / (r * r * r);
+          axx = pre * inv_r3 * dx;
+          ayy = pre * inv_r3 * dy;
+          azz = pre * inv_r3 * dz;
+          if (g.ic) {
+            double icx, icy, icz;
+            if (j > i) {
+              tip_ic_force64(x1, y1, z1, x2, y2, z2, d1, xi1, yi1, zi1, g.R, g.ktip, icx, icy,
+                             icz);
+              axx = axx + pre * 1.0 * icx;
+              ayy = ayy + pre * 1.0 * icy;
+            } else {
+              tip_ic_force64(x2, y2, z2, x1, y1, z1, sd[jj], sxi[jj], syi[jj], szi[jj], g.R,
+                             g.ktip, icx, icy, icz);
+              axx = axx + pre * -1.0 * icx;
+              ayy = ayy + pre * -1.0 * icy;
+            }
+            azz = azz + pre * icz;
+          }
         }
         px = px + axx;
         py = py + ayy;
         pz = pz + azz;
       }
-      part[(size_t)(0 * nchunks + c) * n + i] = px;
-      part[(size_t)(1 * nchunks + c) * n + i] = py;
-      part[(size_t)(2 * nchunks + c) * n + i] = pz;
+      const size_t S = (size_t)v.e.nchunks_ub * v.e.n_ub;
+      P[0 * S + (size_t)c * v.e.n_ub + i] = px;
+      P[1 * S + (size_t)c * v.e.n_ub + i] = py;
+      P[2 * S + (size_t)c * v.e.n_ub + i] = pz;
     }
   }
 }
 
 // Sum the chunk partials in order and form a = (F + q E_vac)/m.
-__global__ void k_accel_finish64(int n, int nchunks, const double* __restrict__ part,
-                                 const double* __restrict__ x, const double* __restrict__ y,
-                                 const double* __restrict__ z, const double* __restrict__ q,
-                                 const double* __restrict__ m, DevGeom g, double* __restrict__ ax,
-                                 double* __restrict__ ay, double* __restrict__ az) {
+__global__ void k_accel_finish64(const Ent* tab, const SimDesc* desc, const SimState* st,
+                                 const double* __restrict__ part, const double* __restrict__ X,
+                                 const double* __restrict__ Y, const double* __restrict__ Z,
+                                 const double* __restrict__ Q, const double* __restrict__ M,
+                                 double* __restrict__ AX, double* __restrict__ AY,
+                                 double* __restrict__ AZ) {
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const DevGeom& g = desc[e.slot].g;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const double* P = part + e.part_off;
+  const size_t S = (size_t)e.nchunks_ub * e.n_ub;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int c = 0; c < nchunks; ++c) {
-    sx = sx + part[(size_t)(0 * nchunks + c) * n + i];
-    sy = sy + part[(size_t)(1 * nchunks + c) * n + i];
-    sz = sz + part[(size_t)(2 * nchunks + c) * n + i];
+    sx = sx + P[0 * S + (size_t)c * e.n_ub + i];
+    sy = sy + P[1 * S + (size_t)c * e.n_ub + i];
+    sz = sz + P[2 * S + (size_t)c * e.n_ub + i];
   }
-  double q1 = q[i];
-  double im = 1.0 / m[i];
+  const int gi = e.base + i;
+  double q1 = Q[gi];
+  double im = 1.0 / M[gi];
   if (g.kind == RD_GEOM_PLANAR) {
-    ax[i] = sx * im;
-    ay[i] = sy * im;
-    az[i] = (sz + q1 * g.Ez) * im;
+    AX[gi] = sx * im;
+    AY[gi] = sy * im;
+    AZ[gi] = (sz + q1 * g.Ez) * im;
   } else {
     double fx, fy, fz;
-    tip_vacuum64(g, x[i], y[i], z[i], fx, fy, fz);
-    ax[i] = (sx + q1 * fx) * im;
-    ay[i] = (sy + q1 * fy) * im;
-    az[i] = (sz + q1 * fz) * im;
+    tip_vacuum64(g, X[gi], Y[gi], Z[gi], fx, fy, fz);
+    AX[gi] = (sx + q1 * fx) * im;
+    AY[gi] = (sy + q1 * fy) * im;
+    AZ[gi] = (sz + q1 * fz) * im;
   }
 }
 
 // ------------------------------------------------------------------------
-// FP64 field at points (SoA px,py,pz), planar and tip.
+// FP64 field at points.  Points of an entry: [pbase, pbase + m) of the SoA
+// point arrays; part layout per entry: [3][nchunks_ub][m].
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_field64(
-    int m, const double* __restrict__ ptx, const double* __restrict__ pty,
-    const double* __restrict__ ptz, int n, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ q,
-    DevGeom g, int cps, int nchunks, double* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) void k_field64(const Ent* tab, const SimDesc* desc,
+                                                    const SimState* st,
+                                                    const double* __restrict__ ptx,
+                                                    const double* __restrict__ pty,
+                                                    const double* __restrict__ ptz,
+                                                    const double* __restrict__ X,
+                                                    const double* __restrict__ Y,
+                                                    const double* __restrict__ Z,
+                                                    const double* __restrict__ Q,
+                                                    double* __restrict__ part) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
+  const Ent e = tab[blockIdx.z];
+  const int m = e.m;
+  if ((int)(blockIdx.x * kBlock) >= m) return;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int c0 = blockIdx.y * e.cps;
+  if (c0 >= nchunks) return;
+  const int c1 = min(nchunks, c0 + e.cps);
+  const DevGeom g = desc[e.slot].g;
+  if (g.kind != KIND) return;
+  const double* x = X + e.base;
+  const double* y = Y + e.base;
+  const double* z = Z + e.base;
+  const double* q = Q + e.base;
+  double* P = part + e.part_off;
   const int t = threadIdx.x;
   const int p = blockIdx.x * kBlock + t;
   const bool act = p < m;
   double x1 = 0, y1 = 0, z1 = 0, da = 0, xim = 0, yim = 0, zim = 0;
   if (act) {
-    x1 = ptx[p];
-    y1 = pty[p];
-    z1 = ptz[p];
+    x1 = ptx[e.pbase + p];
+    y1 = pty[e.pbase + p];
+    z1 = ptz[e.pbase + p];
     if (KIND == RD_GEOM_TIP && g.ic) tip_image_point64(x1, y1, z1, g.z0, g.R, da, xim, yim, zim);
   }
-  const int c0 = blockIdx.y * cps;
-  const int c1 = min(nchunks, c0 + cps);
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
@@ -392,51 +432,84 @@ __global__ __launch_bounds__(kBlock) void k_field64(
         fys = fys + ay;
         fzs = fzs + az;
       }
-      part[(size_t)(0 * nchunks + c) * m + p] = fxs;
-      part[(size_t)(1 * nchunks + c) * m + p] = fys;
-      part[(size_t)(2 * nchunks + c) * m + p] = fzs;
+      const size_t S = (size_t)e.nchunks_ub * m;
+      P[0 * S + (size_t)c * m + p] = fxs;
+      P[1 * S + (size_t)c * m + p] = fys;
+      P[2 * S + (size_t)c * m + p] = fzs;
     }
   }
 }
 
-// field = E_vac + sum of chunk partials (in order).  nchunks may be 0.
-__global__ void k_field_finish64(int m, int nchunks, const double* __restrict__ part,
-                                 const double* __restrict__ ptx, const double* __restrict__ pty,
-                                 const double* __restrict__ ptz, DevGeom g,
+// field = E_vac + sum of chunk partials (in order).
+__global__ void k_field_finish64(const Ent* tab, const SimDesc* desc, const SimState* st,
+                                 const double* __restrict__ part, const double* __restrict__ ptx,
+                                 const double* __restrict__ pty, const double* __restrict__ ptz,
                                  double* __restrict__ fx, double* __restrict__ fy,
                                  double* __restrict__ fz) {
+  const Ent e = tab[blockIdx.z];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= m) return;
+  if (p >= e.m) return;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const double* P = part + e.part_off;
+  const size_t S = (size_t)e.nchunks_ub * e.m;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int c = 0; c < nchunks; ++c) {
-    sx = sx + part[(size_t)(0 * nchunks + c) * m + p];
-    sy = sy + part[(size_t)(1 * nchunks + c) * m + p];
-    sz = sz + part[(size_t)(2 * nchunks + c) * m + p];
+    sx = sx + P[0 * S + (size_t)c * e.m + p];
+    sy = sy + P[1 * S + (size_t)c * e.m + p];
+    sz = sz + P[2 * S + (size_t)c * e.m + p];
   }
-  double ex = 0.0, ey = 0.0, ez = g.Ez;
-  if (g.kind == RD_GEOM_TIP) tip_vacuum64(g, ptx[p], pty[p], ptz[p], ex, ey, ez);
-  fx[p] = ex + sx;
-  fy[p] = ey + sy;
-  fz[p] = ez + sz;
+  const int gp = e.pbase + p;
+  double ex, ey, ez;
+  vacuum64(desc[e.slot].g, ptx[gp], pty[gp], ptz[gp], ex, ey, ez);
+  fx[gp] = ex + sx;
+  fy[gp] = ey + sy;
+  fz[gp] = ez + sz;
 }
 
 // ------------------------------------------------------------------------
 // Mixed precision.
 //
-// Packed source/target record (nm): hi = (x_hi, y_hi, z_hi, q/q_0),
-// lo = (x_lo, y_lo, z_lo, d - z) with x = x_hi + x_lo to ~2^-48 relative.
-__global__ void k_pack_mixed(int n, const double* __restrict__ x, const double* __restrict__ y,
-                             const double* __restrict__ z, const double* __restrict__ q,
-                             double d, float4* __restrict__ hi, float4* __restrict__ lo) {
+// Packed record (nm): hi = (x_hi, y_hi, z_hi, q/q_0), lo = (x_lo, y_lo, z_lo,
+// d - z) with x = x_hi + x_lo to ~2^-48 relative.
+__device__ __forceinline__ void pack1(double xm, double ym, double zm, double qv, double d,
+                                      float4& hi, float4& lo) {
+  const double s = 1.0e9;
+  double xn = xm * s, yn = ym * s, zn = zm * s;
+  float xh = (float)xn, yh = (float)yn, zh = (float)zn;
+  hi = make_float4(xh, yh, zh, (float)(qv / Consts::q_0));
+  lo = make_float4((float)(xn - (double)xh), (float)(yn - (double)yh), (float)(zn - (double)zh),
+                   (float)(d * s - zn));
+}
+
+// Pack the particles of every entry (targets = sources).
+__global__ void k_pack_particles(const Ent* tab, const SimDesc* desc, const SimState* st,
+                                 const double* __restrict__ X, const double* __restrict__ Y,
+                                 const double* __restrict__ Z, const double* __restrict__ Q,
+                                 float4* __restrict__ hi, float4* __restrict__ lo) {
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double s = 1.0e9;
-  double xn = x[i] * s, yn = y[i] * s, zn = z[i] * s;
-  float xh = (float)xn, yh = (float)yn, zh = (float)zn;
-  float qr = q ? (float)(q[i] / Consts::q_0) : 0.0f;
-  hi[i] = make_float4(xh, yh, zh, qr);
-  lo[i] = make_float4((float)(xn - (double)xh), (float)(yn - (double)yh),
-                      (float)(zn - (double)zh), (float)(d * s - zn));
+  const int gi = e.base + i;
+  float4 h, l;
+  pack1(X[gi], Y[gi], Z[gi], Q[gi], desc[e.slot].g.d, h, l);
+  hi[gi] = h;
+  lo[gi] = l;
+}
+
+// Pack the field points of every entry.
+__global__ void k_pack_points(const Ent* tab, const SimDesc* desc, const double* __restrict__ ptx,
+                              const double* __restrict__ pty, const double* __restrict__ ptz,
+                              float4* __restrict__ hi, float4* __restrict__ lo) {
+  const Ent e = tab[blockIdx.z];
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= e.m) return;
+  const int gp = e.pbase + p;
+  float4 h, l;
+  pack1(ptx[gp], pty[gp], ptz[gp], 0.0, desc[e.slot].g.d, h, l);
+  hi[gp] = h;
+  lo[gp] = l;
 }
 
 __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
@@ -448,14 +521,14 @@ __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v);
 constexpr float kR2Min = 1.0e-20f;
 
 // One image partner: accumulate 1/r^3 and dz/r^3.
-#define RD_PARTNER(DZ, W, WZ)                 \
-  {                                           \
-    float _dz = (DZ);                         \
+#define RD_PARTNER(DZ, W, WZ)                   \
+  {                                             \
+    float _dz = (DZ);                           \
     float _r2 = __builtin_fmaf(_dz, _dz, dxy2); \
-    float _ri = rsq(_r2);                     \
-    float _r3 = _ri * _ri * _ri;              \
-    W += _r3;                                 \
-    WZ = __builtin_fmaf(_dz, _r3, WZ);        \
+    float _ri = rsq(_r2);                       \
+    float _r3 = _ri * _ri * _ri;                \
+    W += _r3;                                   \
+    WZ = __builtin_fmaf(_dz, _r3, WZ);          \
   }
 
 // Image-charge weights of one pair: returns (sum_same - sum_opp) of 1/r^3 in
@@ -496,12 +569,27 @@ __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, 
   wz = wzs - wzo;
 }
 
-// Mixed accelerations (planar).  part layout: [3][segs][n] FP64 sums of
-// q_j/q_0 * (offset / r^3 + image terms) in nm^-2.
-__global__ __launch_bounds__(kBlock) void k_accel_planar32(
-    int n, const float4* __restrict__ hi, const float4* __restrict__ lo, int ic, int nic,
-    float two_d, int cps, int nchunks, double* __restrict__ part) {
+// Mixed accelerations (planar).  part layout per entry: [3][segs][n_ub] FP64
+// sums of q_j/q_0 * (offset / r^3 + image terms) in nm^-2.
+__global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, const SimDesc* desc,
+                                                           const SimState* st,
+                                                           const float4* __restrict__ HI,
+                                                           const float4* __restrict__ LO,
+                                                           double* __restrict__ part) {
   __shared__ float4 shi[kChunk], slo[kChunk];
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
+  if ((int)(blockIdx.x * kBlock) >= n) return;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int c0 = blockIdx.y * e.cps;
+  if (c0 >= nchunks) return;
+  const int c1 = min(nchunks, c0 + e.cps);
+  const DevGeom& gg = desc[e.slot].g;
+  if (gg.kind != RD_GEOM_PLANAR) return;
+  const int ic = gg.ic, nic = gg.nic;
+  const float two_d = (float)(2.0 * gg.d * 1e9);
+  const float4* hi = HI + e.base;
+  const float4* lo = LO + e.base;
   const int t = threadIdx.x;
   const int i = blockIdx.x * kBlock + t;
   const bool act = i < n;
@@ -511,8 +599,6 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(
     l1 = lo[i];
   }
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  const int c0 = blockIdx.y * cps;
-  const int c1 = min(nchunks, c0 + cps);
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
@@ -551,50 +637,74 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(
     }
   }
   if (act) {
-    const int segs = gridDim.y;
-    part[(size_t)(0 * segs + blockIdx.y) * n + i] = fx;
-    part[(size_t)(1 * segs + blockIdx.y) * n + i] = fy;
-    part[(size_t)(2 * segs + blockIdx.y) * n + i] = fz;
+    double* P = part + e.part_off;
+    const size_t S = (size_t)e.segs * e.n_ub;
+    P[0 * S + (size_t)blockIdx.y * e.n_ub + i] = fx;
+    P[1 * S + (size_t)blockIdx.y * e.n_ub + i] = fy;
+    P[2 * S + (size_t)blockIdx.y * e.n_ub + i] = fz;
   }
 }
 
-__global__ void k_accel_finish32(int n, int segs, const double* __restrict__ part,
-                                 const double* __restrict__ q, const double* __restrict__ m,
-                                 DevGeom g, double* __restrict__ ax, double* __restrict__ ay,
-                                 double* __restrict__ az) {
+__global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimState* st,
+                                 const double* __restrict__ part, const double* __restrict__ Q,
+                                 const double* __restrict__ M, double* __restrict__ AX,
+                                 double* __restrict__ AY, double* __restrict__ AZ) {
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const DevGeom& g = desc[e.slot].g;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int segs = min(e.segs, (nchunks + e.cps - 1) / e.cps);
+  const double* P = part + e.part_off;
+  const size_t S = (size_t)e.segs * e.n_ub;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int s = 0; s < segs; ++s) {
-    sx += part[(size_t)(0 * segs + s) * n + i];
-    sy += part[(size_t)(1 * segs + s) * n + i];
-    sz += part[(size_t)(2 * segs + s) * n + i];
+    sx += P[0 * S + (size_t)s * e.n_ub + i];
+    sy += P[1 * S + (size_t)s * e.n_ub + i];
+    sz += P[2 * S + (size_t)s * e.n_ub + i];
   }
-  double q1 = q[i];
-  double im = 1.0 / m[i];
+  const int gi = e.base + i;
+  double q1 = Q[gi];
+  double im = 1.0 / M[gi];
   double pre = g.dfc * q1 * Consts::q_0 * 1.0e18 * im;
-  ax[i] = pre * sx;
-  ay[i] = pre * sy;
-  az[i] = pre * sz + q1 * g.Ez * im;
+  AX[gi] = pre * sx;
+  AY[gi] = pre * sy;
+  AZ[gi] = pre * sz + q1 * g.Ez * im;
 }
 
 // Mixed field at points (planar).
-__global__ __launch_bounds__(kBlock) void k_field_planar32(
-    int m, const float4* __restrict__ phi, const float4* __restrict__ plo, int n,
-    const float4* __restrict__ hi, const float4* __restrict__ lo, int ic, int nic, float two_d,
-    int cps, int nchunks, double* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, const SimDesc* desc,
+                                                           const SimState* st,
+                                                           const float4* __restrict__ PHI,
+                                                           const float4* __restrict__ PLO,
+                                                           const float4* __restrict__ HI,
+                                                           const float4* __restrict__ LO,
+                                                           double* __restrict__ part) {
   __shared__ float4 shi[kChunk], slo[kChunk];
+  const Ent e = tab[blockIdx.z];
+  const int m = e.m;
+  if ((int)(blockIdx.x * kBlock) >= m) return;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int c0 = blockIdx.y * e.cps;
+  if (c0 >= max(nchunks, 1)) return;
+  const int c1 = min(nchunks, c0 + e.cps);
+  const DevGeom& gg = desc[e.slot].g;
+  if (gg.kind != RD_GEOM_PLANAR) return;
+  const int ic = gg.ic, nic = gg.nic;
+  const float two_d = (float)(2.0 * gg.d * 1e9);
+  const float4* hi = HI + e.base;
+  const float4* lo = LO + e.base;
   const int t = threadIdx.x;
   const int p = blockIdx.x * kBlock + t;
   const bool act = p < m;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
   if (act) {
-    h1 = phi[p];
-    l1 = plo[p];
+    h1 = PHI[e.pbase + p];
+    l1 = PLO[e.pbase + p];
   }
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  const int c0 = blockIdx.y * cps;
-  const int c1 = min(nchunks, c0 + cps);
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
@@ -630,28 +740,37 @@ __global__ __launch_bounds__(kBlock) void k_field_planar32(
     }
   }
   if (act) {
-    const int segs = gridDim.y;
-    part[(size_t)(0 * segs + blockIdx.y) * m + p] = fx;
-    part[(size_t)(1 * segs + blockIdx.y) * m + p] = fy;
-    part[(size_t)(2 * segs + blockIdx.y) * m + p] = fz;
+    double* P = part + e.part_off;
+    const size_t S = (size_t)e.segs * m;
+    P[0 * S + (size_t)blockIdx.y * m + p] = fx;
+    P[1 * S + (size_t)blockIdx.y * m + p] = fy;
+    P[2 * S + (size_t)blockIdx.y * m + p] = fz;
   }
 }
 
-__global__ void k_field_finish32(int m, int segs, const double* __restrict__ part, DevGeom g,
-                                 double* __restrict__ fx, double* __restrict__ fy,
-                                 double* __restrict__ fz) {
+__global__ void k_field_finish32(const Ent* tab, const SimDesc* desc, const SimState* st,
+                                 const double* __restrict__ part, double* __restrict__ fx,
+                                 double* __restrict__ fy, double* __restrict__ fz) {
+  const Ent e = tab[blockIdx.z];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= m) return;
+  if (p >= e.m) return;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
+  const double* P = part + e.part_off;
+  const size_t S = (size_t)e.segs * e.m;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int s = 0; s < segs; ++s) {
-    sx += part[(size_t)(0 * segs + s) * m + p];
-    sy += part[(size_t)(1 * segs + s) * m + p];
-    sz += part[(size_t)(2 * segs + s) * m + p];
+    sx += P[0 * S + (size_t)s * e.m + p];
+    sy += P[1 * S + (size_t)s * e.m + p];
+    sz += P[2 * S + (size_t)s * e.m + p];
   }
+  const DevGeom& g = desc[e.slot].g;
   double pre = g.dfc * Consts::q_0 * 1.0e18;
-  fx[p] = pre * sx;
-  fy[p] = pre * sy;
-  fz[p] = g.Ez + pre * sz;
+  const int gp = e.pbase + p;
+  fx[gp] = pre * sx;
+  fy[gp] = pre * sy;
+  fz[gp] = g.Ez + pre * sz;
 }
 
 __global__ void k_grid_points(int nx, int ny, double x0, double y0, double lx, double ly,
@@ -668,141 +787,208 @@ __global__ void k_grid_points(int nx, int ny, double x0, double y0, double lx, d
 // ------------------------------------------------------------------------
 // Launch helpers.
 
-// Segments of the source range: enough workgroups for ~8 per CU.
-static int pick_segments(int target_blocks, int nchunks) {
+// Segments of the source range: enough workgroups for ~8 per CU overall.
+static void segment(int target_blocks_total, int nchunks, int& segs, int& cps) {
   const int want = 2048;
-  int segs = (want + target_blocks - 1) / target_blocks;
-  segs = std::max(1, std::min(segs, nchunks));
-  return segs;
+  int s = (want + target_blocks_total - 1) / std::max(target_blocks_total, 1);
+  s = std::max(1, std::min(s, std::max(nchunks, 1)));
+  cps = (std::max(nchunks, 1) + s - 1) / s;
+  segs = (std::max(nchunks, 1) + cps - 1) / cps;
 }
 
-int launch_accel(rd_ctx* c, int precision, hipEvent_t k0, hipEvent_t k1) {
-  const int n = c->n;
-  if (n == 0) return RD_OK;
-  const int tb = (n + kBlock - 1) / kBlock;
-  const int nchunks = (n + kChunk - 1) / kChunk;
-  const int fin = (n + 255) / 256;
-  if (precision == RD_PREC_FP64) {
-    int r = ensure(c, c->part, sizeof(double) * 3 * (size_t)nchunks * n);
-    if (r) return r;
-    int segs = pick_segments(tb, nchunks);
-    int cps = (nchunks + segs - 1) / segs;
-    segs = (nchunks + cps - 1) / cps;
-    dim3 grid(tb, segs);
-    if (c->g.kind == RD_GEOM_PLANAR) {
-      if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-      k_accel_planar64<<<grid, kBlock, 0, c->stream>>>(n, c->x, c->y, c->z, c->q, c->g, cps,
-                                                        nchunks, (double*)c->part.p);
-      if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-    } else {
-      r = ensure(c, c->img64, sizeof(double) * 4 * (size_t)n);
-      if (r) return r;
-      if (c->g.ic)
-        k_tip_images64<<<fin, 256, 0, c->stream>>>(n, c->x, c->y, c->z, c->g,
-                                                    (double*)c->img64.p);
-      if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-      k_accel_tip64<<<grid, kBlock, 0, c->stream>>>(n, c->x, c->y, c->z, c->q,
-                                                     (const double*)c->img64.p, c->g, cps,
-                                                     nchunks, (double*)c->part.p);
-      if (k1) RD_HIP(hipEventRecord(k1, c->stream));
+// Launch tables travel through a ring of pinned host slots and device slots.
+// A ring slot is rewritten only after the stream has been synchronised since
+// its last use (the host waits when the ring is exhausted), and a table equal
+// to the previous upload reuses it, so the Metropolis-Hastings loop, which
+// launches the same table for every jump, uploads it once.
+int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab) {
+  const int kRing = 256;
+  const int per = c->nslots_max;
+  if (ents.size() > (size_t)per) return fail(RD_ERR_ARG, "launch table larger than the pool");
+  int r = ensure(c, c->tab, sizeof(Ent) * (size_t)per * kRing);
+  if (r) return r;
+  if (!c->h_tab) {
+    RD_HIP(hipHostMalloc((void**)&c->h_tab, sizeof(Ent) * (size_t)per * kRing));
+    c->tab_used = 0;
+    c->tab_last = -1;
+  }
+  if (c->tab_last >= 0 && c->tab_last_n == ents.size() &&
+      std::memcmp(c->h_tab + (size_t)c->tab_last * per, ents.data(), sizeof(Ent) * ents.size()) ==
+          0) {
+    *dtab = (Ent*)c->tab.p + (size_t)c->tab_last * per;
+    return RD_OK;
+  }
+  if (c->tab_used >= kRing) {
+    RD_HIP(hipStreamSynchronize(c->stream));
+    c->tab_used = 0;
+  }
+  int k = c->tab_used++;
+  Ent* h = c->h_tab + (size_t)k * per;
+  Ent* d = (Ent*)c->tab.p + (size_t)k * per;
+  std::memcpy(h, ents.data(), sizeof(Ent) * ents.size());
+  RD_HIP(hipMemcpyAsync(d, h, sizeof(Ent) * ents.size(), hipMemcpyHostToDevice, c->stream));
+  c->tab_last = k;
+  c->tab_last_n = ents.size();
+  *dtab = d;
+  return RD_OK;
+}
+
+int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_t k0,
+                 hipEvent_t k1) {
+  std::vector<Ent> ents;
+  int maxn = 0, tbt = 0;
+  bool any_tip = false, any_planar = false;
+  for (const Ent& e : in)
+    if (e.n_ub > 0) {
+      ents.push_back(e);
+      maxn = std::max(maxn, e.n_ub);
+      tbt += (e.n_ub + kBlock - 1) / kBlock;
+      if (c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
+        any_tip = true;
+      else
+        any_planar = true;
     }
-    k_accel_finish64<<<fin, 256, 0, c->stream>>>(n, nchunks, (const double*)c->part.p, c->x,
-                                                  c->y, c->z, c->q, c->m, c->g, c->ax, c->ay,
-                                                  c->az);
+  if (ents.empty()) return RD_OK;
+  if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
+    return fail(RD_ERR_ARG, "unknown precision");
+  if (precision == RD_PREC_MIXED && any_tip)
+    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
+  long long off = 0;
+  int maxsegs = 1;
+  for (Ent& e : ents) {
+    int nch = (e.n_ub + kChunk - 1) / kChunk;
+    e.nchunks_ub = nch;
+    segment(tbt, nch, e.segs, e.cps);
+    maxsegs = std::max(maxsegs, e.segs);
+    e.part_off = off;
+    off += 3LL * (precision == RD_PREC_FP64 ? nch : e.segs) * e.n_ub;
+  }
+  int r = ensure(c, c->part, sizeof(double) * (size_t)off);
+  if (r) return r;
+  Ent* dt = nullptr;
+  r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  const int ne = (int)ents.size();
+  const int tb = (maxn + kBlock - 1) / kBlock;
+  dim3 grid(tb, maxsegs, ne);
+  dim3 fin((maxn + 255) / 256, 1, ne);
+  double* part = (double*)c->part.p;
+  if (precision == RD_PREC_FP64) {
+    if (any_tip) {
+      r = ensure(c, c->img64, sizeof(double) * 4 * (size_t)c->cap);
+      if (r) return r;
+      k_tip_images64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->x, c->y, c->z,
+                                                  (double*)c->img64.p);
+    }
+    if (k0) RD_HIP(hipEventRecord(k0, c->stream));
+    if (any_planar)
+      k_accel64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(
+          dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
+    if (any_tip)
+      k_accel64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(
+          dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
+    if (k1) RD_HIP(hipEventRecord(k1, c->stream));
+    k_accel_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->x, c->y,
+                                                  c->z, c->q, c->m, c->ax, c->ay, c->az);
     RD_HIP(hipGetLastError());
     return RD_OK;
   }
-  if (precision != RD_PREC_MIXED) return fail(RD_ERR_ARG, "unknown precision");
-  if (c->g.kind != RD_GEOM_PLANAR)
-    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
-  int r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)n);
-  if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)n);
+  r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)c->cap);
+  if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)c->cap);
   if (r) return r;
-  int segs = pick_segments(tb, nchunks);
-  int cps = (nchunks + segs - 1) / segs;
-  segs = (nchunks + cps - 1) / cps;
-  r = ensure(c, c->part, sizeof(double) * 3 * (size_t)segs * n);
-  if (r) return r;
-  k_pack_mixed<<<fin, 256, 0, c->stream>>>(n, c->x, c->y, c->z, c->q, c->g.d,
-                                            (float4*)c->pack_hi.p, (float4*)c->pack_lo.p);
+  k_pack_particles<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->x, c->y, c->z,
+                                                c->q, (float4*)c->pack_hi.p,
+                                                (float4*)c->pack_lo.p);
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-  k_accel_planar32<<<dim3(tb, segs), kBlock, 0, c->stream>>>(
-      n, (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p, c->g.ic, c->g.nic,
-      (float)(2.0 * c->g.d * 1e9), cps, nchunks, (double*)c->part.p);
+  k_accel_planar32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                    (const float4*)c->pack_hi.p,
+                                                    (const float4*)c->pack_lo.p, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-  k_accel_finish32<<<fin, 256, 0, c->stream>>>(n, segs, (const double*)c->part.p, c->q, c->m,
-                                                c->g, c->ax, c->ay, c->az);
+  k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->q, c->m,
+                                                c->ax, c->ay, c->az);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }
 
-// Field at the m points held in c->pts (SoA); result SoA in c->fld.
-int launch_field(rd_ctx* c, int m, int precision, hipEvent_t k0, hipEvent_t k1) {
-  if (m == 0) return RD_OK;
-  const int n = c->n;
-  double* ptx = (double*)c->pts.p;
-  double* pty = ptx + m;
-  double* ptz = pty + m;
-  double* fx = (double*)c->fld.p;
-  double* fy = fx + m;
-  double* fz = fy + m;
-  const int tb = (m + kBlock - 1) / kBlock;
-  const int nchunks = (n + kChunk - 1) / kChunk;
-  const int fin = (m + 255) / 256;
-  if (precision == RD_PREC_FP64) {
-    int segs = 1, cps = 1;
-    if (nchunks > 0) {
-      int r = ensure(c, c->part, sizeof(double) * 3 * (size_t)nchunks * m);
-      if (r) return r;
-      segs = pick_segments(tb, nchunks);
-      cps = (nchunks + segs - 1) / segs;
-      segs = (nchunks + cps - 1) / cps;
-      if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-      if (c->g.kind == RD_GEOM_PLANAR)
-        k_field64<RD_GEOM_PLANAR><<<dim3(tb, segs), kBlock, 0, c->stream>>>(
-            m, ptx, pty, ptz, n, c->x, c->y, c->z, c->q, c->g, cps, nchunks,
-            (double*)c->part.p);
+int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision, hipEvent_t k0,
+                 hipEvent_t k1) {
+  std::vector<Ent> ents;
+  int maxm = 0, tbt = 0;
+  bool any_tip = false, any_planar = false;
+  for (const Ent& e : in)
+    if (e.m > 0) {
+      ents.push_back(e);
+      maxm = std::max(maxm, e.m);
+      tbt += (e.m + kBlock - 1) / kBlock;
+      if (c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
+        any_tip = true;
       else
-        k_field64<RD_GEOM_TIP><<<dim3(tb, segs), kBlock, 0, c->stream>>>(
-            m, ptx, pty, ptz, n, c->x, c->y, c->z, c->q, c->g, cps, nchunks,
-            (double*)c->part.p);
-      if (k1) RD_HIP(hipEventRecord(k1, c->stream));
+        any_planar = true;
     }
-    k_field_finish64<<<fin, 256, 0, c->stream>>>(m, nchunks, (const double*)c->part.p, ptx, pty,
-                                                  ptz, c->g, fx, fy, fz);
+  if (ents.empty()) return RD_OK;
+  if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
+    return fail(RD_ERR_ARG, "unknown precision");
+  if (precision == RD_PREC_MIXED && any_tip)
+    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
+  long long off = 0;
+  int maxsegs = 1, maxn = 0;
+  for (Ent& e : ents) {
+    int nch = (e.n_ub + kChunk - 1) / kChunk;
+    e.nchunks_ub = nch;
+    segment(tbt, nch, e.segs, e.cps);
+    maxsegs = std::max(maxsegs, e.segs);
+    maxn = std::max(maxn, e.n_ub);
+    e.part_off = off;
+    off += 3LL * (precision == RD_PREC_FP64 ? std::max(nch, 1) : e.segs) * e.m;
+  }
+  int r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
+  if (r) return r;
+  Ent* dt = nullptr;
+  r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  const int ne = (int)ents.size();
+  double* ptx = (double*)c->pts.p;
+  double* pty = ptx + mtot;
+  double* ptz = pty + mtot;
+  double* fx = (double*)c->fld.p;
+  double* fy = fx + mtot;
+  double* fz = fy + mtot;
+  double* part = (double*)c->part.p;
+  dim3 grid((maxm + kBlock - 1) / kBlock, maxsegs, ne);
+  dim3 fin((maxm + 255) / 256, 1, ne);
+  if (precision == RD_PREC_FP64) {
+    if (k0) RD_HIP(hipEventRecord(k0, c->stream));
+    if (maxn > 0) {
+      if (any_planar)
+        k_field64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(
+            dt, c->d_desc, c->d_state, ptx, pty, ptz, c->x, c->y, c->z, c->q, part);
+      if (any_tip)
+        k_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(
+            dt, c->d_desc, c->d_state, ptx, pty, ptz, c->x, c->y, c->z, c->q, part);
+    }
+    if (k1) RD_HIP(hipEventRecord(k1, c->stream));
+    k_field_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, ptx, pty, ptz,
+                                                  fx, fy, fz);
     RD_HIP(hipGetLastError());
     return RD_OK;
   }
-  if (precision != RD_PREC_MIXED) return fail(RD_ERR_ARG, "unknown precision");
-  if (c->g.kind != RD_GEOM_PLANAR)
-    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
-  int r = ensure(c, c->pts_hi, sizeof(float4) * (size_t)m);
-  if (!r) r = ensure(c, c->pts_lo, sizeof(float4) * (size_t)m);
-  if (!r) r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)std::max(n, 1));
-  if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)std::max(n, 1));
+  r = ensure(c, c->pts_hi, sizeof(float4) * (size_t)mtot);
+  if (!r) r = ensure(c, c->pts_lo, sizeof(float4) * (size_t)mtot);
+  if (!r) r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)c->cap);
+  if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)c->cap);
   if (r) return r;
-  int segs = 1, cps = 1;
-  if (nchunks > 0) {
-    segs = pick_segments(tb, nchunks);
-    cps = (nchunks + segs - 1) / segs;
-    segs = (nchunks + cps - 1) / cps;
-  }
-  r = ensure(c, c->part, sizeof(double) * 3 * (size_t)segs * m);
-  if (r) return r;
-  k_pack_mixed<<<fin, 256, 0, c->stream>>>(m, ptx, pty, ptz, nullptr, c->g.d,
-                                            (float4*)c->pts_hi.p, (float4*)c->pts_lo.p);
-  if (n > 0)
-    k_pack_mixed<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->x, c->y, c->z, c->q, c->g.d,
-                                                          (float4*)c->pack_hi.p,
-                                                          (float4*)c->pack_lo.p);
+  k_pack_points<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, ptx, pty, ptz, (float4*)c->pts_hi.p,
+                                             (float4*)c->pts_lo.p);
+  if (maxn > 0)
+    k_pack_particles<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(
+        dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (float4*)c->pack_hi.p,
+        (float4*)c->pack_lo.p);
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-  k_field_planar32<<<dim3(tb, segs), kBlock, 0, c->stream>>>(
-      m, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p, n, (const float4*)c->pack_hi.p,
-      (const float4*)c->pack_lo.p, c->g.ic, c->g.nic, (float)(2.0 * c->g.d * 1e9), cps, nchunks,
-      (double*)c->part.p);
+  k_field_planar32<<<grid, kBlock, 0, c->stream>>>(
+      dt, c->d_desc, c->d_state, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p,
+      (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-  k_field_finish32<<<fin, 256, 0, c->stream>>>(m, segs, (const double*)c->part.p, c->g, fx, fy,
-                                                fz);
+  k_field_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, fx, fy, fz);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }

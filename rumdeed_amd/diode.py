@@ -52,19 +52,85 @@ class Diode:
         self.close()
 
     # ---- set-up (Read_Input_Variables, Init_Field_Emission_v2, Init_Emission_Tip)
-    def set_planar(self, V: float, d: float, image_charge: bool = True, n_ic_max: int = 0):
+    def set_planar(self, V: float, d: float, image_charge: bool = True, n_ic_max: int = 0,
+                   box_top: float = 0.0):
         g = L.Geometry(kind=RD_GEOM_PLANAR, image_charge=int(bool(image_charge)),
-                       n_ic_max=int(n_ic_max), V=float(V), d=float(d))
+                       n_ic_max=int(n_ic_max), V=float(V), d=float(d), box_top=float(box_top))
         check(self._lib.rd_set_geometry(self._h, ctypes.byref(g)))
 
     def set_tip(self, V: float, d_tip: float, R_base: float, h_tip: float,
-                image_charge: bool = True):
+                image_charge: bool = True, box_top: float = 0.0):
         g = L.Geometry(kind=RD_GEOM_TIP, image_charge=int(bool(image_charge)), V=float(V),
-                       d_tip=float(d_tip), R_base=float(R_base), h_tip=float(h_tip))
+                       d_tip=float(d_tip), R_base=float(R_base), h_tip=float(h_tip),
+                       box_top=float(box_top))
         check(self._lib.rd_set_geometry(self._h, ctypes.byref(g)))
 
     def set_time_step(self, dt: float):
         check(self._lib.rd_set_time_step(self._h, c_double(dt)))
+
+    def set_seed(self, seed: int, stream: int = 0):
+        check(self._lib.rd_set_seed(self._h, int(seed), int(stream)))
+
+    def set_emitter(self, mode: int, delay: int = 0, x0: float = 0.0, y0: float = 0.0,
+                    lx: float = 0.0, ly: float = 0.0, T: float = 300.0, w_tip: float = 4.7,
+                    wf=None, quad: int = 0, ndim: int = 0):
+        """Emitter + work function (`work` file: rows top first)."""
+        wf = np.array([[4.7]]) if wf is None else np.atleast_2d(np.asarray(wf, dtype=np.float64))
+        ny, nx = wf.shape
+        self._wf = np.ascontiguousarray(wf.ravel())
+        em = L.Emitter(mode=int(mode), delay=int(delay), x0=x0, y0=y0, lx=lx, ly=ly, T=T,
+                       w_tip=w_tip, wf_nx=nx, wf_ny=ny, wf_table=dptr(self._wf),
+                       quad_n=int(quad), ndim=int(ndim))
+        check(self._lib.rd_set_emitter(self._h, ctypes.byref(em)))
+
+    # ---- the time step (main.F90:175-219)
+    def add_particles(self, pos, vel=None, species=None, step=1, emit=1, sec=None):
+        """Add_Particle for every row of pos (n,3), in order."""
+        pos = f64(pos).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else f64(vel).reshape(n, 3)
+        sp = None if species is None else np.ascontiguousarray(species, dtype=np.int32)
+        sc = None if sec is None else np.ascontiguousarray(sec, dtype=np.int32)
+        check(self._lib.rd_add_particles(self._h, n, dptr(pos), dptr(vel), iptr(sp), int(step),
+                                         int(emit), iptr(sc)))
+
+    def do_emission(self, step: int, precision="fp64"):
+        rec = L.StepRecord()
+        check(self._lib.rd_do_emission(self._h, int(step), _prec(precision), ctypes.byref(rec)))
+        return rec
+
+    def update_position(self, step: int, precision="fp64"):
+        rec = L.StepRecord()
+        check(self._lib.rd_update_position(self._h, int(step), _prec(precision),
+                                           ctypes.byref(rec)))
+        return rec
+
+    def remove_particles(self, step: int):
+        rec = L.StepRecord()
+        check(self._lib.rd_remove_particles(self._h, int(step), ctypes.byref(rec)))
+        return rec
+
+    def run(self, nsteps: int, precision="fp64"):
+        recs = (L.StepRecord * max(int(nsteps), 1))()
+        check(self._lib.rd_run(self._h, int(nsteps), _prec(precision), recs))
+        return list(recs)[:nsteps]
+
+    def get_state(self):
+        n = self.n
+        arrs = [np.zeros((n, 3)) for _ in range(6)]
+        ints = np.zeros((n, 5), dtype=np.int32)
+        check(self._lib.rd_get_state(self._h, *[dptr(a) for a in arrs], iptr(ints)))
+        keys = ["pos", "prev_pos", "vel", "acc", "acc_prev", "acc_prev2"]
+        st = dict(zip(keys, arrs))
+        st.update(species=ints[:, 0], step=ints[:, 1], emitter=ints[:, 2], section=ints[:, 3],
+                  id=ints[:, 4])
+        st["q"] = self.get_particles()["q"]
+        return st
+
+    def life_time(self):
+        h = np.zeros(2000, dtype=np.int32)
+        check(self._lib.rd_get_life_time(self._h, iptr(h)))
+        return h.reshape(2, 1000).T
 
     # ---- particle store (mod_pair)
     def put_particles(self, pos, q, m, vel=None, species=None):
@@ -141,3 +207,57 @@ def device_count() -> int:
     v = c_int32()
     code = L.lib().rd_device_count(ctypes.byref(v))
     return v.value if code == 0 else 0
+
+
+class Pool(Diode):
+    """Many independent simulations sharing one particle store on one device.
+
+    Every `step()` advances all open simulations with one launch per kernel.
+    The single-simulation methods inherited from Diode act on the selected
+    slot (`select`)."""
+
+    def __init__(self, total_particles: int, max_slots: int, device: int = 0):
+        self._lib = L.lib()
+        h = c_void_p()
+        check(self._lib.rd_pool_create(device, int(total_particles), int(max_slots),
+                                       ctypes.byref(h)))
+        self._h = h
+        self.max_particles = int(total_particles)
+        self.max_slots = int(max_slots)
+
+    @staticmethod
+    def footprint(capacity: int) -> int:
+        v = ctypes.c_int64()
+        check(L.lib().rd_slot_footprint(int(capacity), ctypes.byref(v)))
+        return v.value
+
+    def open(self, capacity: int) -> int:
+        s = c_int32()
+        check(self._lib.rd_slot_open(self._h, int(capacity), ctypes.byref(s)))
+        return s.value
+
+    def try_open(self, capacity: int):
+        s = c_int32()
+        code = self._lib.rd_slot_open(self._h, int(capacity), ctypes.byref(s))
+        if code == L.RD_OK:
+            return s.value
+        if code == 3:  # RD_ERR_CAPACITY
+            return None
+        check(code)
+
+    def close_slot(self, slot: int):
+        check(self._lib.rd_slot_close(self._h, int(slot)))
+
+    def select(self, slot: int):
+        check(self._lib.rd_slot_select(self._h, int(slot)))
+
+    def slot_step(self, slot: int) -> int:
+        v = c_int32()
+        check(self._lib.rd_slot_step(self._h, int(slot), ctypes.byref(v)))
+        return v.value
+
+    def step(self, precision="fp64"):
+        """One main-loop step of every open simulation; records indexed by slot."""
+        recs = (L.StepRecord * self.max_slots)()
+        check(self._lib.rd_pool_step(self._h, _prec(precision), recs))
+        return recs

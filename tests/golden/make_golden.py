@@ -134,6 +134,46 @@ def main():
             os.chdir(cwd)
     np.savez(os.path.join(HERE, "work_checker.npz"), table=tab, x0=x0, y0=y0, lx=lx, ly=ly,
              pts=pts, w=w, sec=sec)
+
+    # ---- time steps through the reference's own Update_Position and
+    # Remove_Particles: planar, image series N_ic_max = 1, electrons and ions,
+    # some of them about to cross the cathode or the anode.
+    d, V, dt = 1000 * NM, 2000.0, 0.25e-15
+    R.setup_planar(d, V, dt=dt, image_charge=True, n_ic_max=1)
+    R.clear()
+    rng = np.random.Generator(np.random.Philox(13))
+    n = 60
+    pos = np.stack([rng.uniform(-50, 50, n), rng.uniform(-50, 50, n),
+                    rng.uniform(5, 995, n)], axis=1) * NM
+    vel = rng.normal(0.0, 2.0e5, (n, 3))
+    pos[:10, 2] = rng.uniform(0.2, 2.0, 10) * NM     # close to the cathode, moving down
+    vel[:10, 2] = -rng.uniform(2e6, 8e6, 10)
+    pos[10:20, 2] = d - rng.uniform(0.2, 2.0, 10) * NM  # close to the anode, moving up
+    vel[10:20, 2] = rng.uniform(2e6, 8e6, 10)
+    perm = rng.permutation(n)
+    pos, vel = pos[perm], vel[perm]
+    species = np.where(np.arange(n) % 7 == 3, 2, 1).astype(np.int32)
+    birth = 1 + (np.arange(n) % 3)
+    for i in range(n):
+        R.add_particle(pos[i], vel[i], species=int(species[i]), step=int(birth[i]), emit=1, sec=1)
+    steps = 8
+    out = dict(d=d, V=V, dt=dt, pos0=pos, vel0=vel, species0=species, birth0=birth.astype(np.int32),
+               steps=steps)
+    for s in range(1, steps + 1):
+        R.update_position(s)
+        cnt = R.counts()
+        out[f"ramo_{s}"] = R.ramo()
+        out[f"removed_{s}"] = np.array([cnt["nrPart_remove"], cnt["nrPart_remove_top"],
+                                        cnt["nrPart_remove_bot"], cnt["nrElec_remove"],
+                                        cnt["nrIon_remove"]], dtype=np.int32)
+        R.remove_particles(s)
+        cnt = R.counts()
+        st = R.state(cnt["nrPart"])
+        for k in ("pos", "prev_pos", "vel", "acc", "acc_prev", "acc_prev2", "q", "m", "species",
+                  "step", "id"):
+            out[f"{k}_{s}"] = st[k]
+    out["life_time"] = R.life_time()
+    np.savez(os.path.join(HERE, "steps.npz"), **out)
     print("golden fixtures written to", HERE)
 
 

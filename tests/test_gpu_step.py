@@ -1,0 +1,213 @@
+"""The device-resident time step (emission, Beeman, boundaries, Ramo,
+compaction) against the oracle (bit for bit on the FP64 path), the
+reference's own routines (golden steps.npz) and closed forms."""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+NM = 1e-9
+
+REC_INT = ("n", "emitted", "n_round", "absorbed", "absorbed_top", "absorbed_bot", "dropped",
+           "start_rounds")
+REC_DBL = ("n_sup", "ramo_current", "mh_std", "a_rate", "df_avg")
+
+
+def assert_records_equal(rg, ro, ctx=""):
+    for k in REC_INT:
+        assert getattr(rg, k) == getattr(ro, k), (ctx, k, getattr(rg, k), getattr(ro, k))
+    for k in REC_DBL:
+        assert getattr(rg, k) == getattr(ro, k), (ctx, k, getattr(rg, k), getattr(ro, k))
+    assert list(rg.f_avg) == list(ro.f_avg), ctx
+    assert list(rg.ramo) == list(ro.ramo), ctx
+
+
+def assert_state_equal(sg, so, ctx=""):
+    for k in ("pos", "prev_pos", "vel", "acc", "acc_prev", "acc_prev2"):
+        assert np.array_equal(sg[k], so[k]), (ctx, k)
+    for k in ("species", "step", "emitter", "section", "id"):
+        assert np.array_equal(sg[k], so[k]), (ctx, k)
+
+
+@pytest.fixture(scope="module")
+def Diode():
+    from rumdeed_amd.diode import Diode
+
+    return Diode
+
+
+def test_beeman_kinematics_gpu(Diode, oracle_mod):
+    """Test_Beeman_Kinematics (src/mod_tests.F90:1403-1452) on the device."""
+    c = oracle_mod.consts()
+    d, V, dt, vx0 = 1000 * NM, 2.0e3, 0.25e-15, 1.0e3
+    dd = Diode(16)
+    dd.set_planar(V, d, False, 0)
+    dd.set_time_step(dt)
+    dd.add_particles(np.array([[0.0, 0.0, 500 * NM]]), np.array([[vx0, 0.0, 0.0]]))
+    for k in range(1, 4):
+        r = dd.update_position(k)
+        dd.remove_particles(k)
+    st = dd.get_state()
+    a_z = c.q_0 * V / (c.m_0 * d)
+    assert abs((st["pos"][0, 2] - 500 * NM) - 4.5 * a_z * dt ** 2) < 1e-9 * 4.5 * a_z * dt ** 2
+    assert abs(st["vel"][0, 2] - 3 * a_z * dt) < 1e-9 * 3 * a_z * dt
+    assert abs(st["pos"][0, 0] - 3 * vx0 * dt) < 1e-9 * 3 * vx0 * dt
+    assert abs(r.ramo[0] - c.q_0 * 3 * a_z * dt / d) < 1e-9 * c.q_0 * 3 * a_z * dt / d
+
+
+def test_steps_vs_reference_and_oracle(Diode, oracle_mod):
+    """Update_Position + Remove_Particles: the device matches the reference's
+    routines (golden) to 1e-12 with identical survivors/ids/counts, and the
+    oracle bit for bit."""
+    g = golden("steps.npz")
+    dd = Diode(256)
+    dd.set_planar(float(g["V"]), float(g["d"]), True, 1)
+    dd.set_time_step(float(g["dt"]))
+    so = oracle_mod.Sim(256, nthreads=2)
+    so.set_geometry(oracle_mod.geom_planar(float(g["d"]), float(g["V"]), True, 1))
+    so.set_time_step(float(g["dt"]))
+    n = len(g["species0"])
+    for i in range(n):
+        kw = dict(species=g["species0"][i:i + 1], step=int(g["birth0"][i]), emit=1)
+        dd.add_particles(g["pos0"][i:i + 1], g["vel0"][i:i + 1], **kw)
+        so.add(g["pos0"][i:i + 1], g["vel0"][i:i + 1], **kw)
+    for k in range(1, int(g["steps"]) + 1):
+        rg = dd.update_position(k)
+        ro = so.step(k, phases=2)
+        ref = g[f"ramo_{k}"]
+        assert abs(rg.ramo_current - ref[3]) <= 1e-12 * np.abs(ref[:3]).sum()
+        rg2 = dd.remove_particles(k)
+        so.step(k, phases=4)
+        rem = g[f"removed_{k}"]
+        assert rg2.absorbed == rem[0] and rg2.absorbed_top == rem[1] and rg2.absorbed_bot == rem[2]
+        sg = dd.get_state()
+        assert np.array_equal(sg["id"], g[f"id_{k}"])
+        for key in ("pos", "vel", "prev_pos", "acc_prev", "acc_prev2"):
+            assert rel_err(sg[key], g[f"{key}_{k}"], floor=1e-300).max() < 1e-12, (k, key)
+        assert_state_equal(sg, so.state(), f"step {k}")
+        assert list(rg.ramo) == list(ro.ramo)
+    assert np.array_equal(dd.life_time(), g["life_time"][:, :2])
+
+
+def _fe(target, oracle_mod, seed=5, V=1600.0, d=200 * NM, dt=0.5e-15, wf=((4.7,),), nic=1):
+    if hasattr(target, "set_planar"):
+        target.set_planar(V, d, True, nic)
+        target.set_time_step(dt)
+        target.set_seed(seed)
+        target.set_emitter(10, 0, -50 * NM, -50 * NM, 100 * NM, 100 * NM, wf=np.array(wf))
+    else:
+        target.set_geometry(oracle_mod.geom_planar(d, V, True, nic))
+        target.set_time_step(dt)
+        target.set_seed(seed)
+        target.set_emitter(10, 0, -50 * NM, -50 * NM, 100 * NM, 100 * NM, wf=np.array(wf))
+
+
+def test_planar_fe_run_bit_exact(Diode, oracle_mod):
+    """Planar field emission with space charge and absorption: every count,
+    record and particle coordinate of the FP64 device run equals the oracle."""
+    dd = Diode(4096)
+    so = oracle_mod.Sim(4096, nthreads=8)
+    _fe(dd, oracle_mod)
+    _fe(so, oracle_mod)
+    nsteps = 45
+    recs = dd.run(nsteps, "fp64")
+    absorbed = 0
+    for k in range(1, nsteps + 1):
+        ro = so.step(k)
+        print(f"FE step {k}: n={recs[k-1].n} emitted={recs[k-1].emitted}", flush=True)
+        assert_records_equal(recs[k - 1], ro, f"step {k}")
+        absorbed += ro.absorbed
+    assert absorbed > 0 and dd.n > 100
+    assert_state_equal(dd.get_state(), so.state(), "end")
+    assert np.array_equal(dd.life_time(), so.life_time())
+
+
+def test_planar_gtf_run_bit_exact(Diode, oracle_mod):
+    """Thermal-field emission, checkerboard work function, Poisson supply,
+    Maxwell-Boltzmann velocities."""
+    def setup(t):
+        wf = np.array([[4.0, 4.5], [4.5, 4.0]])
+        if hasattr(t, "set_planar"):
+            t.set_planar(6000.0, 1000 * NM, True, 0)
+            t.set_time_step(0.25e-15)
+            t.set_seed(7)
+            t.set_emitter(9, 0, -100 * NM, -100 * NM, 200 * NM, 200 * NM, T=1500.0, wf=wf)
+        else:
+            t.set_geometry(oracle_mod.geom_planar(1000 * NM, 6000.0, True, 0))
+            t.set_time_step(0.25e-15)
+            t.set_seed(7)
+            t.set_emitter(9, 0, -100 * NM, -100 * NM, 200 * NM, 200 * NM, T=1500.0, wf=wf)
+
+    dd = Diode(4096)
+    so = oracle_mod.Sim(4096, nthreads=8)
+    setup(dd)
+    setup(so)
+    recs = dd.run(20, "fp64")
+    for k in range(1, 21):
+        assert_records_equal(recs[k - 1], so.step(k), f"step {k}")
+    st = dd.get_state()
+    assert_state_equal(st, so.state(), "end")
+    assert set(np.unique(st["section"])) <= {1, 2, 3, 4} and dd.n > 100
+
+
+def test_tip_fe_run_bit_exact(Diode, oracle_mod):
+    """Hyperboloid tip field emission (100 x 100 surface supply, MH in
+    (xi, phi), emission along the surface normal, tip boundary)."""
+    V, dtip, rb, ht = 1000.0, 1000 * NM, 250 * NM, 500 * NM
+    dd = Diode(4096)
+    dd.set_tip(V, dtip, rb, ht, True, box_top=1000 * NM)
+    dd.set_time_step(0.25e-16)
+    dd.set_seed(9)
+    dd.set_emitter(3, 0, w_tip=4.7)
+    so = oracle_mod.Sim(4096, nthreads=8)
+    so.set_geometry(oracle_mod.geom_tip(V, dtip, rb, ht, True), 1000 * NM)
+    so.set_time_step(0.25e-16)
+    so.set_seed(9)
+    so.set_emitter(3, 0, w_tip=4.7)
+    recs = dd.run(6, "fp64")
+    for k in range(1, 7):
+        assert_records_equal(recs[k - 1], so.step(k), f"step {k}")
+    assert_state_equal(dd.get_state(), so.state(), "end")
+
+
+def test_pool_alone_equals_in_the_mix(oracle_mod):
+    """Three different diodes advanced together in one pool give exactly the
+    results each gives alone (the oracle runs each alone)."""
+    from rumdeed_amd.diode import Pool
+
+    cfgs = [dict(seed=5, V=1600.0, d=200 * NM), dict(seed=6, V=1500.0, d=180 * NM),
+            dict(seed=7, V=1700.0, d=220 * NM, wf=((4.6, 4.8),))]
+    pool = Pool(3 * 2048, 8)
+    slots = []
+    oracles = []
+    for cfg in cfgs:
+        s = pool.open(2048)
+        _fe(pool, oracle_mod, **cfg)
+        slots.append(s)
+        so = oracle_mod.Sim(2048, nthreads=8)
+        _fe(so, oracle_mod, **cfg)
+        oracles.append(so)
+    for k in range(1, 16):
+        recs = pool.step("fp64")
+        for s, so in zip(slots, oracles):
+            assert_records_equal(recs[s], so.step(k), f"slot {s} step {k}")
+    for s, so in zip(slots, oracles):
+        pool.select(s)
+        assert_state_equal(pool.get_state(), so.state(), f"slot {s}")
+
+
+def test_mixed_precision_run_statistics(Diode, oracle_mod):
+    """The mixed-precision step follows the FP64 step statistically: same
+    supply within 1e-5, emitted totals within counting noise."""
+    d64 = Diode(4096)
+    d32 = Diode(4096)
+    _fe(d64, oracle_mod, seed=11)
+    _fe(d32, oracle_mod, seed=11)
+    r64 = d64.run(30, "fp64")
+    r32 = d32.run(30, "mixed")
+    e64 = sum(r.emitted for r in r64)
+    e32 = sum(r.emitted for r in r32)
+    assert abs(r64[0].n_sup - r32[0].n_sup) < 1e-5 * r64[0].n_sup
+    assert abs(e64 - e32) < 5 * np.sqrt(e64 + 1)
