@@ -66,8 +66,105 @@ def cpu_baseline(n_sample: int, nic: int, threads: int):
                       f"{dt:.2f} s"}
 
 
+def serve_bench(n, precision, realtime, steps, rate):
+    """The batched serving stream (Poisson arrivals, continuous batching)."""
+    from rumdeed_amd.serve import Server, stream_requests
+
+    reqs = stream_requests(n=n, rate=rate, steps=steps, regime="intended")
+    srv = Server(total_particles=1 << 23, max_slots=512, hbm_budget=96 << 30,
+                 precision=precision)
+    done = []
+
+    def progress(res, k):
+        if k % 16 == 0 or k == n:
+            print(f"[serve] {k}/{n} done, last latency {res.latency:.2f} s", file=sys.stderr,
+                  flush=True)
+        done.append(res)
+
+    results, stats = srv.run(reqs, realtime=realtime, progress=progress)
+    srv.close()
+    emitted = np.array([r.emitted for r in results])
+    nfin = np.array([r.n_final for r in results])
+    stats.update({"precision": precision, "realtime_arrivals": realtime, "steps_per_request": steps,
+                  "arrival_rate_per_s": rate,
+                  "emitted_total_median": float(np.median(emitted)),
+                  "emitted_total_max": int(emitted.max()),
+                  "electrons_final_median": float(np.median(nfin)),
+                  "electrons_final_max": int(nfin.max()),
+                  "dropped_total": int(sum(r.dropped for r in results))})
+    return stats
+
+
+E2E = {
+    # intended-regime variants of the end-to-end configurations
+    "planar_fe": dict(kind="planar", V=8000.0, d=1000e-9, side=100e-9, phi=4.7, dt=0.25e-15,
+                      steps=4000, seed=1, nic=0, mode=10),
+    "tip_fe": dict(kind="tip", V=1000.0, d_tip=1000e-9, R_base=250e-9, h_tip=500e-9,
+                   box_top=1500e-9, phi=4.5, dt=0.25e-15, steps=4000, seed=1, mode=3),
+    "checker_gtf": dict(kind="planar", V=6000.0, d=1000e-9, side=1000e-9, T=1500.0, dt=0.25e-15,
+                        steps=2000, seed=7, nic=0, mode=9),
+}
+
+
+def e2e_bench(name, steps, precision):
+    from rumdeed_amd.diode import Diode
+
+    cfg = dict(E2E[name])
+    if steps:
+        cfg["steps"] = steps
+    dd = Diode(1 << 18)
+    if cfg["kind"] == "planar":
+        dd.set_planar(cfg["V"], cfg["d"], True, cfg["nic"])
+    else:
+        dd.set_tip(cfg["V"], cfg["d_tip"], cfg["R_base"], cfg["h_tip"], True,
+                   box_top=cfg["box_top"])
+    dd.set_time_step(cfg["dt"])
+    dd.set_seed(cfg["seed"])
+    if cfg["mode"] == 10:
+        dd.set_emitter(10, 0, -cfg["side"] / 2, -cfg["side"] / 2, cfg["side"], cfg["side"],
+                       wf=np.array([[cfg["phi"]]]))
+    elif cfg["mode"] == 9:
+        wf = np.where((np.arange(10)[:, None] + np.arange(10)[None, :]) % 2 == 0, 4.0, 4.5)
+        dd.set_emitter(9, 0, -cfg["side"] / 2, -cfg["side"] / 2, cfg["side"], cfg["side"],
+                       T=cfg["T"], wf=wf)
+    else:
+        dd.set_emitter(3, 0, w_tip=cfg["phi"])
+    t0 = time.perf_counter()
+    recs = []
+    chunk = 100
+    done = 0
+    pairs = 0
+    while done < cfg["steps"]:
+        k = min(chunk, cfg["steps"] - done)
+        rr = dd.run(k, precision)
+        recs += rr
+        pairs += sum(r.n * (r.n - 1) for r in rr)
+        done += k
+        print(f"[e2e {name}] step {done}: n={rr[-1].n}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    half = recs[len(recs) // 2:]
+    out = {"workload": name, "precision": precision, "steps": cfg["steps"],
+           "wall_s": wall, "ms_per_step": wall / cfg["steps"] * 1e3,
+           "force_pair_interactions_per_s": pairs / wall,
+           "electrons_final": recs[-1].n, "electrons_max": max(r.n for r in recs),
+           "emitted_total": sum(r.emitted for r in recs),
+           "absorbed_total": sum(r.absorbed for r in recs),
+           "ramo_current_second_half_A": float(np.mean([r.ramo_current for r in half])),
+           "config": {k: v for k, v in cfg.items()}}
+    dd.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--serve", type=int, default=0, help="serve N requests of the stream")
+    ap.add_argument("--serve-steps", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-precision", default="mixed")
+    ap.add_argument("--no-realtime", action="store_true")
+    ap.add_argument("--e2e", default="", help="end-to-end workload: " + ", ".join(E2E))
+    ap.add_argument("--e2e-steps", type=int, default=0)
+    ap.add_argument("--e2e-precision", default="mixed")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -84,6 +181,16 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.serve:
+        print(json.dumps({"serving": serve_bench(args.serve, args.serve_precision,
+                                                 not args.no_realtime, args.serve_steps,
+                                                 args.serve_rate)}), flush=True)
+        return
+    if args.e2e:
+        print(json.dumps({"e2e": e2e_bench(args.e2e, args.e2e_steps, args.e2e_precision)}),
+              flush=True)
+        return
 
     from rumdeed_amd import workloads as W
     from rumdeed_amd.diode import Diode

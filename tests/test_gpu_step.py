@@ -211,3 +211,24 @@ def test_mixed_precision_run_statistics(Diode, oracle_mod):
     e32 = sum(r.emitted for r in r32)
     assert abs(r64[0].n_sup - r32[0].n_sup) < 1e-5 * r64[0].n_sup
     assert abs(e64 - e32) < 5 * np.sqrt(e64 + 1)
+
+
+def test_serving_alone_equals_in_the_mix():
+    """Continuous batching: requests served together (overlapping in the
+    pool) give the same per-step records as each request served alone."""
+    from rumdeed_amd.serve import Request, Server
+
+    reqs = [Request(rid=i, V=v, gap=g * NM, side=s * NM, phi=p, steps=st, seed=i)
+            for i, (v, g, s, p, st) in enumerate([(1600.0, 200, 100, 4.7, 30),
+                                                  (1500.0, 180, 60, 4.5, 20),
+                                                  (1700.0, 220, 80, 4.6, 25)])]
+    srv = Server(total_particles=1 << 16, max_slots=8, precision="fp64")
+    mix, stats = srv.run(reqs, realtime=False, keep_records=True)
+    srv.close()
+    assert stats["max_active"] == 3
+    for r in reqs:
+        s1 = Server(total_particles=1 << 15, max_slots=2, precision="fp64")
+        alone, _ = s1.run([r], realtime=False, keep_records=True)
+        s1.close()
+        assert alone[0].records == mix[r.rid].records, r.rid
+        assert alone[0].emitted == mix[r.rid].emitted > 0
