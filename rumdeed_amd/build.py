@@ -55,5 +55,38 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN_DIR = os.path.join(PKG, "fortran")
+FORTRAN_TESTS = os.path.join(FORTRAN_DIR, "rd_fortran_tests")
+
+
+def flang() -> str | None:
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang", "/opt/rocm/bin/amdflang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the ISO_C_BINDING layer and its test driver with flang and
+    link them against librumdeed.so (found next to the package at run time)."""
+    fc = flang()
+    if fc is None:
+        return None
+    bdir = os.path.join(FORTRAN_DIR, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    cmd = [fc, "-O2", "-J", bdir, os.path.join(FORTRAN_DIR, "rumdeed_hip.f90"),
+           os.path.join(FORTRAN_DIR, "rd_fortran_tests.f90"), "-L", PKG, "-lrumdeed",
+           "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_TESTS + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("flang failed building the Fortran layer")
+    os.replace(FORTRAN_TESTS + ".tmp", FORTRAN_TESTS)
+    return FORTRAN_TESTS
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_fortran(verbose=True))
