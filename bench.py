@@ -33,6 +33,15 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_VALU_TFLOPS = 157.3
 PEAK_FP64_VALU_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per launch
+
+# HBM bytes per dispatch of the dominant kernel from the FETCH_SIZE and
+# WRITE_SIZE counter passes over `bench.py --profile-pass` (one rocprofv3
+# --pmc run per counter), each scaled by what the counter reads for the 1 GiB
+# calibration copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot be
+# read from inside this process, so the figures are those of the committed
+# summary profiles/snapshot65536_nic10_hbm_counters.txt, for that workload only.
+MEASURED_TRAFFIC = {("k_accel_planar32", 65536, 10): 16.819e6 + 12.583e6}
 
 
 def cpu_baseline(n_sample: int, nic: int, threads: int):
@@ -61,9 +70,18 @@ def cpu_baseline(n_sample: int, nic: int, threads: int):
         kind = "port"
         what = "oracle FP64 gather restatement (gcc -O2 -fopenmp)"
     pairs = n_sample * (n_sample - 1)
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": pairs / dt, "unit": "pair-interactions/s", "cores": threads, "kind": kind,
             "sample": f"one force evaluation, N={n_sample}, N_ic_max={nic}, {what}, "
-                      f"{dt:.2f} s"}
+                      f"{dt:.2f} s, {threads} OpenMP threads on {cpu}"}
 
 
 def serve_bench(n, precision, realtime, steps, rate):
@@ -143,6 +161,14 @@ def e2e_bench(name, steps, precision):
         print(f"[e2e {name}] step {done}: n={rr[-1].n}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     half = recs[len(recs) // 2:]
+    force_err = None
+    if cfg["kind"] == "planar" and dd.n > 1:
+        # the final snapshot: mixed-precision accelerations against plain FP64
+        a64 = dd.calculate_acceleration_particles("fp64")
+        a32 = dd.calculate_acceleration_particles("mixed")
+        rel = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
+        force_err = {"n": int(dd.n), "max_rel": float(rel.max()),
+                     "rms_rel": float(np.sqrt(np.mean(rel ** 2)))}
     out = {"workload": name, "precision": precision, "steps": cfg["steps"],
            "wall_s": wall, "ms_per_step": wall / cfg["steps"] * 1e3,
            "force_pair_interactions_per_s": pairs / wall,
@@ -150,13 +176,41 @@ def e2e_bench(name, steps, precision):
            "emitted_total": sum(r.emitted for r in recs),
            "absorbed_total": sum(r.absorbed for r in recs),
            "ramo_current_second_half_A": float(np.mean([r.ramo_current for r in half])),
+           "final_accel_mixed_vs_fp64": force_err,
            "config": {k: v for k, v in cfg.items()}}
     dd.close()
     return out
 
 
+def profile_pass(n, nic, G, reps):
+    """A short run for hardware-counter passes: `reps` launches of each
+    snapshot kernel, then a plain 1 GiB device-to-device copy (three times)
+    whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    from rumdeed_amd import workloads as W
+    from rumdeed_amd._lib import check
+    from rumdeed_amd.diode import Diode
+
+    pos, q, m = W.snapshot(n, seed=2)
+    dd = Diode(n + 64)
+    dd.set_planar(W.SNAPSHOT_V, W.SNAPSHOT_D, True, nic)
+    dd.put_particles(pos, q, m)
+    box = W.SNAPSHOT_BOX
+    for prec in (1, 0):
+        for _ in range(reps if prec == 1 else 1):
+            check(dd._lib.rd_accel(dd._h, prec))
+            check(dd._lib.rd_field_grid(dd._h, box[0], box[1], box[2], box[3], G, G, 0.0, prec,
+                                        None))
+    print(f"[profile] snapshot kernels done: n={n} nic={nic} grid={G} reps={reps}", flush=True)
+    ms = dd.time_copy(COPY_BYTES, reps=2)
+    dd.close()  # synchronises the context's stream before freeing
+    print(f"[profile] calibration copy k_copy16 done: {COPY_BYTES} bytes read and written per "
+          f"launch, {ms:.3f} ms, {2 * COPY_BYTES / ms / 1e6:.0f} GB/s", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="short run of the snapshot kernels plus a 1 GiB calibration copy")
     ap.add_argument("--serve", type=int, default=0, help="serve N requests of the stream")
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -182,6 +236,9 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.profile_pass:
+        profile_pass(args.n, args.nic, args.grid, args.steps)
+        return
     if args.serve:
         print(json.dumps({"serving": serve_bench(args.serve, args.serve_precision,
                                                  not args.no_realtime, args.serve_steps,
@@ -227,6 +284,7 @@ def main():
     achieved = flops / (ker_ms * 1e-3) / 1e12
     ftot, fker = dd.time_field_grid(*box, G, G, 0.0, "mixed", reps=3)
     t64, k64 = dd.time_accel("fp64", reps=2)
+    copy_ms = dd.time_copy(COPY_BYTES, reps=3)
 
     value = interactions / (ms_mixed * 1e-3)
     out = {
@@ -254,9 +312,11 @@ def main():
         "kernels_ms": {"accel_mixed": ker_ms, "accel_mixed_total": tot_ms,
                        "field_grid_mixed": fker, "field_grid_mixed_total": ftot,
                        "accel_fp64": k64},
+        "hbm_copy_gbs": 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
-                     "traffic": None,
+                     "traffic": MEASURED_TRAFFIC.get(("k_accel_planar32", n, nic)),
+                     "traffic_unit": "bytes per launch (HBM read + write, counter passes)",
                      "kernel": "k_accel_planar32",
                      "flops_per_launch": flops,
                      "flops_per_interaction": W.pair_flops(nic)},

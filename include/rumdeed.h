@@ -205,6 +205,11 @@ int rd_time_accel(rd_ctx* ctx, int32_t precision, int32_t reps, double* total_ms
 int rd_time_field_grid(rd_ctx* ctx, double x0, double y0, double lx, double ly, int32_t nx,
                        int32_t ny, double z, int32_t precision, int32_t reps, double* total_ms,
                        double* kernel_ms);
+/* Average device time (ms) of a plain copy of `bytes` bytes (multiple of 16,
+ * at most 16 GiB) between two scratch buffers allocated for the call: the
+ * streaming-bandwidth probe and the known-size kernel against which HBM
+ * traffic counters are calibrated.  No reference counterpart. */
+int rd_time_copy(rd_ctx* ctx, int64_t bytes, int32_t reps, double* kernel_ms);
 
 #ifdef __cplusplus
 }

@@ -113,6 +113,7 @@ _SIGS = {
     "rd_time_accel": (c_int32, [c_void_p, c_int32, c_int32, _dp, _dp]),
     "rd_time_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double,
                                      c_int32, c_int32, c_double, c_int32, c_int32, _dp, _dp]),
+    "rd_time_copy": (c_int32, [c_void_p, c_int64, c_int32, _dp]),
 }
 
 _lib = None

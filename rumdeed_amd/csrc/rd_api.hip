@@ -843,3 +843,51 @@ int rd_time_field_grid(rd_ctx* c, double x0, double y0, double lx, double ly, in
 }
 
 }  // extern "C"
+
+// Plain device-to-device copy: 16 B per lane and iteration, grid-stride.
+__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src,
+                                                float4* __restrict__ dst, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
+struct CopyArg {
+  const float4* src;
+  float4* dst;
+  int64_t n16;
+};
+static int copy_body(rd_ctx* c, void* a, hipEvent_t k0, hipEvent_t k1) {
+  CopyArg* g = (CopyArg*)a;
+  if (k0) RD_HIP(hipEventRecord(k0, c->stream));
+  k_copy16<<<4096, 256, 0, c->stream>>>(g->src, g->dst, g->n16);
+  if (k1) RD_HIP(hipEventRecord(k1, c->stream));
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
+extern "C" {
+
+int rd_time_copy(rd_ctx* c, int64_t bytes, int32_t reps, double* kernel_ms) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  if (bytes < 16 || bytes % 16 != 0 || bytes > (int64_t(1) << 34))
+    return fail(RD_ERR_ARG, "bytes must be a multiple of 16 in [16, 16 GiB]");
+  RD_HIP(hipSetDevice(c->device));
+  void *src = nullptr, *dst = nullptr;
+  hipError_t e = hipMalloc(&src, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(src, 0, bytes, c->stream);
+  int r = RD_OK;
+  if (e != hipSuccess) {
+    r = hip_fail(e, "copy buffers");
+  } else {
+    CopyArg a{(const float4*)src, (float4*)dst, bytes / 16};
+    r = time_loop(c, reps, nullptr, kernel_ms, copy_body, &a);
+    hipStreamSynchronize(c->stream);
+  }
+  if (src) hipFree(src);
+  if (dst) hipFree(dst);
+  return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+// Pair and field arithmetic shared by the pair kernels and the emission
+// engine: the FP64 forms of the reference expressions (planar image series,
+// hyperboloid tip vacuum field and sphere image) and the mixed-precision
+// packed records and image weights.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rd_internal.h"
+
+namespace rd {
+
+// FP64 device forms of the reference expressions.
+__device__ __forceinline__ void ic_planar64(double z_a, double z_b, double dx, double dy,
+                                            double dxy2, int nic, double dd, double es,
+                                            double& icx, double& icy, double& icz) {
+  double z_ic = -1.0 * z_b;
+  double diff_z = z_a - z_ic;
+  double r = __builtin_sqrt(dxy2 + diff_z * diff_z) + es;
+  double inv_r3 = 1.0 / (r * r * r);
+  double ix = -dx * inv_r3, iy = -dy * inv_r3, iz = -diff_z * inv_r3;
+  for (int n = 1; n <= nic; ++n) {
+    double tnd = 2.0 * n * dd;
+    z_ic = tnd - z_b;
+    diff_z = z_a - z_ic;
+    r = __builtin_sqrt(dxy2 + diff_z * diff_z) + es;
+    inv_r3 = 1.0 / (r * r * r);
+    ix = ix - dx * inv_r3;
+    iy = iy - dy * inv_r3;
+    iz = iz - diff_z * inv_r3;
+    z_ic = -2.0 * n * dd - z_b;
+    diff_z = z_a - z_ic;
+    r = __builtin_sqrt(dxy2 + diff_z * diff_z) + es;
+    inv_r3 = 1.0 / (r * r * r);
+    ix = ix - dx * inv_r3;
+    iy = iy - dy * inv_r3;
+    iz = iz - diff_z * inv_r3;
+    z_ic = tnd + z_b;
+    diff_z = z_a - z_ic;
+    r = __builtin_sqrt(dxy2 + diff_z * diff_z) + es;
+    inv_r3 = 1.0 / (r * r * r);
+    ix = ix + dx * inv_r3;
+    iy = iy + dy * inv_r3;
+    iz = iz + diff_z * inv_r3;
+    z_ic = -2.0 * n * dd + z_b;
+    diff_z = z_a - z_ic;
+    r = __builtin_sqrt(dxy2 + diff_z * diff_z) + es;
+    inv_r3 = 1.0 / (r * r * r);
+    ix = ix + dx * inv_r3;
+    iy = iy + dy * inv_r3;
+    iz = iz + diff_z * inv_r3;
+  }
+  icx = ix;
+  icy = iy;
+  icz = iz;
+}
+
+// Tip vacuum field (acc_tip_field_E.inc:13-34); cos/sin of atan2(y,x) are
+// taken as x/rho, y/rho.
+__device__ __host__ __forceinline__ void tip_vacuum64(const DevGeom& g, double x, double y,
+                                                      double z, double& fx, double& fy,
+                                                      double& fz) {
+  double af = g.a_foci, sz = g.shift_z;
+  double r_p = __builtin_sqrt(x * x + y * y + (z + af - sz) * (z + af - sz));
+  double r_m = __builtin_sqrt(x * x + y * y + (z - af - sz) * (z - af - sz));
+  double xi = (r_p + r_m) / (2.0 * af);
+  double eta = (r_p - r_m) / (2.0 * af);
+  double c, s;
+  if ((__builtin_fabs(x) < 1.0e-18) && (__builtin_fabs(y) < 1.0e-18)) {
+    c = 1.0;
+    s = 0.0;
+  } else {
+    double rho = __builtin_sqrt(x * x + y * y);
+    c = x / rho;
+    s = y / rho;
+  }
+  double pre = g.pre_E * 1.0 / (xi * xi - eta * eta);
+  double fac_xy;
+  if (__builtin_fabs(xi - 1.0) < 1.0e-6)
+    fac_xy = 0.0;
+  else
+    fac_xy = eta * __builtin_sqrt((xi * xi - 1.0) / (1.0 - eta * eta));
+  fx = -1.0 * pre * fac_xy * c;
+  fy = -1.0 * pre * fac_xy * s;
+  fz = pre * xi;
+}
+
+// Vacuum field of either geometry.
+__device__ __forceinline__ void vacuum64(const DevGeom& g, double x, double y, double z,
+                                         double& fx, double& fy, double& fz) {
+  if (g.kind == RD_GEOM_TIP) {
+    tip_vacuum64(g, x, y, z, fx, fy, fz);
+  } else {
+    fx = 0.0;
+    fy = 0.0;
+    fz = g.Ez;
+  }
+}
+
+// Sphere image point of the particle at a (acc_tip_image_point.inc:14-18).
+__device__ __forceinline__ void tip_image_point64(double x_a, double y_a, double z_a, double z0,
+                                                  double R, double& dis_a, double& x_im,
+                                                  double& y_im, double& z_im) {
+  double dz = z_a - z0;
+  double da = __builtin_sqrt(x_a * x_a + y_a * y_a + dz * dz);
+  double zi =
+      z0 + R * R / (__builtin_sqrt(1 + x_a * x_a / (dz * dz) + y_a * y_a / (dz * dz)) * da);
+  z_im = zi;
+  x_im = (zi - z0) * x_a / dz;
+  y_im = (zi - z0) * y_a / dz;
+  dis_a = da;
+}
+
+// Sphere image force (acc_tip_ic_force.inc:17-22), t**1.5 = t*sqrt(t).
+__device__ __forceinline__ void tip_ic_force64(double x_a, double y_a, double z_a, double x_b,
+                                               double y_b, double z_b, double dis_a,
+                                               double x_im, double y_im, double z_im, double R,
+                                               double kt, double& icx, double& icy,
+                                               double& icz) {
+  double ta = (x_b - x_a) * (x_b - x_a) + (y_b - y_a) * (y_b - y_a) + (z_b - z_a) * (z_b - z_a);
+  double tb = (x_b - x_im) * (x_b - x_im) + (y_b - y_im) * (y_b - y_im) +
+              (z_b - z_im) * (z_b - z_im);
+  double tda = ta * __builtin_sqrt(ta);
+  double tdb = tb * __builtin_sqrt(tb);
+  icx = kt * ((x_a - x_b) / tda - (R * (x_im - x_b)) / (dis_a * tdb));
+  icy = kt * ((y_a - y_b) / tda - (R * (y_im - y_b)) / (dis_a * tdb));
+  icz = kt * ((z_a - z_b) / tda - (R * (z_im - z_b)) / (dis_a * tdb));
+}
+
+// ------------------------------------------------------------------------
+// Mixed precision.
+//
+// Packed record (nm): hi = (x_hi, y_hi, z_hi, q/q_0), lo = (x_lo, y_lo, z_lo,
+// d - z) with x = x_hi + x_lo to ~2^-48 relative.
+__device__ __forceinline__ void pack1(double xm, double ym, double zm, double qv, double d,
+                                      float4& hi, float4& lo) {
+  const double s = 1.0e9;
+  double xn = xm * s, yn = ym * s, zn = zm * s;
+  float xh = (float)xn, yh = (float)yn, zh = (float)zn;
+  hi = make_float4(xh, yh, zh, (float)(qv / Consts::q_0));
+  lo = make_float4((float)(xn - (double)xh), (float)(yn - (double)yh), (float)(zn - (double)zh),
+                   (float)(d * s - zn));
+}
+
+__device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
+
+// Guard for squared distances that can be exactly zero (a pair with itself
+// or coincident particles, a particle exactly on an electrode): keeps every
+// FP32 term finite; the reference's r + 1e-18 m softening gives the same 0
+// contribution for zero offsets.
+constexpr float kR2Min = 1.0e-20f;
+
+// One image partner: accumulate 1/r^3 and dz/r^3.
+#define RD_PARTNER(DZ, W, WZ)                   \
+  {                                             \
+    float _dz = (DZ);                           \
+    float _r2 = __builtin_fmaf(_dz, _dz, dxy2); \
+    float _ri = rsq(_r2);                       \
+    float _r3 = _ri * _ri * _ri;                \
+    W += _r3;                                   \
+    WZ = __builtin_fmaf(_dz, _r3, WZ);          \
+  }
+
+// Image-charge weights of one pair: returns (sum_same - sum_opp) of 1/r^3 in
+// wxy and of dz/r^3 in wz.  zsum = z_a + z_b, wsum = (d - z_a) + (d - z_b),
+// dzab = z_a - z_b.
+__device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, float dxy2,
+                                            int nic, float two_d, float& wxy, float& wz) {
+  float wo = 0.f, wzo = 0.f, ws = 0.f, wzs = 0.f;
+  {
+    float dz0 = zsum;
+    float r2 = fmaxf(__builtin_fmaf(dz0, dz0, dxy2), kR2Min);
+    float ri = rsq(r2);
+    float r3 = ri * ri * ri;
+    wo += r3;
+    wzo = __builtin_fmaf(dz0, r3, wzo);
+  }
+  if (nic >= 1) {
+    // n = 1: the opposite partner across the anode is at distance -(wsum)
+    float dz1 = -wsum;
+    float r2 = fmaxf(__builtin_fmaf(dz1, dz1, dxy2), kR2Min);
+    float ri = rsq(r2);
+    float r3 = ri * ri * ri;
+    wo += r3;
+    wzo = __builtin_fmaf(dz1, r3, wzo);
+    RD_PARTNER(zsum + two_d, wo, wzo);
+    RD_PARTNER(dzab - two_d, ws, wzs);
+    RD_PARTNER(dzab + two_d, ws, wzs);
+  }
+  float tn = two_d;
+  for (int n = 2; n <= nic; ++n) {
+    tn += two_d;
+    RD_PARTNER(zsum - tn, wo, wzo);
+    RD_PARTNER(zsum + tn, wo, wzo);
+    RD_PARTNER(dzab - tn, ws, wzs);
+    RD_PARTNER(dzab + tn, ws, wzs);
+  }
+  wxy = ws - wo;
+  wz = wzs - wzo;
+}
+
+// ------------------------------------------------------------------------
+// Field at one target per lane from the sources of chunks [c0, c1).  Every
+// lane of the workgroup takes part in the LDS staging whether or not it has
+// a target (`act`), so the calls sit in uniform control flow.
+
+// FP64: one partial per 256-source chunk, P[k * S + c * m + p] for the
+// components k = 0, 1, 2 (the canonical order: sources within a chunk in
+// index order, chunk partials added in chunk order by the finishing step).
+template <int KIND>
+__device__ __forceinline__ void field_chunks64(const DevGeom& g, bool act, double x1, double y1,
+                                               double z1, const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ z,
+                                               const double* __restrict__ q, int n, int c0,
+                                               int c1, double* sx, double* sy, double* sz,
+                                               double* sq, double* __restrict__ P, size_t S,
+                                               int m, int p) {
+  const int t = threadIdx.x;
+  double da = 0, xim = 0, yim = 0, zim = 0;
+  if (KIND == RD_GEOM_TIP && act && g.ic) tip_image_point64(x1, y1, z1, g.z0, g.R, da, xim, yim, zim);
+  for (int c = c0; c < c1; ++c) {
+    const int jb = c * kChunk;
+    __syncthreads();
+    if (jb + t < n) {
+      sx[t] = x[jb + t];
+      sy[t] = y[jb + t];
+      sz[t] = z[jb + t];
+      sq[t] = q[jb + t];
+    }
+    __syncthreads();
+    const int cnt = min(kChunk, n - jb);
+    double fxs = 0.0, fys = 0.0, fzs = 0.0;
+    if (act) {
+      for (int jj = 0; jj < cnt; ++jj) {
+        double x2 = sx[jj], y2 = sy[jj], z2 = sz[jj];
+        double pre = sq[jj] * g.dfc;
+        double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+        double ax, ay, az;
+        if (KIND == RD_GEOM_PLANAR) {
+          double dxy2 = dx * dx + dy * dy;
+          double r = __builtin_sqrt(dxy2 + dz * dz) + g.es;
+          double inv_r3 = 1.0 / (r * r * r);
+          ax = pre * inv_r3 * dx;
+          ay = pre * inv_r3 * dy;
+          az = pre * inv_r3 * dz;
+          if (g.ic) {
+            double icx, icy, icz;
+            ic_planar64(z1, z2, dx, dy, dxy2, g.nic, g.d, g.es, icx, icy, icz);
+            ax = ax + pre * icx;
+            ay = ay + pre * icy;
+            az = az + pre * icz;
+          }
+        } else {
+          double r = __builtin_sqrt(dx * dx + dy * dy + dz * dz) + g.es;
+          double inv_r3 = 1.0 / (r * r * r);
+          ax = pre * inv_r3 * dx;
+          ay = pre * inv_r3 * dy;
+          az = pre * inv_r3 * dz;
+          if (g.ic) {
+            double icx, icy, icz;
+            tip_ic_force64(x1, y1, z1, x2, y2, z2, da, xim, yim, zim, g.R, g.ktip, icx, icy, icz);
+            ax = ax + pre * icx;
+            ay = ay + pre * icy;
+            az = az + pre * icz;
+          }
+        }
+        fxs = fxs + ax;
+        fys = fys + ay;
+        fzs = fzs + az;
+      }
+      P[0 * S + (size_t)c * m + p] = fxs;
+      P[1 * S + (size_t)c * m + p] = fys;
+      P[2 * S + (size_t)c * m + p] = fzs;
+    }
+  }
+}
+
+// Field = E_vac + the chunk partials in chunk order.
+__device__ __forceinline__ void field_finish64(const DevGeom& g, const double* __restrict__ P,
+                                               size_t S, int m, int p, int nchunks, double x,
+                                               double y, double z, double& fx, double& fy,
+                                               double& fz) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    sx = sx + P[0 * S + (size_t)c * m + p];
+    sy = sy + P[1 * S + (size_t)c * m + p];
+    sz = sz + P[2 * S + (size_t)c * m + p];
+  }
+  double ex, ey, ez;
+  vacuum64(g, x, y, z, ex, ey, ez);
+  fx = ex + sx;
+  fy = ey + sy;
+  fz = ez + sz;
+}
+
+// Mixed (planar): FP64 sums, in nm^-2 per unit charge, of the packed sources
+// of chunks [c0, c1) at the packed target (h1, l1).
+__device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h1, float4 l1,
+                                            const float4* __restrict__ hi,
+                                            const float4* __restrict__ lo, int n, int c0, int c1,
+                                            float4* shi, float4* slo, double& fx, double& fy,
+                                            double& fz) {
+  const int ic = g.ic, nic = g.nic;
+  const float two_d = (float)(2.0 * g.d * 1e9);
+  const int t = threadIdx.x;
+  fx = 0.0;
+  fy = 0.0;
+  fz = 0.0;
+  for (int c = c0; c < c1; ++c) {
+    const int jb = c * kChunk;
+    __syncthreads();
+    if (jb + t < n) {
+      shi[t] = hi[jb + t];
+      slo[t] = lo[jb + t];
+    }
+    __syncthreads();
+    const int cnt = min(kChunk, n - jb);
+    if (act) {
+      for (int jj = 0; jj < cnt; ++jj) {
+        const float4 h2 = shi[jj], l2 = slo[jj];
+        float dx = (h1.x - h2.x) + (l1.x - l2.x);
+        float dy = (h1.y - h2.y) + (l1.y - l2.y);
+        float dz = (h1.z - h2.z) + (l1.z - l2.z);
+        float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+        float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
+        float ri = rsq(r2);
+        float r3 = ri * ri * ri;
+        float cxy = r3, cz = dz * r3;
+        if (ic) {
+          float wxy, wz;
+          ic_planar32(h1.z + h2.z, l1.w + l2.w, dz, dxy2, nic, two_d, wxy, wz);
+          cxy += wxy;
+          cz += wz;
+        }
+        cxy *= h2.w;
+        cz *= h2.w;
+        fx += (double)(cxy * dx);
+        fy += (double)(cxy * dy);
+        fz += (double)cz;
+      }
+    }
+  }
+}
+
+// Field (V/m) from the segment sums of a mixed evaluation, in segment order.
+__device__ __forceinline__ void field_finish32(const DevGeom& g, const double* __restrict__ P,
+                                               size_t S, int m, int p, int segs, double& fx,
+                                               double& fy, double& fz) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int s = 0; s < segs; ++s) {
+    sx += P[0 * S + (size_t)s * m + p];
+    sy += P[1 * S + (size_t)s * m + p];
+    sz += P[2 * S + (size_t)s * m + p];
+  }
+  double pre = g.dfc * Consts::q_0 * 1.0e18;
+  fx = pre * sx;
+  fy = pre * sy;
+  fz = g.Ez + pre * sz;
+}
+
+}  // namespace rd

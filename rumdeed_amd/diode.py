@@ -203,6 +203,13 @@ class Diode:
         return tot.value, ker.value
 
 
+    def time_copy(self, nbytes: int, reps: int = 3) -> float:
+        """Average time (ms) of a plain device copy of `nbytes` bytes."""
+        ker = c_double()
+        check(self._lib.rd_time_copy(self._h, int(nbytes), int(reps), ctypes.byref(ker)))
+        return ker.value
+
+
 def device_count() -> int:
     v = c_int32()
     code = L.lib().rd_device_count(ctypes.byref(v))

@@ -244,6 +244,9 @@ contains
   end function Calc_Field_at
 
   ! Add_Particle (one particle, appended to the device-resident state).
+  ! `life` (particles_life, src/mod_pair.F90:73) is read only by the ion
+  ! collision model (src/mod_collisions.F90:123), which this path does not
+  ! include; it is accepted for interface compatibility and not stored.
   subroutine Add_Particle(par_pos, par_vel, par_species, step, emit, life, opt_sec)
     double precision, intent(in) :: par_pos(3), par_vel(3)
     integer, intent(in) :: par_species, step, emit, life
@@ -252,7 +255,6 @@ contains
     sp(1) = int(par_species, c_int)
     sec(1) = 1
     if (present(opt_sec)) sec(1) = int(opt_sec, c_int)
-    if (life /= -1) continue
     call rd_check(rd_add_particles(rd_ctx, 1_c_int, par_pos, par_vel, sp, int(step, c_int), &
                   int(emit, c_int), sec), 'rd_add_particles')
   end subroutine Add_Particle
