@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "rd_field.h"
 #include "rd_internal.h"
 #include "rd_physics.h"
 
@@ -177,15 +178,20 @@ __global__ __launch_bounds__(256) void k_quad_reduce(const Ent* tab, const SimDe
                                                      const double* __restrict__ fx,
                                                      const double* __restrict__ fy,
                                                      const double* __restrict__ fz,
-                                                     double* __restrict__ qv) {
+                                                     double* __restrict__ qv,
+                                                     double* __restrict__ qn) {
+  constexpr int kTile = 512;
+  __shared__ double t0[kTile], t1[kTile], t2[kTile], t3[kTile];
   const Ent e = tab[blockIdx.x];
   const SimDesc& s = desc[e.slot];
   const EmitConsts ec = emit_consts();
+  const bool tip = s.mode == RD_EMIT_TIP_FE;
   for (int k = threadIdx.x; k < e.m; k += blockDim.x) {
     const int p = e.pbase + k;
     double val = 0.0, F;
-    if (s.mode == RD_EMIT_TIP_FE) {
+    if (tip) {
       F = field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]);
+      qn[p] = F;
       if (F >= 0.0) {
         val = 0.0;
       } else {
@@ -214,21 +220,38 @@ __global__ __launch_bounds__(256) void k_quad_reduce(const Ent* tab, const SimDe
     }
     qv[p] = val;
   }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  SimState& S = st[e.slot];
+  // The point-ordered sums, one lane adding tiles staged in LDS.
   double sum = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int k = 0; k < e.m; ++k) {
-    const int p = e.pbase + k;
-    sum = sum + qv[p];
-    if (s.mode == RD_EMIT_TIP_FE) {
-      f2 = f2 + field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]);
-    } else {
-      f0 = f0 + fx[p];
-      f1 = f1 + fy[p];
-      f2 = f2 + fz[p];
+  for (int b = 0; b < e.m; b += kTile) {
+    const int cnt = min(kTile, e.m - b);
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+      const int p = e.pbase + b + k;
+      t0[k] = qv[p];
+      if (tip) {
+        t3[k] = qn[p];
+      } else {
+        t1[k] = fx[p];
+        t2[k] = fy[p];
+        t3[k] = fz[p];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < cnt; ++k) {
+        sum = sum + t0[k];
+        if (tip) {
+          f2 = f2 + t3[k];
+        } else {
+          f0 = f0 + t1[k];
+          f1 = f1 + t2[k];
+          f2 = f2 + t3[k];
+        }
+      }
     }
   }
+  if (threadIdx.x != 0) return;
+  SimState& S = st[e.slot];
   int nr;
   if (s.mode == RD_EMIT_TIP_FE) {
     S.N_sup = sum;
@@ -376,24 +399,29 @@ __global__ __launch_bounds__(256) void k_mh_start_check(const Ent* tab, const Si
   }
 }
 
-// Gaussian proposal of jump `it` (1-based) for every started chain.
-__global__ void k_mh_propose(const Ent* tab, const SimDesc* desc, const SimState* st, Chains ch,
-                             int it, double* __restrict__ ptx, double* __restrict__ pty,
-                             double* __restrict__ ptz) {
-  const Ent e = tab[blockIdx.z];
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= e.nch) return;
-  const SimDesc& s = desc[e.slot];
-  if (it > s.ndim) return;
+// Gaussian proposal of jump `it` (1-based) for chain k of entry e, with the
+// width MH of the start of the jump.  (a, b) are the chain coordinates
+// (planar x, y; tip xi, phi) and (x, y, z) the surface point whose field the
+// accept step needs.  valid: 1 proposal inside the patch, 0 tip proposal
+// outside it, -1 chain without a start site (the point is its final site and
+// nothing is recorded).
+struct Proposal {
+  double a, b, x, y, z;
+  int valid;
+};
+
+__device__ __forceinline__ Proposal mh_proposal(const Ent& e, const SimDesc& s, double MH,
+                                                const Chains& ch, int it, int k) {
+  Proposal P;
   const int c = e.cbase + k;
-  const int p = e.pbase + k;
   if (ch.ok[c] != 1) {
-    ptx[p] = ch.ox[c];
-    pty[p] = ch.oy[c];
-    ptz[p] = ch.oz[c];
-    return;
+    P.a = P.b = 0.0;
+    P.x = ch.ox[c];
+    P.y = ch.oy[c];
+    P.z = ch.oz[c];
+    P.valid = -1;
+    return P;
   }
-  const double MH = st[e.slot].MH_std;
   const uint64_t tg = tag(kTagJump, (uint64_t)it);
   const double PI2 = 2.0 * 3.141592653589793238462643383279502884197;
   if (s.mode == RD_EMIT_TIP_FE) {
@@ -412,22 +440,17 @@ __global__ void k_mh_propose(const Ent* tab, const SimDesc* desc, const SimState
     double nphi = fmodulo(ch.cy[c] + d2, PI2);
     if (nxi > s.g.max_xi) nxi = 2.0 * s.g.max_xi - nxi;
     if (nxi < 1.0) nxi = 2.0 - nxi;
-    int valid = !((nxi < 1.0) || (nxi > s.g.max_xi));
-    ch.pv[c] = valid;
-    ch.px[c] = nxi;
-    ch.py[c] = nphi;
-    double x, y, z;
-    if (valid) {
-      tip_xyz(tip_surf(s.g), nxi, s.g.eta_1, nphi, x, y, z);
+    P.valid = !((nxi < 1.0) || (nxi > s.g.max_xi));
+    P.a = nxi;
+    P.b = nphi;
+    if (P.valid) {
+      tip_xyz(tip_surf(s.g), nxi, s.g.eta_1, nphi, P.x, P.y, P.z);
     } else {
-      x = ch.ox[c];
-      y = ch.oy[c];
-      z = ch.oz[c];
+      P.x = ch.ox[c];
+      P.y = ch.oy[c];
+      P.z = ch.oz[c];
     }
-    ptx[p] = x;
-    pty[p] = y;
-    ptz[p] = z;
-    return;
+    return P;
   }
   double sx, sy;
   if (s.mode == RD_EMIT_PLANAR_FE) {
@@ -462,26 +485,37 @@ __global__ void k_mh_propose(const Ent* tab, const SimDesc* desc, const SimState
     nx = sxs * s.elx + s.ex0;
     ny = sys * s.ely + s.ey0;
   }
-  ch.px[c] = nx;
-  ch.py[c] = ny;
-  ptx[p] = nx;
-  pty[p] = ny;
-  ptz[p] = 0.0;
+  P.a = nx;
+  P.b = ny;
+  P.x = nx;
+  P.y = ny;
+  P.z = 0.0;
+  P.valid = 1;
+  return P;
 }
 
-// Accept/reject of jump `it`; one block per simulation.
-__global__ __launch_bounds__(256) void k_mh_accept(const Ent* tab, const SimDesc* desc,
-                                                   SimState* st, const double* wf, Chains ch,
-                                                   int it, const double* __restrict__ ptx,
-                                                   const double* __restrict__ pty,
-                                                   const double* __restrict__ ptz,
-                                                   const double* __restrict__ fx,
-                                                   const double* __restrict__ fy,
-                                                   const double* __restrict__ fz) {
-  __shared__ int ra[256], rr[256];
-  const Ent e = tab[blockIdx.x];
-  const SimDesc& s = desc[e.slot];
-  if (it > s.ndim) return;
+// Record a proposal: the chain's proposed coordinates and its field point.
+__device__ __forceinline__ void mh_record(const Ent& e, const SimDesc& s, const Chains& ch, int k,
+                                          const Proposal& P, double* __restrict__ ptx,
+                                          double* __restrict__ pty, double* __restrict__ ptz) {
+  const int c = e.cbase + k;
+  const int p = e.pbase + k;
+  if (P.valid >= 0) {
+    if (s.mode == RD_EMIT_TIP_FE) ch.pv[c] = P.valid;
+    ch.px[c] = P.a;
+    ch.py[c] = P.b;
+  }
+  ptx[p] = P.x;
+  pty[p] = P.y;
+  ptz[p] = P.z;
+}
+
+// Accept/reject of jump `it` for every chain of entry e, by one block of 256
+// lanes (ra, rr: 256-entry LDS arrays for the batch acceptance counts).
+__device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, const double* wf,
+                                const Chains& ch, int it, const double* ptx, const double* pty,
+                                const double* ptz, const double* fx, const double* fy,
+                                const double* fz, int* ra, int* rr) {
   const EmitConsts ec = emit_consts();
   const int first = nint(s.ndim * 0.25);
   const bool tip = s.mode == RD_EMIT_TIP_FE;
@@ -560,6 +594,206 @@ __global__ __launch_bounds__(256) void k_mh_accept(const Ent* tab, const SimDesc
         S.MH_std = 0.00005;
     }
   }
+}
+
+// ------------------------------------------------------------------------
+// One Metropolis-Hastings jump per launch.
+//
+// Every workgroup proposes the jump for its own targets (the proposals are
+// counter-based random numbers, so all workgroups of a target agree), adds
+// the field of one source segment and writes its partial sums.  The last
+// workgroup of a target group to arrive (an atomic counter per group) adds
+// the partials, records the proposals and their fields; the last target
+// group of a simulation to finish (a counter per simulation) then runs the
+// accept step for all of that simulation's chains.  No workgroup waits for
+// another: the ones that are not last simply end, so the launch always
+// drains.  Each counter is reset by the workgroup that saw it complete, so
+// all counters are zero between launches.
+struct JumpArgs {
+  double *ptx, *pty, *ptz, *fx, *fy, *fz;
+  double* part;
+  const double *X, *Y, *Z, *Q;  // FP64 sources
+  const float4 *HI, *LO;        // packed sources (mixed)
+  int* cnt;                     // [ne][gridDim.x] group counters, then [ne] simulation counters
+};
+
+// True in every lane of the block that arrives last of `total` at *counter.
+// Release before the arrival, acquire after it (agent scope: the partials
+// written on other XCDs become visible).
+__device__ __forceinline__ bool last_arrival(int* counter, int total, int* flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) *flag = (atomicAdd(counter, 1) == total - 1);
+  __syncthreads();
+  const bool last = *flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// FP64: one lane per chain, sources in 256-source chunks (the canonical
+// order of the field at points, bit-identical to the non-fused path).
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_mh_jump64(const Ent* tab, const SimDesc* desc,
+                                                      SimState* st, const double* wf, Chains ch,
+                                                      int it, JumpArgs a) {
+  __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
+  __shared__ int ra[256], rr[256];
+  __shared__ int flag;
+  const Ent e = tab[blockIdx.z];
+  const SimDesc& s = desc[e.slot];
+  if (s.g.kind != KIND || it > s.ndim) return;
+  const DevGeom g = s.g;
+  const int m = e.nch;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const bool act = k < m;
+  Proposal P{};
+  if (act) P = mh_proposal(e, s, st[e.slot].MH_std, ch, it, k);
+  const int c0 = blockIdx.y * e.cps;
+  const int c1 = min(nchunks, c0 + e.cps);
+  double* part = a.part + e.part_off;
+  const size_t S = (size_t)e.nchunks_ub * m;
+  if ((int)(blockIdx.x * kBlock) < m && c0 < nchunks)
+    field_chunks64<KIND>(g, act, P.x, P.y, P.z, a.X + e.base, a.Y + e.base, a.Z + e.base,
+                         a.Q + e.base, n, c0, c1, sx, sy, sz, sq, part, S, m, k);
+  int* grp = a.cnt + (size_t)blockIdx.z * gridDim.x + blockIdx.x;
+  if (!last_arrival(grp, gridDim.y, &flag)) return;
+  if (threadIdx.x == 0) atomicExch(grp, 0);
+  if (act) {
+    const int p = e.pbase + k;
+    mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
+    field_finish64(g, part, S, m, k, nchunks, P.x, P.y, P.z, a.fx[p], a.fy[p], a.fz[p]);
+  }
+  int* sim = a.cnt + (size_t)gridDim.z * gridDim.x + blockIdx.z;
+  if (!last_arrival(sim, gridDim.x, &flag)) return;
+  if (threadIdx.x == 0) atomicExch(sim, 0);
+  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, a.fx, a.fy, a.fz, ra, rr);
+}
+
+// Mixed (planar): 64 chains per workgroup, one per lane; the four
+// wavefronts take a quarter of every 256-source chunk of the segment each
+// (sources staged in LDS, read as wavefront-wide broadcasts), and their FP64
+// lane sums are added in wavefront order.  The finisher's wavefronts take
+// every fourth segment, again added in wavefront order.
+constexpr int kJumpTargets = 64;
+
+__device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, const float4 h2,
+                                           const float4 l2, int ic, int nic, float two_d,
+                                           double& fx, double& fy, double& fz) {
+  float dx = (h1.x - h2.x) + (l1.x - l2.x);
+  float dy = (h1.y - h2.y) + (l1.y - l2.y);
+  float dz = (h1.z - h2.z) + (l1.z - l2.z);
+  float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+  float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
+  float ri = rsq(r2);
+  float r3 = ri * ri * ri;
+  float cxy = r3, cz = dz * r3;
+  if (ic) {
+    float wxy, wz;
+    ic_planar32(h1.z + h2.z, l1.w + l2.w, dz, dxy2, nic, two_d, wxy, wz);
+    cxy += wxy;
+    cz += wz;
+  }
+  cxy *= h2.w;
+  cz *= h2.w;
+  fx += (double)(cxy * dx);
+  fy += (double)(cxy * dy);
+  fz += (double)cz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mh_jump32(const Ent* tab, const SimDesc* desc,
+                                                      SimState* st, const double* wf, Chains ch,
+                                                      int it, JumpArgs a) {
+  __shared__ float4 shi[kChunk], slo[kChunk];
+  __shared__ double red[3][4][kJumpTargets];
+  __shared__ int ra[256], rr[256];
+  __shared__ int flag;
+  const Ent e = tab[blockIdx.z];
+  const SimDesc& s = desc[e.slot];
+  if (s.g.kind != RD_GEOM_PLANAR || it > s.ndim) return;
+  const DevGeom& g = s.g;
+  const int m = e.nch;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int k = blockIdx.x * kJumpTargets + lane;
+  const bool act = k < m;
+  const bool grp_live = (int)(blockIdx.x * kJumpTargets) < m;  // uniform
+  Proposal P{};
+  if (act) P = mh_proposal(e, s, st[e.slot].MH_std, ch, it, k);
+  const int c0 = blockIdx.y * e.cps;
+  const int c1 = min(nchunks, c0 + e.cps);
+  double* part = a.part + e.part_off;
+  const size_t S = (size_t)e.segs * m;
+  if (grp_live && (int)blockIdx.y < e.segs) {
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
+    if (act) pack1(P.x, P.y, P.z, 0.0, g.d, h1, l1);
+    const float4* hi = a.HI + e.base;
+    const float4* lo = a.LO + e.base;
+    const int ic = g.ic, nic = g.nic;
+    const float two_d = (float)(2.0 * g.d * 1e9);
+    for (int c = c0; c < c1; ++c) {
+      const int jb = c * kChunk;
+      __syncthreads();
+      if (jb + t < n) {
+        shi[t] = hi[jb + t];
+        slo[t] = lo[jb + t];
+      }
+      __syncthreads();
+      const int j0 = w * 64, j1 = min(min(kChunk, n - jb), j0 + 64);
+      if (act)
+        for (int jj = j0; jj < j1; ++jj) mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, fx, fy, fz);
+    }
+    red[0][w][lane] = fx;
+    red[1][w][lane] = fy;
+    red[2][w][lane] = fz;
+    __syncthreads();
+    if (w == 0 && act) {
+      for (int q = 0; q < 3; ++q)
+        part[q * S + (size_t)blockIdx.y * m + k] = ((red[q][0][lane] + red[q][1][lane]) +
+                                                    red[q][2][lane]) + red[q][3][lane];
+    }
+  }
+  int* grp = a.cnt + (size_t)blockIdx.z * gridDim.x + blockIdx.x;
+  if (!last_arrival(grp, gridDim.y, &flag)) return;
+  if (t == 0) atomicExch(grp, 0);
+  if (grp_live) {
+    const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (act)
+      for (int q = w; q < segs; q += 4) {
+        sx += part[0 * S + (size_t)q * m + k];
+        sy += part[1 * S + (size_t)q * m + k];
+        sz += part[2 * S + (size_t)q * m + k];
+      }
+    __syncthreads();
+    red[0][w][lane] = sx;
+    red[1][w][lane] = sy;
+    red[2][w][lane] = sz;
+    __syncthreads();
+    if (w == 0 && act) {
+      double f3[3];
+      for (int q = 0; q < 3; ++q)
+        f3[q] = ((red[q][0][lane] + red[q][1][lane]) + red[q][2][lane]) + red[q][3][lane];
+      const int p = e.pbase + k;
+      const double pre = g.dfc * Consts::q_0 * 1.0e18;
+      a.fx[p] = pre * f3[0];
+      a.fy[p] = pre * f3[1];
+      a.fz[p] = g.Ez + pre * f3[2];
+      mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
+    }
+  }
+  int* sim = a.cnt + (size_t)gridDim.z * gridDim.x + blockIdx.z;
+  if (!last_arrival(sim, gridDim.x, &flag)) return;
+  if (t == 0) atomicExch(sim, 0);
+  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, a.fx, a.fy, a.fz, ra, rr);
+}
+
+__global__ void k_zero_i(int n, int* __restrict__ p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // Per-chain width updates (thermal-field, tip) in chain order and the escape
@@ -1218,6 +1452,89 @@ static int reset_counters(rd_ctx* c, const std::vector<Ent>& ents) {
   return RD_OK;
 }
 
+// The Metropolis-Hastings jumps 1..maxndim of the chain entries `in` (C
+// chains in all: the stride of the point and field arrays), one fused launch
+// per jump (per geometry for FP64).
+static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision,
+                     const Chains& V, int maxndim) {
+  const bool fp64 = precision == RD_PREC_FP64;
+  const int per = fp64 ? kBlock : kJumpTargets;  // chains per workgroup
+  std::vector<Ent> ents;
+  bool any_tip = false, any_planar = false;
+  for (const Ent& e : in) {
+    if (e.nch <= 0) continue;
+    if (c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
+      any_tip = true;
+    else
+      any_planar = true;
+    ents.push_back(e);
+  }
+  if (ents.empty()) return RD_OK;
+  if (!fp64 && any_tip)
+    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
+  JumpArgs a;
+  double* ptx = (double*)c->pts.p;
+  double* f = (double*)c->fld.p;
+  a.ptx = ptx;
+  a.pty = ptx + C;
+  a.ptz = ptx + 2 * (size_t)C;
+  a.fx = f;
+  a.fy = f + C;
+  a.fz = f + 2 * (size_t)C;
+  a.X = c->x;
+  a.Y = c->y;
+  a.Z = c->z;
+  a.Q = c->q;
+  int r = RD_OK;
+  if (!fp64) {
+    r = launch_pack(c, ents);
+    if (r) return r;
+  }
+  a.HI = (const float4*)c->pack_hi.p;
+  a.LO = (const float4*)c->pack_lo.p;
+  int maxg = 0;
+  for (const Ent& e : ents) {
+    const int gx = (e.nch + per - 1) / per;
+    maxg = std::max(maxg, gx);
+  }
+  long long off = 0;
+  int maxsegs = 1;
+  for (Ent& e : ents) {
+    const int nch = (e.n_ub + kChunk - 1) / kChunk;
+    e.nchunks_ub = nch;
+    segment(nch, kSegsJump, e.segs, e.cps);
+    maxsegs = std::max(maxsegs, e.segs);
+    e.part_off = off;
+    off += 3LL * (fp64 ? std::max(nch, 1) : e.segs) * e.nch;
+  }
+  const int ne = (int)ents.size();
+  const int ncnt = ne * maxg + ne;
+  r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
+  if (!r) r = ensure(c, c->cnt, sizeof(int) * (size_t)ncnt);
+  if (r) return r;
+  Ent* dt = nullptr;
+  r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  k_zero_i<<<(ncnt + 255) / 256, 256, 0, c->stream>>>(ncnt, (int*)c->cnt.p);
+  a.part = (double*)c->part.p;
+  a.cnt = (int*)c->cnt.p;
+  const dim3 grid(maxg, maxsegs, ne);
+  for (int it = 1; it <= maxndim; ++it) {
+    if (fp64) {
+      if (any_planar)
+        k_mh_jump64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                                     c->d_wf, V, it, a);
+      if (any_tip)
+        k_mh_jump64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                                  c->d_wf, V, it, a);
+    } else {
+      k_mh_jump32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_wf, V, it, a);
+    }
+  }
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
 // Emission for the entries whose simulation emits at this step.  On return
 // the device state holds the new particle counts; n_round[k] receives the
 // number of chains of entry k (an upper bound of the electrons added).
@@ -1241,7 +1558,7 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
   if (q.empty()) return RD_OK;
   int r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)mtot);
   if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)mtot);
-  if (!r) r = ensure(c, c->stage, sizeof(double) * (size_t)mtot);
+  if (!r) r = ensure(c, c->stage, sizeof(double) * 2 * (size_t)mtot);
   if (r) return r;
   double* ptx = (double*)c->pts.p;
   Ent* dq;
@@ -1256,7 +1573,8 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
   double* f = (double*)c->fld.p;
   k_quad_reduce<<<q.size(), 256, 0, c->stream>>>(dq, c->d_desc, c->d_state, c->d_wf, ptx,
                                                   ptx + mtot, ptx + 2 * mtot, f, f + mtot,
-                                                  f + 2 * mtot, (double*)c->stage.p);
+                                                  f + 2 * mtot, (double*)c->stage.p,
+                                                  (double*)c->stage.p + mtot);
   RD_HIP(hipGetLastError());
   r = sync_state(c);
   if (r) return r;
@@ -1307,13 +1625,10 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
     for (const Ent& e : ch) pend += c->h_state[e.slot].pending;
     if (pend == 0 || last) break;
   }
-  for (int it = 1; it <= maxndim; ++it) {
-    k_mh_propose<<<cg, 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, V, it, ptx, pty, ptz);
-    r = launch_field(c, ch, C, precision, nullptr, nullptr);
-    if (r) return r;
-    k_mh_accept<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V, it,
-                                                   ptx, pty, ptz, fx, fy, fz);
-  }
+  r = run_jumps(c, ch, C, precision, V, maxndim);
+  if (r) return r;
+  r = upload_tab(c, ch, &dc);
+  if (r) return r;
   k_mh_finish<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V);
   k_emit<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V, RD_ARRAYS);
   RD_HIP(hipGetLastError());

@@ -142,6 +142,7 @@ struct rd_ctx {
 
   // work buffers (grown on demand, never shrunk)
   rd::Buf part, pack_hi, pack_lo, pts, fld, pts_hi, pts_lo, stage, img64, tab, red, chains;
+  rd::Buf cnt;  // arrival counters of the fused jump kernels (zero between launches)
   rd::Ent* h_tab = nullptr;  // pinned staging of launch tables (ring)
   int tab_used = 0;          // ring entries used since the last stream synchronisation
   int tab_last = -1;         // ring index of the last upload
@@ -168,6 +169,13 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& ents, int precision, hipEven
 int launch_field(rd_ctx* c, const std::vector<Ent>& ents, int mtot, int precision, hipEvent_t k0,
                  hipEvent_t k1);
 int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab);
+// Packed mixed-precision copies (c->pack_hi / pack_lo) of every entry's particles.
+int launch_pack(rd_ctx* c, const std::vector<Ent>& ents);
+// Split nchunks source chunks into `segs` (at most max_segs) segments of
+// `cps` chunks each.
+void segment(int nchunks, int max_segs, int& segs, int& cps);
+constexpr int kSegsPair = 32;   // accelerations, field at points
+constexpr int kSegsJump = 128;  // fields of the Metropolis-Hastings chains
 int launch_grid_points(rd_ctx* c, double x0, double y0, double lx, double ly, int nx, int ny,
                        double zz);
 

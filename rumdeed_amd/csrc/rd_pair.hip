@@ -482,13 +482,14 @@ __global__ void k_grid_points(int nx, int ny, double x0, double y0, double lx, d
 // ------------------------------------------------------------------------
 // Launch helpers.
 
-// Segments of the source range: enough workgroups for ~8 per CU overall.
-static void segment(int target_blocks_total, int nchunks, int& segs, int& cps) {
-  const int want = 2048;
-  int s = (want + target_blocks_total - 1) / std::max(target_blocks_total, 1);
-  s = std::max(1, std::min(s, std::max(nchunks, 1)));
-  cps = (std::max(nchunks, 1) + s - 1) / s;
-  segs = (std::max(nchunks, 1) + cps - 1) / cps;
+// Segments of a simulation's source range: at most max_segs segments of
+// whole chunks.  The split depends on the simulation's own size only, so the
+// order of every mixed-precision sum, and with it every result, is the same
+// whether the simulation runs alone or in a pool.
+void segment(int nchunks, int max_segs, int& segs, int& cps) {
+  const int nc = std::max(nchunks, 1);
+  cps = (nc + max_segs - 1) / max_segs;
+  segs = (nc + cps - 1) / cps;
 }
 
 // Launch tables travel through a ring of pinned host slots and device slots.
@@ -531,13 +532,12 @@ int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab) {
 int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_t k0,
                  hipEvent_t k1) {
   std::vector<Ent> ents;
-  int maxn = 0, tbt = 0;
+  int maxn = 0;
   bool any_tip = false, any_planar = false;
   for (const Ent& e : in)
     if (e.n_ub > 0) {
       ents.push_back(e);
       maxn = std::max(maxn, e.n_ub);
-      tbt += (e.n_ub + kBlock - 1) / kBlock;
       if (c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
         any_tip = true;
       else
@@ -553,7 +553,7 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   for (Ent& e : ents) {
     int nch = (e.n_ub + kChunk - 1) / kChunk;
     e.nchunks_ub = nch;
-    segment(tbt, nch, e.segs, e.cps);
+    segment(nch, kSegsPair, e.segs, e.cps);
     maxsegs = std::max(maxsegs, e.segs);
     e.part_off = off;
     off += 3LL * (precision == RD_PREC_FP64 ? nch : e.segs) * e.n_ub;
@@ -608,13 +608,12 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
 int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision, hipEvent_t k0,
                  hipEvent_t k1) {
   std::vector<Ent> ents;
-  int maxm = 0, tbt = 0;
+  int maxm = 0;
   bool any_tip = false, any_planar = false;
   for (const Ent& e : in)
     if (e.m > 0) {
       ents.push_back(e);
       maxm = std::max(maxm, e.m);
-      tbt += (e.m + kBlock - 1) / kBlock;
       if (c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
         any_tip = true;
       else
@@ -630,7 +629,7 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   for (Ent& e : ents) {
     int nch = (e.n_ub + kChunk - 1) / kChunk;
     e.nchunks_ub = nch;
-    segment(tbt, nch, e.segs, e.cps);
+    segment(nch, kSegsPair, e.segs, e.cps);
     maxsegs = std::max(maxsegs, e.segs);
     maxn = std::max(maxn, e.n_ub);
     e.part_off = off;
@@ -684,6 +683,23 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
       (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
   k_field_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, fx, fy, fz);
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
+int launch_pack(rd_ctx* c, const std::vector<Ent>& ents) {
+  int maxn = 0;
+  for (const Ent& e : ents) maxn = std::max(maxn, e.n_ub);
+  if (ents.empty() || maxn == 0) return RD_OK;
+  int r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)c->cap);
+  if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)c->cap);
+  if (r) return r;
+  Ent* dt = nullptr;
+  r = upload_tab(c, ents, &dt);
+  if (r) return r;
+  k_pack_particles<<<dim3((maxn + 255) / 256, 1, ents.size()), 256, 0, c->stream>>>(
+      dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (float4*)c->pack_hi.p,
+      (float4*)c->pack_lo.p);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }
