@@ -170,7 +170,7 @@ __global__ void k_quad_points(const Ent* tab, const SimDesc* desc, double* __res
 }
 
 // Integrand per point, then the point-ordered sum and the number of chains.
-__global__ __launch_bounds__(256) void k_quad_reduce(const Ent* tab, const SimDesc* desc,
+__global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimDesc* desc,
                                                      SimState* st, const double* wf,
                                                      const double* __restrict__ ptx,
                                                      const double* __restrict__ pty,
@@ -238,20 +238,86 @@ __global__ __launch_bounds__(256) void k_quad_reduce(const Ent* tab, const SimDe
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int k = 0; k < cnt; ++k) {
-        sum = sum + t0[k];
-        if (tip) {
-          f2 = f2 + t3[k];
-        } else {
-          f0 = f0 + t1[k];
-          f1 = f1 + t2[k];
-          f2 = f2 + t3[k];
-        }
+      // loads of eight points issued together, additions in point order
+      for (int k0 = 0; k0 < cnt; k0 += 8) {
+        double a0[8], a1[8], a2[8], a3[8];
+        const int nk = min(8, cnt - k0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (j < nk) {
+            a0[j] = t0[k0 + j];
+            a3[j] = t3[k0 + j];
+            if (!tip) {
+              a1[j] = t1[k0 + j];
+              a2[j] = t2[k0 + j];
+            }
+          }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (j < nk) {
+            sum = sum + a0[j];
+            if (tip) {
+              f2 = f2 + a3[j];
+            } else {
+              f0 = f0 + a1[j];
+              f1 = f1 + a2[j];
+              f2 = f2 + a3[j];
+            }
+          }
       }
     }
   }
-  if (threadIdx.x != 0) return;
   SimState& S = st[e.slot];
+  __shared__ double s_lambda;
+  if (threadIdx.x == 0) s_lambda = sum / ((double)s.quad * (double)s.quad);
+  if (s.mode == RD_EMIT_PLANAR_GTF) {
+    // Rand_Poisson: the product of uniforms runs in one lane; the uniforms,
+    // in stream order, are drawn kPois at a time by the whole block.
+    constexpr int kPois = 4096;
+    __shared__ double ut[kPois];
+    __shared__ int s_more;
+    __syncthreads();
+    const double lambda = s_lambda;
+    double lambda_left = lambda, pp = 1.0;
+    int kk = 0;
+    for (uint64_t b0 = 0;; b0 += kPois) {
+      for (int j = threadIdx.x; j < kPois / 4; j += blockDim.x) {
+        const u64x4 r =
+            philox4x64((uint64_t)e.step, tag(kTagPoisson, 0), b0 / 4 + j, 0, s.seed, s.stream);
+        for (int q = 0; q < 4; ++q) ut[4 * j + q] = u01(r.v[q]);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        bool more = true;
+        for (int q = 0; q < kPois && more; ++q) {
+          kk = kk + 1;
+          pp = pp * ut[q];
+          while ((pp < 1.0) && (lambda_left > 0.0)) {
+            if (lambda_left > 500.0) {
+              pp = pp * rd_exp(500.0);
+              lambda_left = lambda_left - 500.0;
+            } else {
+              pp = pp * rd_exp(lambda_left);
+              lambda_left = 0.0;
+            }
+          }
+          more = pp >= 1.0 && kk < (1 << 30);
+        }
+        s_more = more;
+      }
+      __syncthreads();
+      if (!s_more) break;
+    }
+    if (threadIdx.x == 0) {
+      S.N_sup = lambda;
+      S.n_round = kk - 1;
+      S.F_avg[0] = f0 / e.m;
+      S.F_avg[1] = f1 / e.m;
+      S.F_avg[2] = f2 / e.m;
+    }
+    return;
+  }
+  if (threadIdx.x != 0) return;
   int nr;
   if (s.mode == RD_EMIT_TIP_FE) {
     S.N_sup = sum;
@@ -269,7 +335,7 @@ __global__ __launch_bounds__(256) void k_quad_reduce(const Ent* tab, const SimDe
       S.residual = nsup - nr;
       if (nr < 0) nr = 0;
     } else {
-      nr = poisson(s.seed, s.stream, (uint64_t)e.step, nsup);
+      nr = 0;  // thermal-field: drawn above
     }
     S.F_avg[0] = f0 / e.m;
     S.F_avg[1] = f1 / e.m;
@@ -400,7 +466,8 @@ __global__ __launch_bounds__(256) void k_mh_start_check(const Ent* tab, const Si
 }
 
 // Gaussian proposal of jump `it` (1-based) for chain k of entry e, with the
-// width MH of the start of the jump.  (a, b) are the chain coordinates
+// width MH of the start of the jump and the unit normal pair (u1, u2) of the
+// chain's (step, jump) stream (box_muller_unit).  (a, b) are the chain coordinates
 // (planar x, y; tip xi, phi) and (x, y, z) the surface point whose field the
 // accept step needs.  valid: 1 proposal inside the patch, 0 tip proposal
 // outside it, -1 chain without a start site (the point is its final site and
@@ -411,7 +478,8 @@ struct Proposal {
 };
 
 __device__ __forceinline__ Proposal mh_proposal(const Ent& e, const SimDesc& s, double MH,
-                                                const Chains& ch, int it, int k) {
+                                                const Chains& ch, int it, int k, double u1,
+                                                double u2) {
   Proposal P;
   const int c = e.cbase + k;
   if (ch.ok[c] != 1) {
@@ -422,7 +490,6 @@ __device__ __forceinline__ Proposal mh_proposal(const Ent& e, const SimDesc& s, 
     P.valid = -1;
     return P;
   }
-  const uint64_t tg = tag(kTagJump, (uint64_t)it);
   const double PI2 = 2.0 * 3.141592653589793238462643383279502884197;
   if (s.mode == RD_EMIT_TIP_FE) {
     const int first = nint(s.ndim * 0.25);
@@ -434,8 +501,7 @@ __device__ __forceinline__ Proposal mh_proposal(const Ent& e, const SimDesc& s, 
       s1 = (s.g.max_xi - 1.0) * 0.10;
       s2 = PI2 * 0.10;
     }
-    double d1, d2;
-    box_muller(s.seed, s.stream, (uint64_t)e.step, tg, (uint64_t)k, 0.0, 0.0, s1, s2, d1, d2);
+    const double d1 = u1 * s1 + 0.0, d2 = u2 * s2 + 0.0;
     double nxi = ch.cx[c] + d1;
     double nphi = fmodulo(ch.cy[c] + d2, PI2);
     if (nxi > s.g.max_xi) nxi = 2.0 * s.g.max_xi - nxi;
@@ -462,9 +528,7 @@ __device__ __forceinline__ Proposal mh_proposal(const Ent& e, const SimDesc& s, 
     sx = s.elx * MH;
     sy = s.ely * MH;
   }
-  double nx, ny;
-  box_muller(s.seed, s.stream, (uint64_t)e.step, tg, (uint64_t)k, ch.cx[c], ch.cy[c], sx, sy, nx,
-             ny);
+  double nx = u1 * sx + ch.cx[c], ny = u2 * sy + ch.cy[c];
   if (s.mode == RD_EMIT_PLANAR_FE) {
     // check_limits_metro_rec (src/mod_field_emission_v2.F90:1466-1516): reflect
     double x_max = s.ex0 + s.elx, x_min = s.ex0;
@@ -511,11 +575,12 @@ __device__ __forceinline__ void mh_record(const Ent& e, const SimDesc& s, const 
 }
 
 // Accept/reject of jump `it` for every chain of entry e, by one block of 256
-// lanes (ra, rr: 256-entry LDS arrays for the batch acceptance counts).
+// lanes (ra, rr: 256-entry LDS arrays for the batch acceptance counts; lr:
+// the log-uniforms of the jump, by chain).
 __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, const double* wf,
                                 const Chains& ch, int it, const double* ptx, const double* pty,
                                 const double* ptz, const double* fx, const double* fy,
-                                const double* fz, int* ra, int* rr) {
+                                const double* fz, const double* lr, int* ra, int* rr) {
   const EmitConsts ec = emit_consts();
   const int first = nint(s.ndim * 0.25);
   const bool tip = s.mode == RD_EMIT_TIP_FE;
@@ -541,9 +606,7 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
         if (sup_new >= ch.sup[c]) {
           acc = true;
         } else {
-          double rnd = uniform(s.seed, s.stream, (uint64_t)e.step, tag(kTagAccept, (uint64_t)it),
-                               (uint64_t)k, 0);
-          acc = rd_log(rnd) <= alpha;
+          acc = lr[c] <= alpha;
         }
         if (acc) {
           ch.cx[c] = ch.px[c];
@@ -597,85 +660,87 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
 }
 
 // ------------------------------------------------------------------------
-// One Metropolis-Hastings jump per launch.
+// One Metropolis-Hastings jump in two launches.
 //
-// Every workgroup proposes the jump for its own targets (the proposals are
-// counter-based random numbers, so all workgroups of a target agree), adds
-// the field of one source segment and writes its partial sums.  The last
-// workgroup of a target group to arrive (an atomic counter per group) adds
-// the partials, records the proposals and their fields; the last target
-// group of a simulation to finish (a counter per simulation) then runs the
-// accept step for all of that simulation's chains.  No workgroup waits for
-// another: the ones that are not last simply end, so the launch always
-// drains.  Each counter is reset by the workgroup that saw it complete, so
-// all counters are zero between launches.
+// k_mh_field64 / k_mh_field32: every workgroup proposes the jump for its own
+// chains (the proposals are counter-based, so every workgroup of a chain
+// computes the same one; the first source segment's workgroup records it)
+// and writes the field partial sums of one source segment.  k_mh_settle: one
+// workgroup per simulation adds the partials in a fixed order and runs the
+// accept step for all of the simulation's chains.  The kernel boundary is
+// the only synchronisation: no counters, no device-wide fences.
 struct JumpArgs {
   double *ptx, *pty, *ptz, *fx, *fy, *fz;
   double* part;
-  const double *X, *Y, *Z, *Q;  // FP64 sources
-  const float4 *HI, *LO;        // packed sources (mixed)
-  int* cnt;                     // [ne][gridDim.x] group counters, then [ne] simulation counters
+  const double *X, *Y, *Z, *Q;    // FP64 sources
+  const float4 *HI, *LO;          // packed sources (mixed)
+  const double *ru1, *ru2, *rlr;  // per jump and chain: unit normals, log-uniform ([it-1][C])
+  int C;                          // chains in all
 };
 
-// True in every lane of the block that arrives last of `total` at *counter.
-// Release before the arrival, acquire after it (agent scope: the partials
-// written on other XCDs become visible).
-__device__ __forceinline__ bool last_arrival(int* counter, int total, int* flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) *flag = (atomicAdd(counter, 1) == total - 1);
-  __syncthreads();
-  const bool last = *flag != 0;
-  if (last) __threadfence();
-  return last;
+// The random numbers of every jump of every chain, drawn before the jumps so
+// that none of them is on a jump's critical path: the unit normal pair of the
+// proposal and the log of the accept uniform, from the chain's (step, jump)
+// streams.
+__global__ void k_mh_random(const Ent* tab, const SimDesc* desc, int C, double* __restrict__ u1,
+                            double* __restrict__ u2, double* __restrict__ lr) {
+  const Ent e = tab[blockIdx.z];
+  const SimDesc& s = desc[e.slot];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int it = blockIdx.y + 1;
+  if (k >= e.nch || it > s.ndim) return;
+  const size_t o = (size_t)(it - 1) * C + e.cbase + k;
+  double a, b;
+  box_muller_unit(s.seed, s.stream, (uint64_t)e.step, tag(kTagJump, (uint64_t)it), (uint64_t)k, a,
+                  b);
+  u1[o] = a;
+  u2[o] = b;
+  lr[o] = rd_log(
+      uniform(s.seed, s.stream, (uint64_t)e.step, tag(kTagAccept, (uint64_t)it), (uint64_t)k, 0));
 }
 
-// FP64: one lane per chain, sources in 256-source chunks (the canonical
-// order of the field at points, bit-identical to the non-fused path).
+__device__ __forceinline__ Proposal jump_proposal(const Ent& e, const SimDesc& s,
+                                                  const SimState* st, const Chains& ch, int it,
+                                                  int k, const JumpArgs& a) {
+  const size_t o = (size_t)(it - 1) * a.C + e.cbase + k;
+  return mh_proposal(e, s, st[e.slot].MH_std, ch, it, k, a.ru1[o], a.ru2[o]);
+}
+
+// FP64: one lane per chain, 256 chains per workgroup, sources in 256-source
+// chunks (the canonical order of the field at points: partials per chunk,
+// bit-identical to the oracle).
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_mh_jump64(const Ent* tab, const SimDesc* desc,
-                                                      SimState* st, const double* wf, Chains ch,
-                                                      int it, JumpArgs a) {
+__global__ __launch_bounds__(kBlock) void k_mh_field64(const Ent* tab, const SimDesc* desc,
+                                                       const SimState* st, Chains ch, int it,
+                                                       JumpArgs a) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
-  __shared__ int ra[256], rr[256];
-  __shared__ int flag;
   const Ent e = tab[blockIdx.z];
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != KIND || it > s.ndim) return;
-  const DevGeom g = s.g;
   const int m = e.nch;
+  if ((int)(blockIdx.x * kBlock) >= m || (int)blockIdx.y >= e.segs) return;
+  const DevGeom g = s.g;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const int k = blockIdx.x * kBlock + threadIdx.x;
   const bool act = k < m;
   Proposal P{};
-  if (act) P = mh_proposal(e, s, st[e.slot].MH_std, ch, it, k);
+  if (act) {
+    P = jump_proposal(e, s, st, ch, it, k, a);
+    if (blockIdx.y == 0) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
+  }
   const int c0 = blockIdx.y * e.cps;
   const int c1 = min(nchunks, c0 + e.cps);
-  double* part = a.part + e.part_off;
-  const size_t S = (size_t)e.nchunks_ub * m;
-  if ((int)(blockIdx.x * kBlock) < m && c0 < nchunks)
+  if (c0 < nchunks)
     field_chunks64<KIND>(g, act, P.x, P.y, P.z, a.X + e.base, a.Y + e.base, a.Z + e.base,
-                         a.Q + e.base, n, c0, c1, sx, sy, sz, sq, part, S, m, k);
-  int* grp = a.cnt + (size_t)blockIdx.z * gridDim.x + blockIdx.x;
-  if (!last_arrival(grp, gridDim.y, &flag)) return;
-  if (threadIdx.x == 0) atomicExch(grp, 0);
-  if (act) {
-    const int p = e.pbase + k;
-    mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
-    field_finish64(g, part, S, m, k, nchunks, P.x, P.y, P.z, a.fx[p], a.fy[p], a.fz[p]);
-  }
-  int* sim = a.cnt + (size_t)gridDim.z * gridDim.x + blockIdx.z;
-  if (!last_arrival(sim, gridDim.x, &flag)) return;
-  if (threadIdx.x == 0) atomicExch(sim, 0);
-  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, a.fx, a.fy, a.fz, ra, rr);
+                         a.Q + e.base, n, c0, c1, sx, sy, sz, sq, a.part + e.part_off,
+                         (size_t)e.nchunks_ub * m, m, k);
 }
 
 // Mixed (planar): 64 chains per workgroup, one per lane; the four
 // wavefronts take a quarter of every 256-source chunk of the segment each
 // (sources staged in LDS, read as wavefront-wide broadcasts), and their FP64
-// lane sums are added in wavefront order.  The finisher's wavefronts take
-// every fourth segment, again added in wavefront order.
+// lane sums are added in wavefront order.
 constexpr int kJumpTargets = 64;
 
 __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, const float4 h2,
@@ -702,122 +767,136 @@ __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, con
   fz += (double)cz;
 }
 
-__global__ __launch_bounds__(kBlock) void k_mh_jump32(const Ent* tab, const SimDesc* desc,
-                                                      SimState* st, const double* wf, Chains ch,
-                                                      int it, JumpArgs a) {
+__global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const SimDesc* desc,
+                                                       const SimState* st, Chains ch, int it,
+                                                       JumpArgs a) {
   __shared__ float4 shi[kChunk], slo[kChunk];
   __shared__ double red[3][4][kJumpTargets];
-  __shared__ int ra[256], rr[256];
-  __shared__ int flag;
   const Ent e = tab[blockIdx.z];
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != RD_GEOM_PLANAR || it > s.ndim) return;
-  const DevGeom& g = s.g;
   const int m = e.nch;
+  if ((int)(blockIdx.x * kJumpTargets) >= m || (int)blockIdx.y >= e.segs) return;
+  const DevGeom& g = s.g;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int k = blockIdx.x * kJumpTargets + lane;
   const bool act = k < m;
-  const bool grp_live = (int)(blockIdx.x * kJumpTargets) < m;  // uniform
   Proposal P{};
-  if (act) P = mh_proposal(e, s, st[e.slot].MH_std, ch, it, k);
+  if (act) {
+    P = jump_proposal(e, s, st, ch, it, k, a);
+    if (blockIdx.y == 0 && w == 0) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
+  }
   const int c0 = blockIdx.y * e.cps;
   const int c1 = min(nchunks, c0 + e.cps);
-  double* part = a.part + e.part_off;
-  const size_t S = (size_t)e.segs * m;
-  if (grp_live && (int)blockIdx.y < e.segs) {
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
-    if (act) pack1(P.x, P.y, P.z, 0.0, g.d, h1, l1);
-    const float4* hi = a.HI + e.base;
-    const float4* lo = a.LO + e.base;
-    const int ic = g.ic, nic = g.nic;
-    const float two_d = (float)(2.0 * g.d * 1e9);
-    for (int c = c0; c < c1; ++c) {
-      const int jb = c * kChunk;
-      __syncthreads();
-      if (jb + t < n) {
-        shi[t] = hi[jb + t];
-        slo[t] = lo[jb + t];
-      }
-      __syncthreads();
-      const int j0 = w * 64, j1 = min(min(kChunk, n - jb), j0 + 64);
-      if (act)
-        for (int jj = j0; jj < j1; ++jj) mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, fx, fy, fz);
-    }
-    red[0][w][lane] = fx;
-    red[1][w][lane] = fy;
-    red[2][w][lane] = fz;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
+  if (act) pack1(P.x, P.y, P.z, 0.0, g.d, h1, l1);
+  const float4* hi = a.HI + e.base;
+  const float4* lo = a.LO + e.base;
+  const int ic = g.ic, nic = g.nic;
+  const float two_d = (float)(2.0 * g.d * 1e9);
+  for (int c = c0; c < c1; ++c) {
+    const int jb = c * kChunk;
     __syncthreads();
-    if (w == 0 && act) {
-      for (int q = 0; q < 3; ++q)
-        part[q * S + (size_t)blockIdx.y * m + k] = ((red[q][0][lane] + red[q][1][lane]) +
-                                                    red[q][2][lane]) + red[q][3][lane];
+    if (jb + t < n) {
+      shi[t] = hi[jb + t];
+      slo[t] = lo[jb + t];
     }
-  }
-  int* grp = a.cnt + (size_t)blockIdx.z * gridDim.x + blockIdx.x;
-  if (!last_arrival(grp, gridDim.y, &flag)) return;
-  if (t == 0) atomicExch(grp, 0);
-  if (grp_live) {
-    const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    __syncthreads();
+    const int j0 = w * 64, j1 = min(min(kChunk, n - jb), j0 + 64);
     if (act)
-      for (int q = w; q < segs; q += 4) {
-        sx += part[0 * S + (size_t)q * m + k];
-        sy += part[1 * S + (size_t)q * m + k];
-        sz += part[2 * S + (size_t)q * m + k];
-      }
-    __syncthreads();
-    red[0][w][lane] = sx;
-    red[1][w][lane] = sy;
-    red[2][w][lane] = sz;
-    __syncthreads();
-    if (w == 0 && act) {
-      double f3[3];
-      for (int q = 0; q < 3; ++q)
-        f3[q] = ((red[q][0][lane] + red[q][1][lane]) + red[q][2][lane]) + red[q][3][lane];
-      const int p = e.pbase + k;
-      const double pre = g.dfc * Consts::q_0 * 1.0e18;
-      a.fx[p] = pre * f3[0];
-      a.fy[p] = pre * f3[1];
-      a.fz[p] = g.Ez + pre * f3[2];
-      mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
-    }
+      for (int jj = j0; jj < j1; ++jj) mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, fx, fy, fz);
   }
-  int* sim = a.cnt + (size_t)gridDim.z * gridDim.x + blockIdx.z;
-  if (!last_arrival(sim, gridDim.x, &flag)) return;
-  if (t == 0) atomicExch(sim, 0);
-  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, a.fx, a.fy, a.fz, ra, rr);
+  red[0][w][lane] = fx;
+  red[1][w][lane] = fy;
+  red[2][w][lane] = fz;
+  __syncthreads();
+  if (w == 0 && act) {
+    double* part = a.part + e.part_off;
+    const size_t S = (size_t)e.segs * m;
+    for (int q = 0; q < 3; ++q)
+      part[q * S + (size_t)blockIdx.y * m + k] =
+          ((red[q][0][lane] + red[q][1][lane]) + red[q][2][lane]) + red[q][3][lane];
+  }
 }
 
-__global__ void k_zero_i(int n, int* __restrict__ p) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// Finish and accept: one workgroup per simulation.  FP64 adds the chunk
+// partials of every chain in chunk order (plus the vacuum field); mixed adds
+// the segment partials in segment order.
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_mh_settle(const Ent* tab, const SimDesc* desc,
+                                                      SimState* st, const double* wf, Chains ch,
+                                                      int it, JumpArgs a) {
+  __shared__ int ra[256], rr[256];
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  if (it > s.ndim) return;
+  const int m = e.nch;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const double* part = a.part + e.part_off;
+  for (int k = threadIdx.x; k < m; k += blockDim.x) {
+    const int p = e.pbase + k;
+    if (PREC == RD_PREC_FP64) {
+      field_finish64(s.g, part, (size_t)e.nchunks_ub * m, m, k, nchunks, a.ptx[p], a.pty[p],
+                     a.ptz[p], a.fx[p], a.fy[p], a.fz[p]);
+    } else {
+      const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
+      field_finish32(s.g, part, (size_t)e.segs * m, m, k, segs, a.fx[p], a.fy[p], a.fz[p]);
+    }
+  }
+  __syncthreads();
+  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, a.fx, a.fy, a.fz,
+                  a.rlr + (size_t)(it - 1) * a.C, ra, rr);
 }
 
 // Per-chain width updates (thermal-field, tip) in chain order and the escape
 // quantity of every chain.
 __global__ __launch_bounds__(256) void k_mh_finish(const Ent* tab, const SimDesc* desc,
                                                    SimState* st, const double* wf, Chains ch) {
+  constexpr int kTile = 1024;
+  __shared__ double fac[kTile], rate[kTile];
   const Ent e = tab[blockIdx.x];
   const SimDesc& s = desc[e.slot];
   const EmitConsts ec = emit_consts();
-  if (threadIdx.x == 0 && s.mode != RD_EMIT_PLANAR_FE) {
+  if (s.mode != RD_EMIT_PLANAR_FE) {
+    // The width factor of every chain depends on its own counts only, so the
+    // exponentials are formed in parallel; the products and clamps, which
+    // chain the chains together, follow in chain order in one lane.
     SimState& S = st[e.slot];
     const double lo = (s.mode == RD_EMIT_TIP_FE) ? 0.0005 : 0.005;
-    for (int k = 0; k < e.nch; ++k) {
-      const int c = e.cbase + k;
-      if (ch.ok[c] != 1) continue;
-      int a = ch.acc[c], r = ch.rej[c];
-      if (a + r > 0) {
-        S.a_rate = (double)a / (double)(r + a);
-        S.MH_std = S.MH_std * rd_exp(0.025 * (S.a_rate - 0.35));
-        if (S.MH_std > 0.1250)
-          S.MH_std = 0.1250;
-        else if (S.MH_std < lo)
-          S.MH_std = lo;
+    double mh = S.MH_std, ar = S.a_rate;
+    for (int b = 0; b < e.nch; b += kTile) {
+      const int cnt = min(kTile, e.nch - b);
+      __syncthreads();
+      for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const int c = e.cbase + b + k;
+        const int a = ch.acc[c], r = ch.rej[c];
+        if (ch.ok[c] == 1 && a + r > 0) {
+          const double ak = (double)a / (double)(r + a);
+          rate[k] = ak;
+          fac[k] = rd_exp(0.025 * (ak - 0.35));
+        } else {
+          rate[k] = -1.0;  // no update
+        }
       }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int k = 0; k < cnt; ++k) {
+          if (rate[k] < 0.0) continue;
+          ar = rate[k];
+          mh = mh * fac[k];
+          if (mh > 0.1250)
+            mh = 0.1250;
+          else if (mh < lo)
+            mh = lo;
+        }
+    }
+    if (threadIdx.x == 0) {
+      S.MH_std = mh;
+      S.a_rate = ar;
     }
   }
   for (int k = threadIdx.x; k < e.nch; k += blockDim.x) {
@@ -1502,33 +1581,47 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   for (Ent& e : ents) {
     const int nch = (e.n_ub + kChunk - 1) / kChunk;
     e.nchunks_ub = nch;
-    segment(nch, kSegsJump, e.segs, e.cps);
+    segment(nch, seg_limit((e.nch + per - 1) / per, kWantJump), e.segs, e.cps);
     maxsegs = std::max(maxsegs, e.segs);
     e.part_off = off;
     off += 3LL * (fp64 ? std::max(nch, 1) : e.segs) * e.nch;
   }
   const int ne = (int)ents.size();
-  const int ncnt = ne * maxg + ne;
   r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
-  if (!r) r = ensure(c, c->cnt, sizeof(int) * (size_t)ncnt);
+  if (r) return r;
+  const size_t nr = (size_t)maxndim * C;
+  r = ensure(c, c->rnd, sizeof(double) * 3 * nr);
   if (r) return r;
   Ent* dt = nullptr;
   r = upload_tab(c, ents, &dt);
   if (r) return r;
-  k_zero_i<<<(ncnt + 255) / 256, 256, 0, c->stream>>>(ncnt, (int*)c->cnt.p);
+  {
+    int maxc = 0;
+    for (const Ent& e : ents) maxc = std::max(maxc, e.nch);
+    double* rb = (double*)c->rnd.p;
+    k_mh_random<<<dim3((maxc + 255) / 256, maxndim, ne), 256, 0, c->stream>>>(
+        dt, c->d_desc, C, rb, rb + nr, rb + 2 * nr);
+    a.ru1 = rb;
+    a.ru2 = rb + nr;
+    a.rlr = rb + 2 * nr;
+    a.C = C;
+  }
   a.part = (double*)c->part.p;
-  a.cnt = (int*)c->cnt.p;
   const dim3 grid(maxg, maxsegs, ne);
   for (int it = 1; it <= maxndim; ++it) {
     if (fp64) {
       if (any_planar)
-        k_mh_jump64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
-                                                                     c->d_wf, V, it, a);
+        k_mh_field64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                                      V, it, a);
       if (any_tip)
-        k_mh_jump64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
-                                                                  c->d_wf, V, it, a);
+        k_mh_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, V,
+                                                                   it, a);
+      k_mh_settle<RD_PREC_FP64><<<ne, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_wf,
+                                                               V, it, a);
     } else {
-      k_mh_jump32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_wf, V, it, a);
+      k_mh_field32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, V, it, a);
+      k_mh_settle<RD_PREC_MIXED><<<ne, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                                c->d_wf, V, it, a);
     }
   }
   RD_HIP(hipGetLastError());
@@ -1571,7 +1664,7 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
   r = launch_field(c, q, mtot, precision, nullptr, nullptr);
   if (r) return r;
   double* f = (double*)c->fld.p;
-  k_quad_reduce<<<q.size(), 256, 0, c->stream>>>(dq, c->d_desc, c->d_state, c->d_wf, ptx,
+  k_quad_reduce<<<q.size(), 1024, 0, c->stream>>>(dq, c->d_desc, c->d_state, c->d_wf, ptx,
                                                   ptx + mtot, ptx + 2 * mtot, f, f + mtot,
                                                   f + 2 * mtot, (double*)c->stage.p,
                                                   (double*)c->stage.p + mtot);

@@ -142,7 +142,7 @@ struct rd_ctx {
 
   // work buffers (grown on demand, never shrunk)
   rd::Buf part, pack_hi, pack_lo, pts, fld, pts_hi, pts_lo, stage, img64, tab, red, chains;
-  rd::Buf cnt;  // arrival counters of the fused jump kernels (zero between launches)
+  rd::Buf rnd;  // random numbers of the Metropolis-Hastings jumps of a step
   rd::Ent* h_tab = nullptr;  // pinned staging of launch tables (ring)
   int tab_used = 0;          // ring entries used since the last stream synchronisation
   int tab_last = -1;         // ring index of the last upload
@@ -174,8 +174,14 @@ int launch_pack(rd_ctx* c, const std::vector<Ent>& ents);
 // Split nchunks source chunks into `segs` (at most max_segs) segments of
 // `cps` chunks each.
 void segment(int nchunks, int max_segs, int& segs, int& cps);
-constexpr int kSegsPair = 32;   // accelerations, field at points
-constexpr int kSegsJump = 128;  // fields of the Metropolis-Hastings chains
+// Segments for a simulation whose targets fill `target_blocks` workgroups:
+// about `want` workgroups for that simulation.
+inline int seg_limit(int target_blocks, int want) {
+  const int tb = target_blocks > 0 ? target_blocks : 1;
+  return (want + tb - 1) / tb;
+}
+constexpr int kWantPair = 8192;  // accelerations, field at points
+constexpr int kWantJump = 512;   // fields of the Metropolis-Hastings chains
 int launch_grid_points(rd_ctx* c, double x0, double y0, double lx, double ly, int nx, int ny,
                        double zz);
 

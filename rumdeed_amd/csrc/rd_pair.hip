@@ -553,7 +553,7 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   for (Ent& e : ents) {
     int nch = (e.n_ub + kChunk - 1) / kChunk;
     e.nchunks_ub = nch;
-    segment(nch, kSegsPair, e.segs, e.cps);
+    segment(nch, seg_limit((e.n_ub + kBlock - 1) / kBlock, kWantPair), e.segs, e.cps);
     maxsegs = std::max(maxsegs, e.segs);
     e.part_off = off;
     off += 3LL * (precision == RD_PREC_FP64 ? nch : e.segs) * e.n_ub;
@@ -629,7 +629,7 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   for (Ent& e : ents) {
     int nch = (e.n_ub + kChunk - 1) / kChunk;
     e.nchunks_ub = nch;
-    segment(nch, kSegsPair, e.segs, e.cps);
+    segment(nch, seg_limit((e.m + kBlock - 1) / kBlock, kWantPair), e.segs, e.cps);
     maxsegs = std::max(maxsegs, e.segs);
     maxn = std::max(maxn, e.n_ub);
     e.part_off = off;

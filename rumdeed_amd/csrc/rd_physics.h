@@ -272,8 +272,11 @@ RD_HD uint64_t tag(uint64_t kind, uint64_t sub) { return (kind << 40) | sub; }
 // Polar Box-Muller (box_muller, src/mod_global.F90:578-595) on the stream
 // (seed; step, tg, chain, block = 0, 1, ...): two draws per attempt, two
 // attempts per Philox block, until 0 < w < 1.
-RD_HD void box_muller(uint64_t seed, uint64_t sim, uint64_t step, uint64_t tg, uint64_t chain,
-                      double m1, double m2, double s1, double s2, double& o1, double& o2) {
+// The unit pair (x1 f, x2 f) of the polar method; box_muller scales it as
+// (x1 f) s + m.  Returns false (pair 0, 0) if no attempt of the bounded
+// stream was accepted.
+RD_HD bool box_muller_unit(uint64_t seed, uint64_t sim, uint64_t step, uint64_t tg,
+                           uint64_t chain, double& u1, double& u2) {
   for (uint64_t blk = 0; blk < 4096; ++blk) {
     u64x4 r = philox4x64(step, tg, chain, blk, seed, sim);
     for (int a = 0; a < 2; ++a) {
@@ -282,14 +285,27 @@ RD_HD void box_muller(uint64_t seed, uint64_t sim, uint64_t step, uint64_t tg, u
       double w = x1 * x1 + x2 * x2;
       if ((w < 1.0) && (w > 0.0)) {
         double f = __builtin_sqrt((-2.0 * rd_log(w)) / w);
-        o1 = x1 * f * s1 + m1;
-        o2 = x2 * f * s2 + m2;
-        return;
+        u1 = x1 * f;
+        u2 = x2 * f;
+        return true;
       }
     }
   }
-  o1 = m1;
-  o2 = m2;
+  u1 = 0.0;
+  u2 = 0.0;
+  return false;
+}
+
+RD_HD void box_muller(uint64_t seed, uint64_t sim, uint64_t step, uint64_t tg, uint64_t chain,
+                      double m1, double m2, double s1, double s2, double& o1, double& o2) {
+  double u1, u2;
+  if (box_muller_unit(seed, sim, step, tg, chain, u1, u2)) {
+    o1 = u1 * s1 + m1;
+    o2 = u2 * s2 + m2;
+  } else {
+    o1 = m1;
+    o2 = m2;
+  }
 }
 
 RD_HD double uniform(uint64_t seed, uint64_t sim, uint64_t step, uint64_t tg, uint64_t chain,
@@ -298,32 +314,6 @@ RD_HD double uniform(uint64_t seed, uint64_t sim, uint64_t step, uint64_t tg, ui
   return u01(r.v[which & 3]);
 }
 
-// Rand_Poisson (src/mod_global.F90:600-643): Knuth's product method with
-// exp(500) steps for large means; uniforms from the (step, kTagPoisson) stream.
-RD_HD int poisson(uint64_t seed, uint64_t sim, uint64_t step, double lambda) {
-  double lambda_left = lambda;
-  int k = 0;
-  double p = 1.0;
-  const double STEP = 500.0;
-  uint64_t cnt = 0;
-  do {
-    k = k + 1;
-    u64x4 r = philox4x64(step, tag(kTagPoisson, 0), cnt >> 2, 0, seed, sim);
-    double u = u01(r.v[cnt & 3]);
-    ++cnt;
-    p = p * u;
-    while ((p < 1.0) && (lambda_left > 0.0)) {
-      if (lambda_left > STEP) {
-        p = p * rd_exp(STEP);
-        lambda_left = lambda_left - STEP;
-      } else {
-        p = p * rd_exp(lambda_left);
-        lambda_left = 0.0;
-      }
-    }
-  } while (p >= 1.0 && k < (1 << 30));
-  return k - 1;
-}
 
 // Fortran nint: round half away from zero (x - trunc(x) is exact).
 RD_HD int nint(double x) {
