@@ -25,7 +25,7 @@
  *     (step, purpose<<40 | sub, chain, block);
  *   - the supply integral is the midpoint rule on quad x quad points of the
  *     emitter (100 x 100 (xi, phi) cells on the tip, as the reference),
- *     summed in point order;
+ *     summed in tiles of 1024 points, pairwise within a tile, tiles in order;
  *   - all chains of a step see the particles of the start of the step; the
  *     planar FE width is updated per jump from the batch acceptance rate, the
  *     thermal-field and tip widths per chain in chain order after the step.
@@ -521,7 +521,7 @@ static void do_emission(rdo_sim* s, int step) {
     }
   }
   field_at(s, M, pts, fld);
-  double sum = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  double* rv = malloc(sizeof(double) * 4 * M); /* val, fx, fy, F (or fz) per point */
   const int ic = s->g.image_charge;
   for (int k = 0; k < M; ++k) {
     double val = 0.0;
@@ -538,7 +538,9 @@ static void do_emission(rdo_sim* s, int step) {
         rdo_get_consts(&c);
         val = A_f * a_FN_() * (F * F) * s->dt / (c.q_0 * s->w_tip * (t * t));
       }
-      f2 = f2 + F;
+      rv[4 * k + 1] = 0.0;
+      rv[4 * k + 2] = 0.0;
+      rv[4 * k + 3] = F;
     } else {
       double F = fld[3 * k + 2];
       if (F < 0.0) {
@@ -548,12 +550,26 @@ static void do_emission(rdo_sim* s, int step) {
         else val = rdo_gtf(F, s->T, w) * s->dt_q0;
       }
       val = (s->elx * s->ely) * val;
-      f0 = f0 + fld[3 * k];
-      f1 = f1 + fld[3 * k + 1];
-      f2 = f2 + fld[3 * k + 2];
+      rv[4 * k + 1] = fld[3 * k];
+      rv[4 * k + 2] = fld[3 * k + 1];
+      rv[4 * k + 3] = fld[3 * k + 2];
     }
-    sum = sum + val;
+    rv[4 * k] = val;
   }
+  /* the device's order: tiles of 1024 points, a pairwise tree within a tile
+     (zeros beyond the last point), tile sums added in tile order */
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < M; b += 1024) {
+    double t[4][1024];
+    for (int q = 0; q < 4; ++q)
+      for (int j = 0; j < 1024; ++j) t[q][j] = (b + j < M) ? rv[4 * (b + j) + q] : 0.0;
+    for (int w = 512; w > 0; w >>= 1)
+      for (int q = 0; q < 4; ++q)
+        for (int j = 0; j < w; ++j) t[q][j] = t[q][j] + t[q][j + w];
+    for (int q = 0; q < 4; ++q) tot[q] = tot[q] + t[q][0];
+  }
+  free(rv);
+  const double sum = tot[0], f0 = tot[1], f1 = tot[2], f2 = tot[3];
   int nr;
   if (tip) {
     s->N_sup = sum;

@@ -29,8 +29,9 @@
 // reference updates the width once per chain, apply the per-chain updates in
 // chain order after the step's chains (all chains of a step use the width of
 // the start of the step).  The supply integral replaces the Cuba adaptive
-// integrator (not available) with a midpoint rule on a fixed grid summed in
-// point order.  Random numbers come from Philox4x64-10 streams keyed by the
+// integrator (not available) with a midpoint rule on a fixed grid, summed in
+// tiles of 1024 points (a fixed pairwise tree within a tile, tiles in order).
+// Random numbers come from Philox4x64-10 streams keyed by the
 // simulation seed and indexed by (step, purpose, chain).
 #include <hip/hip_runtime.h>
 
@@ -180,8 +181,10 @@ __global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimD
                                                      const double* __restrict__ fz,
                                                      double* __restrict__ qv,
                                                      double* __restrict__ qn) {
-  constexpr int kTile = 512;
-  __shared__ double t0[kTile], t1[kTile], t2[kTile], t3[kTile];
+  // LDS: four 1024-entry reduction rows, reused as the Poisson uniforms
+  constexpr int kTile = 1024;
+  __shared__ double lds[4 * kTile];
+  double *t0 = lds, *t1 = lds + kTile, *t2 = lds + 2 * kTile, *t3 = lds + 3 * kTile;
   const Ent e = tab[blockIdx.x];
   const SimDesc& s = desc[e.slot];
   const EmitConsts ec = emit_consts();
@@ -220,51 +223,42 @@ __global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimD
     }
     qv[p] = val;
   }
-  // The point-ordered sums, one lane adding tiles staged in LDS.
+  // The sums over the points: tiles of 1024 points, a fixed pairwise tree
+  // within a tile (zeros beyond the last point), tile sums added in order.
   double sum = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
   for (int b = 0; b < e.m; b += kTile) {
-    const int cnt = min(kTile, e.m - b);
+    const int k = threadIdx.x;  // blockDim.x == kTile
     __syncthreads();
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    if (b + k < e.m) {
       const int p = e.pbase + b + k;
       t0[k] = qv[p];
       if (tip) {
+        t1[k] = 0.0;
+        t2[k] = 0.0;
         t3[k] = qn[p];
       } else {
         t1[k] = fx[p];
         t2[k] = fy[p];
         t3[k] = fz[p];
       }
+    } else {
+      t0[k] = t1[k] = t2[k] = t3[k] = 0.0;
+    }
+    for (int w = kTile / 2; w > 0; w >>= 1) {
+      __syncthreads();
+      if (k < w) {
+        t0[k] = t0[k] + t0[k + w];
+        t1[k] = t1[k] + t1[k + w];
+        t2[k] = t2[k] + t2[k + w];
+        t3[k] = t3[k] + t3[k + w];
+      }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      // loads of eight points issued together, additions in point order
-      for (int k0 = 0; k0 < cnt; k0 += 8) {
-        double a0[8], a1[8], a2[8], a3[8];
-        const int nk = min(8, cnt - k0);
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          if (j < nk) {
-            a0[j] = t0[k0 + j];
-            a3[j] = t3[k0 + j];
-            if (!tip) {
-              a1[j] = t1[k0 + j];
-              a2[j] = t2[k0 + j];
-            }
-          }
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          if (j < nk) {
-            sum = sum + a0[j];
-            if (tip) {
-              f2 = f2 + a3[j];
-            } else {
-              f0 = f0 + a1[j];
-              f1 = f1 + a2[j];
-              f2 = f2 + a3[j];
-            }
-          }
-      }
+    if (k == 0) {
+      sum = sum + t0[0];
+      f0 = f0 + t1[0];
+      f1 = f1 + t2[0];
+      f2 = f2 + t3[0];
     }
   }
   SimState& S = st[e.slot];
@@ -273,8 +267,8 @@ __global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimD
   if (s.mode == RD_EMIT_PLANAR_GTF) {
     // Rand_Poisson: the product of uniforms runs in one lane; the uniforms,
     // in stream order, are drawn kPois at a time by the whole block.
-    constexpr int kPois = 4096;
-    __shared__ double ut[kPois];
+    constexpr int kPois = 4 * kTile;
+    double* ut = lds;
     __shared__ int s_more;
     __syncthreads();
     const double lambda = s_lambda;
@@ -884,14 +878,25 @@ __global__ __launch_bounds__(256) void k_mh_finish(const Ent* tab, const SimDesc
       }
       __syncthreads();
       if (threadIdx.x == 0)
-        for (int k = 0; k < cnt; ++k) {
-          if (rate[k] < 0.0) continue;
-          ar = rate[k];
-          mh = mh * fac[k];
-          if (mh > 0.1250)
-            mh = 0.1250;
-          else if (mh < lo)
-            mh = lo;
+        for (int k0 = 0; k0 < cnt; k0 += 8) {
+          // eight loads issued together, updates in chain order
+          double rk[8], fk[8];
+          const int nk = min(8, cnt - k0);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            rk[j] = j < nk ? rate[k0 + j] : -1.0;
+            fk[j] = j < nk ? fac[k0 + j] : 1.0;
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            if (rk[j] < 0.0) continue;
+            ar = rk[j];
+            mh = mh * fk[j];
+            if (mh > 0.1250)
+              mh = 0.1250;
+            else if (mh < lo)
+              mh = lo;
+          }
         }
     }
     if (threadIdx.x == 0) {
