@@ -162,7 +162,7 @@ def e2e_bench(name, steps, precision):
     wall = time.perf_counter() - t0
     half = recs[len(recs) // 2:]
     force_err = None
-    if cfg["kind"] == "planar" and dd.n > 1:
+    if dd.n > 1:
         # the final snapshot: mixed-precision accelerations against plain FP64
         a64 = dd.calculate_acceleration_particles("fp64")
         a32 = dd.calculate_acceleration_particles("mixed")

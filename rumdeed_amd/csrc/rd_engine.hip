@@ -568,9 +568,9 @@ __device__ __forceinline__ void mh_record(const Ent& e, const SimDesc& s, const 
   ptz[p] = P.z;
 }
 
-// Accept/reject of jump `it` for every chain of entry e, by one block of 256
-// lanes (ra, rr: 256-entry LDS arrays for the batch acceptance counts; lr:
-// the log-uniforms of the jump, by chain).
+// Accept/reject of jump `it` for every chain of entry e, by one block (a
+// power of two of lanes; ra, rr: LDS arrays of blockDim.x entries for the
+// batch acceptance counts; lr: the log-uniforms of the jump, by chain).
 __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, const double* wf,
                                 const Chains& ch, int it, const double* ptx, const double* pty,
                                 const double* ptz, const double* fx, const double* fy,
@@ -630,7 +630,7 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
   ra[threadIdx.x] = na;
   rr[threadIdx.x] = nrj;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       ra[threadIdx.x] += ra[threadIdx.x + w];
       rr[threadIdx.x] += rr[threadIdx.x + w];
@@ -761,6 +761,7 @@ __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, con
   fz += (double)cz;
 }
 
+template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const SimDesc* desc,
                                                        const SimState* st, Chains ch, int it,
                                                        JumpArgs a) {
@@ -768,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const Sim
   __shared__ double red[3][4][kJumpTargets];
   const Ent e = tab[blockIdx.z];
   const SimDesc& s = desc[e.slot];
-  if (s.g.kind != RD_GEOM_PLANAR || it > s.ndim) return;
+  if (s.g.kind != KIND || it > s.ndim) return;
   const int m = e.nch;
   if ((int)(blockIdx.x * kJumpTargets) >= m || (int)blockIdx.y >= e.segs) return;
   const DevGeom& g = s.g;
@@ -786,11 +787,17 @@ __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const Sim
   const int c1 = min(nchunks, c0 + e.cps);
   double fx = 0.0, fy = 0.0, fz = 0.0;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
-  if (act) pack1(P.x, P.y, P.z, 0.0, g.d, h1, l1);
+  float4 ih1 = make_float4(0, 0, 0, 1), il1 = make_float4(0, 0, 0, 0);
+  if (act) {
+    pack1(P.x, P.y, P.z, 0.0, g.d, h1, l1);
+    if (KIND == RD_GEOM_TIP) pack_image(P.x, P.y, P.z, g.z0, g.R, ih1, il1);
+  }
   const float4* hi = a.HI + e.base;
   const float4* lo = a.LO + e.base;
   const int ic = g.ic, nic = g.nic;
   const float two_d = (float)(2.0 * g.d * 1e9);
+  const float kt = g.ic ? (float)g.ktip : 0.0f;
+  const float Rn = (float)(g.R * 1e9);
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
@@ -800,8 +807,20 @@ __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const Sim
     }
     __syncthreads();
     const int j0 = w * 64, j1 = min(min(kChunk, n - jb), j0 + 64);
-    if (act)
-      for (int jj = j0; jj < j1; ++jj) mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, fx, fy, fz);
+    if (act) {
+      if (KIND == RD_GEOM_PLANAR) {
+        for (int jj = j0; jj < j1; ++jj)
+          mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, fx, fy, fz);
+      } else {
+        for (int jj = j0; jj < j1; ++jj) {
+          float cx, cy, cz;
+          tip_pair32_target(h1, l1, ih1, il1, shi[jj], slo[jj], kt, Rn, cx, cy, cz);
+          fx += (double)cx;
+          fy += (double)cy;
+          fz += (double)cz;
+        }
+      }
+    }
   }
   red[0][w][lane] = fx;
   red[1][w][lane] = fy;
@@ -819,11 +838,13 @@ __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const Sim
 // Finish and accept: one workgroup per simulation.  FP64 adds the chunk
 // partials of every chain in chunk order (plus the vacuum field); mixed adds
 // the segment partials in segment order.
+constexpr int kSettle = 1024;  // lanes of the per-simulation finish and accept
+
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_mh_settle(const Ent* tab, const SimDesc* desc,
-                                                      SimState* st, const double* wf, Chains ch,
-                                                      int it, JumpArgs a) {
-  __shared__ int ra[256], rr[256];
+__global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, const SimDesc* desc,
+                                                       SimState* st, const double* wf, Chains ch,
+                                                       int it, JumpArgs a) {
+  __shared__ int ra[kSettle], rr[kSettle];
   const Ent e = tab[blockIdx.x];
   const SimDesc& s = desc[e.slot];
   if (it > s.ndim) return;
@@ -838,7 +859,11 @@ __global__ __launch_bounds__(kBlock) void k_mh_settle(const Ent* tab, const SimD
                      a.ptz[p], a.fx[p], a.fy[p], a.fz[p]);
     } else {
       const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
-      field_finish32(s.g, part, (size_t)e.segs * m, m, k, segs, a.fx[p], a.fy[p], a.fz[p]);
+      if (s.g.kind == RD_GEOM_TIP)
+        field_finish32_tip(s.g, part, (size_t)e.segs * m, m, k, segs, a.ptx[p], a.pty[p],
+                           a.ptz[p], a.fx[p], a.fy[p], a.fz[p]);
+      else
+        field_finish32(s.g, part, (size_t)e.segs * m, m, k, segs, a.fx[p], a.fy[p], a.fz[p]);
     }
   }
   __syncthreads();
@@ -1554,8 +1579,6 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     ents.push_back(e);
   }
   if (ents.empty()) return RD_OK;
-  if (!fp64 && any_tip)
-    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
   JumpArgs a;
   double* ptx = (double*)c->pts.p;
   double* f = (double*)c->fld.p;
@@ -1621,11 +1644,16 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
       if (any_tip)
         k_mh_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, V,
                                                                    it, a);
-      k_mh_settle<RD_PREC_FP64><<<ne, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_wf,
+      k_mh_settle<RD_PREC_FP64><<<ne, kSettle, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_wf,
                                                                V, it, a);
     } else {
-      k_mh_field32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, V, it, a);
-      k_mh_settle<RD_PREC_MIXED><<<ne, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+      if (any_planar)
+        k_mh_field32<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                                      V, it, a);
+      if (any_tip)
+        k_mh_field32<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, V,
+                                                                   it, a);
+      k_mh_settle<RD_PREC_MIXED><<<ne, kSettle, 0, c->stream>>>(dt, c->d_desc, c->d_state,
                                                                 c->d_wf, V, it, a);
     }
   }

@@ -359,4 +359,142 @@ __device__ __forceinline__ void field_finish32(const DevGeom& g, const double* _
   fz = g.Ez + pre * sz;
 }
 
+// Field (V/m) of the tip geometry from the segment sums of a mixed
+// evaluation: the FP64 vacuum field at the point plus the particle sums.
+__device__ __forceinline__ void field_finish32_tip(const DevGeom& g, const double* __restrict__ P,
+                                                   size_t S, int m, int p, int segs, double x,
+                                                   double y, double z, double& fx, double& fy,
+                                                   double& fz) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int s = 0; s < segs; ++s) {
+    sx += P[0 * S + (size_t)s * m + p];
+    sy += P[1 * S + (size_t)s * m + p];
+    sz += P[2 * S + (size_t)s * m + p];
+  }
+  double ex, ey, ez;
+  tip_vacuum64(g, x, y, z, ex, ey, ez);
+  const double pre = g.dfc * Consts::q_0 * 1.0e18;
+  fx = ex + pre * sx;
+  fy = ey + pre * sy;
+  fz = ez + pre * sz;
+}
+
+// ------------------------------------------------------------------------
+// Mixed precision, hyperboloid tip.  The sphere image of a point (FP64,
+// tip_image_point64) is packed like a position: ihi = (x_im, y_im, z_im in nm
+// as float, distance of the point from the sphere centre in nm), ilo = the
+// low parts of the coordinates.  Pair terms are FP32 in nm; the image term
+// keeps the reference's factor k_tip (acc_tip_ic_force.inc:17-22), so a
+// pair adds  q_j/q_0 * [ d/r^3 + k_tip (d/r^3 + R e/(D t^3)) ]  (nm^-2,
+// scaled by 1e18 * dfc * q_i * q_0 at the end), with d = target - source,
+// e = source - image, t = |e|, D the image owner's distance from the centre.
+__device__ __forceinline__ void pack_image(double xm, double ym, double zm, double z0, double R,
+                                           float4& ihi, float4& ilo) {
+  double da, xi, yi, zi;
+  tip_image_point64(xm, ym, zm, z0, R, da, xi, yi, zi);
+  const double s = 1.0e9;
+  double xn = xi * s, yn = yi * s, zn = zi * s;
+  float xh = (float)xn, yh = (float)yn, zh = (float)zn;
+  ihi = make_float4(xh, yh, zh, (float)(da * s));
+  ilo = make_float4((float)(xn - (double)xh), (float)(yn - (double)yh), (float)(zn - (double)zh),
+                    0.0f);
+}
+
+// Target-image pair (the field at points, and accelerations with j > i):
+// the image is the target's.  Returns the FP32 terms scaled by q_j/q_0.
+__device__ __forceinline__ void tip_pair32_target(const float4 h1, const float4 l1,
+                                                  const float4 ih1, const float4 il1,
+                                                  const float4 h2, const float4 l2, float kt,
+                                                  float Rn, float& cx, float& cy, float& cz) {
+  const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+  const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+  const float dz = (h1.z - h2.z) + (l1.z - l2.z);
+  const float r2 = fmaxf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), kR2Min);
+  const float ri = rsq(r2);
+  const float r3 = ri * ri * ri;
+  if (kt == 0.0f) {  // no image charge
+    cx = h2.w * (dx * r3);
+    cy = h2.w * (dy * r3);
+    cz = h2.w * (dz * r3);
+    return;
+  }
+  // e = source - image of the target
+  const float ex = (h2.x - ih1.x) + (l2.x - il1.x);
+  const float ey = (h2.y - ih1.y) + (l2.y - il1.y);
+  const float ez = (h2.z - ih1.z) + (l2.z - il1.z);
+  const float t2 = fmaxf(__builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)), kR2Min);
+  const float ti = rsq(t2);
+  const float w = Rn * (ti * ti * ti) / ih1.w;
+  const float a = 1.0f + kt;
+  cx = h2.w * (dx * r3 * a + kt * (w * ex));
+  cy = h2.w * (dy * r3 * a + kt * (w * ey));
+  cz = h2.w * (dz * r3 * a + kt * (w * ez));
+}
+
+// Source-image pair (accelerations with j < i): the image is the source's;
+// the reference negates the x and y image components (acc_tip_ic_force.inc
+// with the operands exchanged).  f = target - image of the source.
+__device__ __forceinline__ void tip_pair32_source(const float4 h1, const float4 l1,
+                                                  const float4 h2, const float4 l2,
+                                                  const float4 ih2, const float4 il2, float kt,
+                                                  float Rn, float& cx, float& cy, float& cz) {
+  const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+  const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+  const float dz = (h1.z - h2.z) + (l1.z - l2.z);
+  const float r2 = fmaxf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), kR2Min);
+  const float ri = rsq(r2);
+  const float r3 = ri * ri * ri;
+  if (kt == 0.0f) {  // no image charge
+    cx = h2.w * (dx * r3);
+    cy = h2.w * (dy * r3);
+    cz = h2.w * (dz * r3);
+    return;
+  }
+  const float fx = (h1.x - ih2.x) + (l1.x - il2.x);
+  const float fy = (h1.y - ih2.y) + (l1.y - il2.y);
+  const float fz = (h1.z - ih2.z) + (l1.z - il2.z);
+  const float t2 = fmaxf(__builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)), kR2Min);
+  const float ti = rsq(t2);
+  const float w = Rn * (ti * ti * ti) / ih2.w;
+  // x, y: d/r^3 + k_tip (d/r^3 - R f/(D t^3));  z: d/r^3 + k_tip (-d/r^3 + R f/(D t^3))
+  cx = h2.w * (dx * r3 * (1.0f + kt) - kt * (w * fx));
+  cy = h2.w * (dy * r3 * (1.0f + kt) - kt * (w * fy));
+  cz = h2.w * (dz * r3 * (1.0f - kt) + kt * (w * fz));
+}
+
+// Mixed (tip): FP64 sums of the packed sources of chunks [c0, c1) at the
+// packed target (h1, l1) with its packed image (ih1, il1).
+__device__ __forceinline__ void field_seg32_tip(const DevGeom& g, bool act, float4 h1, float4 l1,
+                                                float4 ih1, float4 il1,
+                                                const float4* __restrict__ hi,
+                                                const float4* __restrict__ lo, int n, int c0,
+                                                int c1, float4* shi, float4* slo, double& fx,
+                                                double& fy, double& fz) {
+  const float kt = g.ic ? (float)g.ktip : 0.0f;
+  const float Rn = (float)(g.R * 1e9);
+  const int t = threadIdx.x;
+  fx = 0.0;
+  fy = 0.0;
+  fz = 0.0;
+  for (int c = c0; c < c1; ++c) {
+    const int jb = c * kChunk;
+    __syncthreads();
+    if (jb + t < n) {
+      shi[t] = hi[jb + t];
+      slo[t] = lo[jb + t];
+    }
+    __syncthreads();
+    const int cnt = min(kChunk, n - jb);
+    if (act) {
+      for (int jj = 0; jj < cnt; ++jj) {
+        float cx, cy, cz;
+        tip_pair32_target(h1, l1, ih1, il1, shi[jj], slo[jj], kt, Rn, cx, cy, cz);
+        fx += (double)cx;
+        fy += (double)cy;
+        fz += (double)cz;
+      }
+    }
+  }
+}
+
 }  // namespace rd

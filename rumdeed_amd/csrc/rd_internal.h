@@ -143,6 +143,7 @@ struct rd_ctx {
   // work buffers (grown on demand, never shrunk)
   rd::Buf part, pack_hi, pack_lo, pts, fld, pts_hi, pts_lo, stage, img64, tab, red, chains;
   rd::Buf rnd;  // random numbers of the Metropolis-Hastings jumps of a step
+  rd::Buf img_hi, img_lo;  // packed sphere images of the particles (mixed, tip)
   rd::Ent* h_tab = nullptr;  // pinned staging of launch tables (ring)
   int tab_used = 0;          // ring entries used since the last stream synchronisation
   int tab_last = -1;         // ring index of the last upload

@@ -390,9 +390,11 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, const
 }
 
 __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimState* st,
-                                 const double* __restrict__ part, const double* __restrict__ Q,
-                                 const double* __restrict__ M, double* __restrict__ AX,
-                                 double* __restrict__ AY, double* __restrict__ AZ) {
+                                 const double* __restrict__ part, const double* __restrict__ X,
+                                 const double* __restrict__ Y, const double* __restrict__ Z,
+                                 const double* __restrict__ Q, const double* __restrict__ M,
+                                 double* __restrict__ AX, double* __restrict__ AY,
+                                 double* __restrict__ AZ) {
   const Ent e = tab[blockIdx.z];
   const int n = st[e.slot].n;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,9 +414,148 @@ __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimS
   double q1 = Q[gi];
   double im = 1.0 / M[gi];
   double pre = g.dfc * q1 * Consts::q_0 * 1.0e18 * im;
+  if (g.kind == RD_GEOM_TIP) {
+    double ex, ey, ez;
+    tip_vacuum64(g, X[gi], Y[gi], Z[gi], ex, ey, ez);
+    AX[gi] = pre * sx + q1 * ex * im;
+    AY[gi] = pre * sy + q1 * ey * im;
+    AZ[gi] = pre * sz + q1 * ez * im;
+    return;
+  }
   AX[gi] = pre * sx;
   AY[gi] = pre * sy;
   AZ[gi] = pre * sz + q1 * g.Ez * im;
+}
+
+// Packed sphere images of every particle of the tip entries (mixed).
+__global__ void k_pack_images(const Ent* tab, const SimDesc* desc, const SimState* st,
+                              const double* __restrict__ X, const double* __restrict__ Y,
+                              const double* __restrict__ Z, float4* __restrict__ ihi,
+                              float4* __restrict__ ilo) {
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevGeom& g = desc[e.slot].g;
+  if (g.kind != RD_GEOM_TIP) return;
+  const int gi = e.base + i;
+  float4 h, l;
+  pack_image(X[gi], Y[gi], Z[gi], g.z0, g.R, h, l);
+  ihi[gi] = h;
+  ilo[gi] = l;
+}
+
+// Mixed accelerations (tip).  Same layout as the planar kernel; a source
+// with j > i uses the target's sphere image, j < i the source's (the lower
+// index owns the image, as in Calculate_Acceleration_Particles_ACC).
+__global__ __launch_bounds__(kBlock) void k_accel_tip32(const Ent* tab, const SimDesc* desc,
+                                                        const SimState* st,
+                                                        const float4* __restrict__ HI,
+                                                        const float4* __restrict__ LO,
+                                                        const float4* __restrict__ IHI,
+                                                        const float4* __restrict__ ILO,
+                                                        double* __restrict__ part) {
+  __shared__ float4 shi[kChunk], slo[kChunk], sih[kChunk], sil[kChunk];
+  const Ent e = tab[blockIdx.z];
+  const int n = st[e.slot].n;
+  if ((int)(blockIdx.x * kBlock) >= n) return;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int c0 = blockIdx.y * e.cps;
+  if (c0 >= nchunks) return;
+  const int c1 = min(nchunks, c0 + e.cps);
+  const DevGeom& gg = desc[e.slot].g;
+  if (gg.kind != RD_GEOM_TIP) return;
+  const float kt = gg.ic ? (float)gg.ktip : 0.0f;
+  const float Rn = (float)(gg.R * 1e9);
+  const float4* hi = HI + e.base;
+  const float4* lo = LO + e.base;
+  const float4* ih = IHI + e.base;
+  const float4* il = ILO + e.base;
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * kBlock + t;
+  const bool act = i < n;
+  float4 h1 = make_float4(0, 0, 0, 0), l1 = h1, ih1 = make_float4(0, 0, 0, 1), il1 = h1;
+  if (act) {
+    h1 = hi[i];
+    l1 = lo[i];
+    ih1 = ih[i];
+    il1 = il[i];
+  }
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int c = c0; c < c1; ++c) {
+    const int jb = c * kChunk;
+    __syncthreads();
+    if (jb + t < n) {
+      shi[t] = hi[jb + t];
+      slo[t] = lo[jb + t];
+      sih[t] = ih[jb + t];
+      sil[t] = il[jb + t];
+    }
+    __syncthreads();
+    const int cnt = min(kChunk, n - jb);
+    if (act) {
+      for (int jj = 0; jj < cnt; ++jj) {
+        const int j = jb + jj;
+        float4 h2 = shi[jj];
+        if (j == i) h2.w = 0.0f;
+        float cx, cy, cz;
+        if (j > i)
+          tip_pair32_target(h1, l1, ih1, il1, h2, slo[jj], kt, Rn, cx, cy, cz);
+        else
+          tip_pair32_source(h1, l1, h2, slo[jj], sih[jj], sil[jj], kt, Rn, cx, cy, cz);
+        fx += (double)cx;
+        fy += (double)cy;
+        fz += (double)cz;
+      }
+    }
+  }
+  if (act) {
+    double* P = part + e.part_off;
+    const size_t S = (size_t)e.segs * e.n_ub;
+    P[0 * S + (size_t)blockIdx.y * e.n_ub + i] = fx;
+    P[1 * S + (size_t)blockIdx.y * e.n_ub + i] = fy;
+    P[2 * S + (size_t)blockIdx.y * e.n_ub + i] = fz;
+  }
+}
+
+// Mixed field at points (tip): the image of every point is its own.
+__global__ __launch_bounds__(kBlock) void k_field_tip32(const Ent* tab, const SimDesc* desc,
+                                                        const SimState* st,
+                                                        const double* __restrict__ ptx,
+                                                        const double* __restrict__ pty,
+                                                        const double* __restrict__ ptz,
+                                                        const float4* __restrict__ HI,
+                                                        const float4* __restrict__ LO,
+                                                        double* __restrict__ part) {
+  __shared__ float4 shi[kChunk], slo[kChunk];
+  const Ent e = tab[blockIdx.z];
+  const int m = e.m;
+  if ((int)(blockIdx.x * kBlock) >= m) return;
+  const int n = st[e.slot].n;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int c0 = blockIdx.y * e.cps;
+  if (c0 >= max(nchunks, 1)) return;
+  const int c1 = min(nchunks, c0 + e.cps);
+  const DevGeom& g = desc[e.slot].g;
+  if (g.kind != RD_GEOM_TIP) return;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const bool act = p < m;
+  float4 h1 = make_float4(0, 0, 0, 0), l1 = h1, ih1 = make_float4(0, 0, 0, 1), il1 = h1;
+  if (act) {
+    const double x = ptx[e.pbase + p], y = pty[e.pbase + p], z = ptz[e.pbase + p];
+    pack1(x, y, z, 0.0, g.d, h1, l1);
+    pack_image(x, y, z, g.z0, g.R, ih1, il1);
+  }
+  double fx, fy, fz;
+  field_seg32_tip(g, act, h1, l1, ih1, il1, HI + e.base, LO + e.base, n, c0, c1, shi, slo, fx,
+                  fy, fz);
+  if (act) {
+    double* P = part + e.part_off;
+    const size_t S = (size_t)e.segs * m;
+    P[0 * S + (size_t)blockIdx.y * m + p] = fx;
+    P[1 * S + (size_t)blockIdx.y * m + p] = fy;
+    P[2 * S + (size_t)blockIdx.y * m + p] = fz;
+  }
 }
 
 // Mixed field at points (planar).
@@ -455,8 +596,10 @@ __global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, const
 }
 
 __global__ void k_field_finish32(const Ent* tab, const SimDesc* desc, const SimState* st,
-                                 const double* __restrict__ part, double* __restrict__ fx,
-                                 double* __restrict__ fy, double* __restrict__ fz) {
+                                 const double* __restrict__ part, const double* __restrict__ ptx,
+                                 const double* __restrict__ pty, const double* __restrict__ ptz,
+                                 double* __restrict__ fx, double* __restrict__ fy,
+                                 double* __restrict__ fz) {
   const Ent e = tab[blockIdx.z];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= e.m) return;
@@ -464,8 +607,13 @@ __global__ void k_field_finish32(const Ent* tab, const SimDesc* desc, const SimS
   const int nchunks = (n + kChunk - 1) / kChunk;
   const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
   const int gp = e.pbase + p;
-  field_finish32(desc[e.slot].g, part + e.part_off, (size_t)e.segs * e.m, e.m, p, segs, fx[gp],
-                 fy[gp], fz[gp]);
+  const DevGeom& g = desc[e.slot].g;
+  if (g.kind == RD_GEOM_TIP)
+    field_finish32_tip(g, part + e.part_off, (size_t)e.segs * e.m, e.m, p, segs, ptx[gp],
+                       pty[gp], ptz[gp], fx[gp], fy[gp], fz[gp]);
+  else
+    field_finish32(g, part + e.part_off, (size_t)e.segs * e.m, e.m, p, segs, fx[gp], fy[gp],
+                   fz[gp]);
 }
 
 __global__ void k_grid_points(int nx, int ny, double x0, double y0, double lx, double ly,
@@ -546,8 +694,6 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   if (ents.empty()) return RD_OK;
   if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
     return fail(RD_ERR_ARG, "unknown precision");
-  if (precision == RD_PREC_MIXED && any_tip)
-    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
   long long off = 0;
   int maxsegs = 1;
   for (Ent& e : ents) {
@@ -594,13 +740,25 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   k_pack_particles<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->x, c->y, c->z,
                                                 c->q, (float4*)c->pack_hi.p,
                                                 (float4*)c->pack_lo.p);
+  if (any_tip) {
+    r = ensure(c, c->img_hi, sizeof(float4) * (size_t)c->cap);
+    if (!r) r = ensure(c, c->img_lo, sizeof(float4) * (size_t)c->cap);
+    if (r) return r;
+    k_pack_images<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->x, c->y, c->z,
+                                               (float4*)c->img_hi.p, (float4*)c->img_lo.p);
+  }
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-  k_accel_planar32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
-                                                    (const float4*)c->pack_hi.p,
-                                                    (const float4*)c->pack_lo.p, part);
+  if (any_planar)
+    k_accel_planar32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                      (const float4*)c->pack_hi.p,
+                                                      (const float4*)c->pack_lo.p, part);
+  if (any_tip)
+    k_accel_tip32<<<grid, kBlock, 0, c->stream>>>(
+        dt, c->d_desc, c->d_state, (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p,
+        (const float4*)c->img_hi.p, (const float4*)c->img_lo.p, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-  k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->q, c->m,
-                                                c->ax, c->ay, c->az);
+  k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->x, c->y,
+                                                c->z, c->q, c->m, c->ax, c->ay, c->az);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }
@@ -622,8 +780,6 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   if (ents.empty()) return RD_OK;
   if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
     return fail(RD_ERR_ARG, "unknown precision");
-  if (precision == RD_PREC_MIXED && any_tip)
-    return fail(RD_ERR_ARG, "mixed precision is implemented for the planar geometry only");
   long long off = 0;
   int maxsegs = 1, maxn = 0;
   for (Ent& e : ents) {
@@ -678,11 +834,17 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
         dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (float4*)c->pack_hi.p,
         (float4*)c->pack_lo.p);
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-  k_field_planar32<<<grid, kBlock, 0, c->stream>>>(
-      dt, c->d_desc, c->d_state, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p,
-      (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p, part);
+  if (any_planar)
+    k_field_planar32<<<grid, kBlock, 0, c->stream>>>(
+        dt, c->d_desc, c->d_state, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p,
+        (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p, part);
+  if (any_tip)
+    k_field_tip32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, ptx, pty, ptz,
+                                                   (const float4*)c->pack_hi.p,
+                                                   (const float4*)c->pack_lo.p, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-  k_field_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, fx, fy, fz);
+  k_field_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, ptx, pty, ptz,
+                                                fx, fy, fz);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }

@@ -130,6 +130,49 @@ def test_mixed_error_vs_long_double(Diode, oracle_mod, n, nic):
     assert ef.max() < MIXED_TOL
 
 
+def vacuum_accel(oracle_mod, g, p, q, m):
+    return q / m * oracle_mod.vacuum_field(g, p)
+
+
+@pytest.mark.parametrize("n", [1, 300, 3000])
+@pytest.mark.parametrize("ic", [0, 1])
+def test_mixed_tip_error_vs_long_double(Diode, oracle_mod, n, ic):
+    """Mixed precision on the hyperboloid tip (vacuum field in FP64, pair and
+    sphere-image terms in FP32): per-particle relative error <= 1e-5 against
+    the long-double oracle, for accelerations and for the field at points."""
+    V, dtip, rb, ht = 1000.0, 1000e-9, 250e-9, 500e-9
+    rng = np.random.Generator(np.random.Philox(17))
+    pos = np.column_stack([rng.uniform(-150e-9, 150e-9, n), rng.uniform(-150e-9, 150e-9, n),
+                           rng.uniform(ht + 5e-9, ht + dtip - 5e-9, n)])
+    q = np.full(n, -1.602176634e-19)
+    m = np.full(n, 9.1093837015e-31)
+    dd = Diode(n + 16)
+    dd.set_tip(V, dtip, rb, ht, bool(ic))
+    dd.put_particles(pos, q, m)
+    g = oracle_mod.geom_tip(V, dtip, rb, ht, bool(ic))
+    ref = oracle_mod.accel(g, pos, q, m, long_double=True)
+    am = dd.calculate_acceleration_particles("mixed")
+    e = rel_err(am, ref)
+    print(f"mixed tip accel n={n} ic={ic}: max {e.max():.2e}")
+    assert e.max() < MIXED_TOL
+    if n > 1:
+        # the particle part alone (the vacuum field dominates the total):
+        # relative to its own size, floored at 1% of its mean
+        vac = np.array([vacuum_accel(oracle_mod, g, p, qq, mm) for p, qq, mm in zip(pos, q, m)])
+        pr, pm = ref - vac, am - vac
+        floor = 0.01 * np.linalg.norm(pr, axis=1).mean()
+        ep = rel_err(pm, pr, floor=floor)
+        print(f"mixed tip particle part n={n} ic={ic}: max {ep.max():.2e}")
+        assert ep.max() < MIXED_TOL
+    pts = np.column_stack([rng.uniform(-100e-9, 100e-9, 64), rng.uniform(-100e-9, 100e-9, 64),
+                           rng.uniform(ht + 1e-9, ht + dtip - 1e-9, 64)])
+    fref = oracle_mod.field(g, pos, q, pts, long_double=True)
+    fm = dd.calc_field_at_batch(pts, "mixed")
+    ef = rel_err(fm, fref)
+    print(f"mixed tip field n={n} ic={ic}: max {ef.max():.2e}")
+    assert ef.max() < MIXED_TOL
+
+
 def test_absorbed_particles_have_no_effect(Diode, oracle_mod):
     """Particles with charge 0 (marked for removal, src/mod_pair.F90:206-207)
     contribute nothing but still receive their vacuum-field acceleration."""

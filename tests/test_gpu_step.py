@@ -213,6 +213,31 @@ def test_mixed_precision_run_statistics(Diode, oracle_mod):
     assert abs(e64 - e32) < 5 * np.sqrt(e64 + 1)
 
 
+def test_mixed_tip_run_statistics(Diode):
+    """Tip field emission in mixed precision follows the FP64 run: the same
+    supply on the first step (empty diode) and emitted totals within counting
+    noise."""
+    def setup(dd):
+        dd.set_tip(1000.0, 1000 * NM, 250 * NM, 500 * NM, True, box_top=1500 * NM)
+        dd.set_time_step(0.25e-15)
+        dd.set_seed(13)
+        dd.set_emitter(3, 0, w_tip=4.5)
+
+    d64 = Diode(1 << 14)
+    d32 = Diode(1 << 14)
+    setup(d64)
+    setup(d32)
+    r64 = d64.run(12, "fp64")
+    r32 = d32.run(12, "mixed")
+    assert r32[0].n_round == r64[0].n_round
+    assert abs(r32[0].n_sup - r64[0].n_sup) <= 1e-12 * r64[0].n_sup
+    e64 = sum(r.emitted for r in r64)
+    e32 = sum(r.emitted for r in r32)
+    print("tip emitted fp64", e64, "mixed", e32, flush=True)
+    assert e64 > 10
+    assert abs(e64 - e32) < 5 * np.sqrt(e64)
+
+
 def test_mixed_precision_many_chains_statistics(Diode):
     """A 1000 x 1000 nm emitter at 8 GV/m starts hundreds of chains per step,
     so the jumps take the multi-workgroup path in both precisions; the mixed
