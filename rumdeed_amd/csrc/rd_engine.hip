@@ -676,11 +676,22 @@ struct JumpArgs {
 // that none of them is on a jump's critical path: the unit normal pair of the
 // proposal and the log of the accept uniform, from the chain's (step, jump)
 // streams.
-__global__ void k_mh_random(const Ent* tab, const SimDesc* desc, int C, double* __restrict__ u1,
-                            double* __restrict__ u2, double* __restrict__ lr) {
-  const Ent e = tab[blockIdx.z];
+__global__ void k_mh_random(const Ent* tab, int ne, const SimDesc* desc, int C,
+                            double* __restrict__ u1, double* __restrict__ u2,
+                            double* __restrict__ lr) {
+  const int cg = blockIdx.x * blockDim.x + threadIdx.x;  // chain over all entries
+  if (cg >= C) return;
+  int lo = 0, hi = ne - 1;  // entry holding chain cg (chains are laid out in entry order)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].cbase <= cg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const Ent e = tab[lo];
   const SimDesc& s = desc[e.slot];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = cg - e.cbase;
   const int it = blockIdx.y + 1;
   if (k >= e.nch || it > s.ndim) return;
   const size_t o = (size_t)(it - 1) * C + e.cbase + k;
@@ -704,26 +715,28 @@ __device__ __forceinline__ Proposal jump_proposal(const Ent& e, const SimDesc& s
 // chunks (the canonical order of the field at points: partials per chunk,
 // bit-identical to the oracle).
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_mh_field64(const Ent* tab, const SimDesc* desc,
-                                                       const SimState* st, Chains ch, int it,
-                                                       JumpArgs a) {
+__global__ __launch_bounds__(kBlock) void k_mh_field64(const Ent* tab, int ne,
+                                                       const SimDesc* desc, const SimState* st,
+                                                       Chains ch, int it, JumpArgs a) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
-  const Ent e = tab[blockIdx.z];
+  const Ent e = tab[find_entry(tab, ne, blockIdx.x)];
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != KIND || it > s.ndim) return;
   const int m = e.nch;
-  if ((int)(blockIdx.x * kBlock) >= m || (int)blockIdx.y >= e.segs) return;
+  const int gx = (m + kBlock - 1) / kBlock;
+  const int local = blockIdx.x - e.wg0;
+  const int bx = local % gx, by = local / gx;  // chain group, source segment
   const DevGeom g = s.g;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const int k = bx * kBlock + threadIdx.x;
   const bool act = k < m;
   Proposal P{};
   if (act) {
     P = jump_proposal(e, s, st, ch, it, k, a);
-    if (blockIdx.y == 0) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
+    if (by == 0) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
   }
-  const int c0 = blockIdx.y * e.cps;
+  const int c0 = by * e.cps;
   const int c1 = min(nchunks, c0 + e.cps);
   if (c0 < nchunks)
     field_chunks64<KIND>(g, act, P.x, P.y, P.z, a.X + e.base, a.Y + e.base, a.Z + e.base,
@@ -762,28 +775,30 @@ __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, con
 }
 
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const SimDesc* desc,
-                                                       const SimState* st, Chains ch, int it,
-                                                       JumpArgs a) {
+__global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, int ne,
+                                                       const SimDesc* desc, const SimState* st,
+                                                       Chains ch, int it, JumpArgs a) {
   __shared__ float4 shi[kChunk], slo[kChunk];
   __shared__ double red[3][4][kJumpTargets];
-  const Ent e = tab[blockIdx.z];
+  const Ent e = tab[find_entry(tab, ne, blockIdx.x)];
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != KIND || it > s.ndim) return;
   const int m = e.nch;
-  if ((int)(blockIdx.x * kJumpTargets) >= m || (int)blockIdx.y >= e.segs) return;
+  const int gx = (m + kJumpTargets - 1) / kJumpTargets;
+  const int local = blockIdx.x - e.wg0;
+  const int bx = local % gx, by = local / gx;  // chain group, source segment
   const DevGeom& g = s.g;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int k = blockIdx.x * kJumpTargets + lane;
+  const int k = bx * kJumpTargets + lane;
   const bool act = k < m;
   Proposal P{};
   if (act) {
     P = jump_proposal(e, s, st, ch, it, k, a);
-    if (blockIdx.y == 0 && w == 0) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
+    if (by == 0 && w == 0) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
   }
-  const int c0 = blockIdx.y * e.cps;
+  const int c0 = by * e.cps;
   const int c1 = min(nchunks, c0 + e.cps);
   double fx = 0.0, fy = 0.0, fz = 0.0;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
@@ -830,7 +845,7 @@ __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, const Sim
     double* part = a.part + e.part_off;
     const size_t S = (size_t)e.segs * m;
     for (int q = 0; q < 3; ++q)
-      part[q * S + (size_t)blockIdx.y * m + k] =
+      part[q * S + (size_t)by * m + k] =
           ((red[q][0][lane] + red[q][1][lane]) + red[q][2][lane]) + red[q][3][lane];
   }
 }
@@ -1599,20 +1614,17 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   }
   a.HI = (const float4*)c->pack_hi.p;
   a.LO = (const float4*)c->pack_lo.p;
-  int maxg = 0;
-  for (const Ent& e : ents) {
-    const int gx = (e.nch + per - 1) / per;
-    maxg = std::max(maxg, gx);
-  }
   long long off = 0;
-  int maxsegs = 1;
+  int nwg = 0;  // workgroups of the compact field launch
   for (Ent& e : ents) {
     const int nch = (e.n_ub + kChunk - 1) / kChunk;
+    const int gx = (e.nch + per - 1) / per;
     e.nchunks_ub = nch;
-    segment(nch, seg_limit((e.nch + per - 1) / per, kWantJump), e.segs, e.cps);
-    maxsegs = std::max(maxsegs, e.segs);
+    segment(nch, seg_limit(gx, kWantJump), e.segs, e.cps);
     e.part_off = off;
     off += 3LL * (fp64 ? std::max(nch, 1) : e.segs) * e.nch;
+    e.wg0 = nwg;
+    nwg += gx * e.segs;
   }
   const int ne = (int)ents.size();
   r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
@@ -1624,35 +1636,33 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   r = upload_tab(c, ents, &dt);
   if (r) return r;
   {
-    int maxc = 0;
-    for (const Ent& e : ents) maxc = std::max(maxc, e.nch);
     double* rb = (double*)c->rnd.p;
-    k_mh_random<<<dim3((maxc + 255) / 256, maxndim, ne), 256, 0, c->stream>>>(
-        dt, c->d_desc, C, rb, rb + nr, rb + 2 * nr);
+    k_mh_random<<<dim3((C + 255) / 256, maxndim), 256, 0, c->stream>>>(dt, ne, c->d_desc, C, rb,
+                                                                         rb + nr, rb + 2 * nr);
     a.ru1 = rb;
     a.ru2 = rb + nr;
     a.rlr = rb + 2 * nr;
     a.C = C;
   }
   a.part = (double*)c->part.p;
-  const dim3 grid(maxg, maxsegs, ne);
+  const int grid = nwg;
   for (int it = 1; it <= maxndim; ++it) {
     if (fp64) {
       if (any_planar)
-        k_mh_field64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
-                                                                      V, it, a);
+        k_mh_field64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc,
+                                                                      c->d_state, V, it, a);
       if (any_tip)
-        k_mh_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, V,
-                                                                   it, a);
+        k_mh_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
+                                                                   V, it, a);
       k_mh_settle<RD_PREC_FP64><<<ne, kSettle, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_wf,
                                                                V, it, a);
     } else {
       if (any_planar)
-        k_mh_field32<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
-                                                                      V, it, a);
+        k_mh_field32<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc,
+                                                                      c->d_state, V, it, a);
       if (any_tip)
-        k_mh_field32<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, V,
-                                                                   it, a);
+        k_mh_field32<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
+                                                                   V, it, a);
       k_mh_settle<RD_PREC_MIXED><<<ne, kSettle, 0, c->stream>>>(dt, c->d_desc, c->d_state,
                                                                 c->d_wf, V, it, a);
     }

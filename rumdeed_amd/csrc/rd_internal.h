@@ -92,7 +92,22 @@ struct Ent {
   int pbase, m;   // field points [pbase, pbase + m)
   int cbase, nch; // MH chains [cbase, cbase + nch)
   int step, nblk; // time step; per-slot block count (reductions)
+  int wg0, pad;   // first workgroup of the entry in a compact launch
 };
+
+// Entry of a compact launch whose workgroups [wg0, next wg0) belong to one
+// entry: the last entry with wg0 <= b (binary search, uniform per block).
+__device__ __forceinline__ int find_entry(const Ent* tab, int ne, int b) {
+  int lo = 0, hi = ne - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].wg0 <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
 
 constexpr int kChunk = 256;  // canonical FP64 summation chunk
 constexpr int kBlock = 256;

@@ -144,7 +144,9 @@ class Server:
                 self._configure(slot, r)
                 pending.popleft()
                 used += fp
-                res = Result(rid=r.rid, arrival=r.arrival, admitted=now, finished=0.0,
+                # without real-time arrivals every request is available at t = 0
+                res = Result(rid=r.rid, arrival=r.arrival if realtime else 0.0, admitted=now,
+                             finished=0.0,
                              steps=r.steps, capacity=cap)
                 active[slot] = [r, res, fp, 0]
             if not active:
