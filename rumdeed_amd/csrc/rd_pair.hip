@@ -36,19 +36,23 @@ namespace rd {
 
 // ------------------------------------------------------------------------
 
-// Per-block view of a launch-table entry.  Returns false (for the whole
-// block) when the block lies beyond the slot's live particles.
-struct View {
+// Workgroup of a compact pair launch: the launch-table entry it belongs to
+// (workgroups [wg0, next wg0) of the 1-D grid), its target block bx and its
+// source segment by.  The host lays the entry out as gx target blocks by
+// segs segments with gx = max(1, ceil(targets / kBlock)).
+struct Tile {
   Ent e;
-  int n;
-  int nchunks;
+  int bx, by;
 };
-__device__ __forceinline__ bool view(const Ent* tab, const SimState* st, View& v, int targets) {
-  v.e = tab[blockIdx.z];
-  v.n = st[v.e.slot].n;
-  v.nchunks = (v.n + kChunk - 1) / kChunk;
-  if ((int)(blockIdx.x * kBlock) >= targets) return false;
-  return true;
+__device__ __forceinline__ Tile tile_of(const Ent* tab, int ne, bool points) {
+  Tile t;
+  t.e = tab[find_entry(tab, ne, blockIdx.x)];
+  const int targets = points ? t.e.m : t.e.n_ub;
+  const int gx = max(1, (targets + kBlock - 1) / kBlock);
+  const int local = blockIdx.x - t.e.wg0;
+  t.bx = local % gx;
+  t.by = local / gx;
+  return t;
 }
 
 // Image points of every particle (they depend on the particle alone).
@@ -73,7 +77,7 @@ __global__ void k_tip_images64(const Ent* tab, const SimDesc* desc, const SimSta
 // ------------------------------------------------------------------------
 // FP64 accelerations.  part layout per entry: [3][nchunks_ub][n_ub].
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, const SimDesc* desc,
+__global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, const SimDesc* desc,
                                                     const SimState* st,
                                                     const double* __restrict__ X,
                                                     const double* __restrict__ Y,
@@ -84,21 +88,23 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, const SimDes
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
   __shared__ double sd[KIND == RD_GEOM_TIP ? kChunk : 1], sxi[KIND == RD_GEOM_TIP ? kChunk : 1],
       syi[KIND == RD_GEOM_TIP ? kChunk : 1], szi[KIND == RD_GEOM_TIP ? kChunk : 1];
-  View v;
-  if (!view(tab, st, v, st[tab[blockIdx.z].slot].n)) return;
-  const int c0 = blockIdx.y * v.e.cps;
-  if (c0 >= v.nchunks) return;
-  const int c1 = min(v.nchunks, c0 + v.e.cps);
-  const DevGeom g = desc[v.e.slot].g;
+  const Tile tl = tile_of(tab, ne, false);
+  const int n = st[tl.e.slot].n;
+  if (tl.bx * kBlock >= n) return;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int c0 = tl.by * tl.e.cps;
+  if (c0 >= nchunks) return;
+  const int c1 = min(nchunks, c0 + tl.e.cps);
+  const DevGeom g = desc[tl.e.slot].g;
   if (g.kind != KIND) return;
-  const int n = v.n, base = v.e.base;
+  const int base = tl.e.base;
   const double* x = X + base;
   const double* y = Y + base;
   const double* z = Z + base;
   const double* q = Q + base;
-  double* P = part + v.e.part_off;
+  double* P = part + tl.e.part_off;
   const int t = threadIdx.x;
-  const int i = blockIdx.x * kBlock + t;
+  const int i = tl.bx * kBlock + t;
   const bool act = i < n;
   double x1 = 0, y1 = 0, z1 = 0, qd1 = 0, d1 = 0, xi1 = 0, yi1 = 0, zi1 = 0;
   if (act) {
@@ -182,10 +188,10 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, const SimDes
         py = py + ayy;
         pz = pz + azz;
       }
-      const size_t S = (size_t)v.e.nchunks_ub * v.e.n_ub;
-      P[0 * S + (size_t)c * v.e.n_ub + i] = px;
-      P[1 * S + (size_t)c * v.e.n_ub + i] = py;
-      P[2 * S + (size_t)c * v.e.n_ub + i] = pz;
+      const size_t S = (size_t)tl.e.nchunks_ub * tl.e.n_ub;
+      P[0 * S + (size_t)c * tl.e.n_ub + i] = px;
+      P[1 * S + (size_t)c * tl.e.n_ub + i] = py;
+      P[2 * S + (size_t)c * tl.e.n_ub + i] = pz;
     }
   }
 }
@@ -231,7 +237,7 @@ __global__ void k_accel_finish64(const Ent* tab, const SimDesc* desc, const SimS
 // FP64 field at points.  Points of an entry: [pbase, pbase + m) of the SoA
 // point arrays; part layout per entry: [3][nchunks_ub][m].
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_field64(const Ent* tab, const SimDesc* desc,
+__global__ __launch_bounds__(kBlock) void k_field64(const Ent* tab, int ne, const SimDesc* desc,
                                                     const SimState* st,
                                                     const double* __restrict__ ptx,
                                                     const double* __restrict__ pty,
@@ -242,17 +248,18 @@ __global__ __launch_bounds__(kBlock) void k_field64(const Ent* tab, const SimDes
                                                     const double* __restrict__ Q,
                                                     double* __restrict__ part) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
-  const Ent e = tab[blockIdx.z];
+  const Tile tl = tile_of(tab, ne, true);
+  const Ent& e = tl.e;
   const int m = e.m;
-  if ((int)(blockIdx.x * kBlock) >= m) return;
+  if (tl.bx * kBlock >= m) return;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  const int c0 = blockIdx.y * e.cps;
+  const int c0 = tl.by * e.cps;
   if (c0 >= nchunks) return;
   const int c1 = min(nchunks, c0 + e.cps);
   const DevGeom g = desc[e.slot].g;
   if (g.kind != KIND) return;
-  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const int p = tl.bx * kBlock + threadIdx.x;
   const bool act = p < m;
   double x1 = 0, y1 = 0, z1 = 0;
   if (act) {
@@ -315,17 +322,18 @@ __global__ void k_pack_points(const Ent* tab, const SimDesc* desc, const double*
 
 // Mixed accelerations (planar).  part layout per entry: [3][segs][n_ub] FP64
 // sums of q_j/q_0 * (offset / r^3 + image terms) in nm^-2.
-__global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, const SimDesc* desc,
+__global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int ne, const SimDesc* desc,
                                                            const SimState* st,
                                                            const float4* __restrict__ HI,
                                                            const float4* __restrict__ LO,
                                                            double* __restrict__ part) {
   __shared__ float4 shi[kChunk], slo[kChunk];
-  const Ent e = tab[blockIdx.z];
+  const Tile tl = tile_of(tab, ne, false);
+  const Ent& e = tl.e;
   const int n = st[e.slot].n;
-  if ((int)(blockIdx.x * kBlock) >= n) return;
+  if (tl.bx * kBlock >= n) return;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  const int c0 = blockIdx.y * e.cps;
+  const int c0 = tl.by * e.cps;
   if (c0 >= nchunks) return;
   const int c1 = min(nchunks, c0 + e.cps);
   const DevGeom& gg = desc[e.slot].g;
@@ -335,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, const
   const float4* hi = HI + e.base;
   const float4* lo = LO + e.base;
   const int t = threadIdx.x;
-  const int i = blockIdx.x * kBlock + t;
+  const int i = tl.bx * kBlock + t;
   const bool act = i < n;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
   if (act) {
@@ -383,9 +391,9 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, const
   if (act) {
     double* P = part + e.part_off;
     const size_t S = (size_t)e.segs * e.n_ub;
-    P[0 * S + (size_t)blockIdx.y * e.n_ub + i] = fx;
-    P[1 * S + (size_t)blockIdx.y * e.n_ub + i] = fy;
-    P[2 * S + (size_t)blockIdx.y * e.n_ub + i] = fz;
+    P[0 * S + (size_t)tl.by * e.n_ub + i] = fx;
+    P[1 * S + (size_t)tl.by * e.n_ub + i] = fy;
+    P[2 * S + (size_t)tl.by * e.n_ub + i] = fz;
   }
 }
 
@@ -448,7 +456,7 @@ __global__ void k_pack_images(const Ent* tab, const SimDesc* desc, const SimStat
 // Mixed accelerations (tip).  Same layout as the planar kernel; a source
 // with j > i uses the target's sphere image, j < i the source's (the lower
 // index owns the image, as in Calculate_Acceleration_Particles_ACC).
-__global__ __launch_bounds__(kBlock) void k_accel_tip32(const Ent* tab, const SimDesc* desc,
+__global__ __launch_bounds__(kBlock) void k_accel_tip32(const Ent* tab, int ne, const SimDesc* desc,
                                                         const SimState* st,
                                                         const float4* __restrict__ HI,
                                                         const float4* __restrict__ LO,
@@ -456,11 +464,12 @@ __global__ __launch_bounds__(kBlock) void k_accel_tip32(const Ent* tab, const Si
                                                         const float4* __restrict__ ILO,
                                                         double* __restrict__ part) {
   __shared__ float4 shi[kChunk], slo[kChunk], sih[kChunk], sil[kChunk];
-  const Ent e = tab[blockIdx.z];
+  const Tile tl = tile_of(tab, ne, false);
+  const Ent& e = tl.e;
   const int n = st[e.slot].n;
-  if ((int)(blockIdx.x * kBlock) >= n) return;
+  if (tl.bx * kBlock >= n) return;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  const int c0 = blockIdx.y * e.cps;
+  const int c0 = tl.by * e.cps;
   if (c0 >= nchunks) return;
   const int c1 = min(nchunks, c0 + e.cps);
   const DevGeom& gg = desc[e.slot].g;
@@ -472,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_accel_tip32(const Ent* tab, const Si
   const float4* ih = IHI + e.base;
   const float4* il = ILO + e.base;
   const int t = threadIdx.x;
-  const int i = blockIdx.x * kBlock + t;
+  const int i = tl.bx * kBlock + t;
   const bool act = i < n;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = h1, ih1 = make_float4(0, 0, 0, 1), il1 = h1;
   if (act) {
@@ -512,14 +521,14 @@ __global__ __launch_bounds__(kBlock) void k_accel_tip32(const Ent* tab, const Si
   if (act) {
     double* P = part + e.part_off;
     const size_t S = (size_t)e.segs * e.n_ub;
-    P[0 * S + (size_t)blockIdx.y * e.n_ub + i] = fx;
-    P[1 * S + (size_t)blockIdx.y * e.n_ub + i] = fy;
-    P[2 * S + (size_t)blockIdx.y * e.n_ub + i] = fz;
+    P[0 * S + (size_t)tl.by * e.n_ub + i] = fx;
+    P[1 * S + (size_t)tl.by * e.n_ub + i] = fy;
+    P[2 * S + (size_t)tl.by * e.n_ub + i] = fz;
   }
 }
 
 // Mixed field at points (tip): the image of every point is its own.
-__global__ __launch_bounds__(kBlock) void k_field_tip32(const Ent* tab, const SimDesc* desc,
+__global__ __launch_bounds__(kBlock) void k_field_tip32(const Ent* tab, int ne, const SimDesc* desc,
                                                         const SimState* st,
                                                         const double* __restrict__ ptx,
                                                         const double* __restrict__ pty,
@@ -528,17 +537,18 @@ __global__ __launch_bounds__(kBlock) void k_field_tip32(const Ent* tab, const Si
                                                         const float4* __restrict__ LO,
                                                         double* __restrict__ part) {
   __shared__ float4 shi[kChunk], slo[kChunk];
-  const Ent e = tab[blockIdx.z];
+  const Tile tl = tile_of(tab, ne, true);
+  const Ent& e = tl.e;
   const int m = e.m;
-  if ((int)(blockIdx.x * kBlock) >= m) return;
+  if (tl.bx * kBlock >= m) return;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  const int c0 = blockIdx.y * e.cps;
+  const int c0 = tl.by * e.cps;
   if (c0 >= max(nchunks, 1)) return;
   const int c1 = min(nchunks, c0 + e.cps);
   const DevGeom& g = desc[e.slot].g;
   if (g.kind != RD_GEOM_TIP) return;
-  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const int p = tl.bx * kBlock + threadIdx.x;
   const bool act = p < m;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = h1, ih1 = make_float4(0, 0, 0, 1), il1 = h1;
   if (act) {
@@ -552,14 +562,14 @@ __global__ __launch_bounds__(kBlock) void k_field_tip32(const Ent* tab, const Si
   if (act) {
     double* P = part + e.part_off;
     const size_t S = (size_t)e.segs * m;
-    P[0 * S + (size_t)blockIdx.y * m + p] = fx;
-    P[1 * S + (size_t)blockIdx.y * m + p] = fy;
-    P[2 * S + (size_t)blockIdx.y * m + p] = fz;
+    P[0 * S + (size_t)tl.by * m + p] = fx;
+    P[1 * S + (size_t)tl.by * m + p] = fy;
+    P[2 * S + (size_t)tl.by * m + p] = fz;
   }
 }
 
 // Mixed field at points (planar).
-__global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, const SimDesc* desc,
+__global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, int ne, const SimDesc* desc,
                                                            const SimState* st,
                                                            const float4* __restrict__ PHI,
                                                            const float4* __restrict__ PLO,
@@ -567,17 +577,18 @@ __global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, const
                                                            const float4* __restrict__ LO,
                                                            double* __restrict__ part) {
   __shared__ float4 shi[kChunk], slo[kChunk];
-  const Ent e = tab[blockIdx.z];
+  const Tile tl = tile_of(tab, ne, true);
+  const Ent& e = tl.e;
   const int m = e.m;
-  if ((int)(blockIdx.x * kBlock) >= m) return;
+  if (tl.bx * kBlock >= m) return;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  const int c0 = blockIdx.y * e.cps;
+  const int c0 = tl.by * e.cps;
   if (c0 >= max(nchunks, 1)) return;
   const int c1 = min(nchunks, c0 + e.cps);
   const DevGeom& g = desc[e.slot].g;
   if (g.kind != RD_GEOM_PLANAR) return;
-  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const int p = tl.bx * kBlock + threadIdx.x;
   const bool act = p < m;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
   if (act) {
@@ -589,9 +600,9 @@ __global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, const
   if (act) {
     double* P = part + e.part_off;
     const size_t S = (size_t)e.segs * m;
-    P[0 * S + (size_t)blockIdx.y * m + p] = fx;
-    P[1 * S + (size_t)blockIdx.y * m + p] = fy;
-    P[2 * S + (size_t)blockIdx.y * m + p] = fz;
+    P[0 * S + (size_t)tl.by * m + p] = fx;
+    P[1 * S + (size_t)tl.by * m + p] = fy;
+    P[2 * S + (size_t)tl.by * m + p] = fz;
   }
 }
 
@@ -695,14 +706,16 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
     return fail(RD_ERR_ARG, "unknown precision");
   long long off = 0;
-  int maxsegs = 1;
+  int nwg = 0;  // workgroups of the compact pair launch
   for (Ent& e : ents) {
     int nch = (e.n_ub + kChunk - 1) / kChunk;
+    const int gx = std::max(1, (e.n_ub + kBlock - 1) / kBlock);
     e.nchunks_ub = nch;
-    segment(nch, seg_limit((e.n_ub + kBlock - 1) / kBlock, kWantPair), e.segs, e.cps);
-    maxsegs = std::max(maxsegs, e.segs);
+    segment(nch, seg_limit(gx, kWantPair), e.segs, e.cps);
     e.part_off = off;
     off += 3LL * (precision == RD_PREC_FP64 ? nch : e.segs) * e.n_ub;
+    e.wg0 = nwg;
+    nwg += gx * e.segs;
   }
   int r = ensure(c, c->part, sizeof(double) * (size_t)off);
   if (r) return r;
@@ -710,8 +723,7 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   r = upload_tab(c, ents, &dt);
   if (r) return r;
   const int ne = (int)ents.size();
-  const int tb = (maxn + kBlock - 1) / kBlock;
-  dim3 grid(tb, maxsegs, ne);
+  const int grid = nwg;
   dim3 fin((maxn + 255) / 256, 1, ne);
   double* part = (double*)c->part.p;
   if (precision == RD_PREC_FP64) {
@@ -724,10 +736,10 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
     if (k0) RD_HIP(hipEventRecord(k0, c->stream));
     if (any_planar)
       k_accel64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(
-          dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
+          dt, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
     if (any_tip)
       k_accel64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(
-          dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
+          dt, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
     if (k1) RD_HIP(hipEventRecord(k1, c->stream));
     k_accel_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->x, c->y,
                                                   c->z, c->q, c->m, c->ax, c->ay, c->az);
@@ -749,12 +761,12 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   }
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
   if (any_planar)
-    k_accel_planar32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+    k_accel_planar32<<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
                                                       (const float4*)c->pack_hi.p,
                                                       (const float4*)c->pack_lo.p, part);
   if (any_tip)
     k_accel_tip32<<<grid, kBlock, 0, c->stream>>>(
-        dt, c->d_desc, c->d_state, (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p,
+        dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p,
         (const float4*)c->img_hi.p, (const float4*)c->img_lo.p, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
   k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->x, c->y,
@@ -781,15 +793,17 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
     return fail(RD_ERR_ARG, "unknown precision");
   long long off = 0;
-  int maxsegs = 1, maxn = 0;
+  int maxn = 0, nwg = 0;  // nwg: workgroups of the compact field launch
   for (Ent& e : ents) {
     int nch = (e.n_ub + kChunk - 1) / kChunk;
+    const int gx = std::max(1, (e.m + kBlock - 1) / kBlock);
     e.nchunks_ub = nch;
-    segment(nch, seg_limit((e.m + kBlock - 1) / kBlock, kWantPair), e.segs, e.cps);
-    maxsegs = std::max(maxsegs, e.segs);
+    segment(nch, seg_limit(gx, kWantPair), e.segs, e.cps);
     maxn = std::max(maxn, e.n_ub);
     e.part_off = off;
     off += 3LL * (precision == RD_PREC_FP64 ? std::max(nch, 1) : e.segs) * e.m;
+    e.wg0 = nwg;
+    nwg += gx * e.segs;
   }
   int r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
   if (r) return r;
@@ -804,17 +818,17 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   double* fy = fx + mtot;
   double* fz = fy + mtot;
   double* part = (double*)c->part.p;
-  dim3 grid((maxm + kBlock - 1) / kBlock, maxsegs, ne);
+  const int grid = nwg;
   dim3 fin((maxm + 255) / 256, 1, ne);
   if (precision == RD_PREC_FP64) {
     if (k0) RD_HIP(hipEventRecord(k0, c->stream));
     if (maxn > 0) {
       if (any_planar)
         k_field64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(
-            dt, c->d_desc, c->d_state, ptx, pty, ptz, c->x, c->y, c->z, c->q, part);
+            dt, ne, c->d_desc, c->d_state, ptx, pty, ptz, c->x, c->y, c->z, c->q, part);
       if (any_tip)
         k_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(
-            dt, c->d_desc, c->d_state, ptx, pty, ptz, c->x, c->y, c->z, c->q, part);
+            dt, ne, c->d_desc, c->d_state, ptx, pty, ptz, c->x, c->y, c->z, c->q, part);
     }
     if (k1) RD_HIP(hipEventRecord(k1, c->stream));
     k_field_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, ptx, pty, ptz,
@@ -836,10 +850,10 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
   if (any_planar)
     k_field_planar32<<<grid, kBlock, 0, c->stream>>>(
-        dt, c->d_desc, c->d_state, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p,
+        dt, ne, c->d_desc, c->d_state, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p,
         (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p, part);
   if (any_tip)
-    k_field_tip32<<<grid, kBlock, 0, c->stream>>>(dt, c->d_desc, c->d_state, ptx, pty, ptz,
+    k_field_tip32<<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state, ptx, pty, ptz,
                                                    (const float4*)c->pack_hi.p,
                                                    (const float4*)c->pack_lo.p, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
