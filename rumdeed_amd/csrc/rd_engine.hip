@@ -744,11 +744,14 @@ __global__ __launch_bounds__(kBlock) void k_mh_field64(const Ent* tab, int ne,
                          (size_t)e.nchunks_ub * m, m, k);
 }
 
-// Mixed (planar): 64 chains per workgroup, one per lane; the four
-// wavefronts take a quarter of every 256-source chunk of the segment each
-// (sources staged in LDS, read as wavefront-wide broadcasts), and their FP64
-// lane sums are added in wavefront order.
+// Mixed: 64 chains per workgroup, one per lane; the kJumpWaves wavefronts
+// take an equal slice of every 256-source chunk of the segment each (sources
+// staged in LDS, read as wavefront-wide broadcasts), and their FP64 lane sums
+// are added in wavefront order.  Many short slices keep the per-wavefront
+// dependency chain short: a jump is latency-bound, not throughput-bound.
 constexpr int kJumpTargets = 64;
+constexpr int kJumpWaves = 16;
+constexpr int kJumpSlice = kChunk / kJumpWaves;
 
 __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, const float4 h2,
                                            const float4 l2, int ic, int nic, float two_d,
@@ -775,11 +778,12 @@ __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, con
 }
 
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, int ne,
-                                                       const SimDesc* desc, const SimState* st,
-                                                       Chains ch, int it, JumpArgs a) {
+__global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, int ne,
+                                                                const SimDesc* desc,
+                                                                const SimState* st, Chains ch,
+                                                                int it, JumpArgs a) {
   __shared__ float4 shi[kChunk], slo[kChunk];
-  __shared__ double red[3][4][kJumpTargets];
+  __shared__ double red[3][kJumpWaves][kJumpTargets];
   const Ent e = tab[find_entry(tab, ne, blockIdx.x)];
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != KIND || it > s.ndim) return;
@@ -816,12 +820,12 @@ __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, int ne,
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
-    if (jb + t < n) {
+    if (t < kChunk && jb + t < n) {
       shi[t] = hi[jb + t];
       slo[t] = lo[jb + t];
     }
     __syncthreads();
-    const int j0 = w * 64, j1 = min(min(kChunk, n - jb), j0 + 64);
+    const int j0 = w * kJumpSlice, j1 = min(min(kChunk, n - jb), j0 + kJumpSlice);
     if (act) {
       if (KIND == RD_GEOM_PLANAR) {
         for (int jj = j0; jj < j1; ++jj)
@@ -841,12 +845,12 @@ __global__ __launch_bounds__(kBlock) void k_mh_field32(const Ent* tab, int ne,
   red[1][w][lane] = fy;
   red[2][w][lane] = fz;
   __syncthreads();
-  if (w == 0 && act) {
+  if (w < 3 && act) {  // wavefront q adds component q over the wavefronts, in order
     double* part = a.part + e.part_off;
     const size_t S = (size_t)e.segs * m;
-    for (int q = 0; q < 3; ++q)
-      part[q * S + (size_t)by * m + k] =
-          ((red[q][0][lane] + red[q][1][lane]) + red[q][2][lane]) + red[q][3][lane];
+    double v = red[w][0][lane];
+    for (int u = 1; u < kJumpWaves; ++u) v += red[w][u][lane];
+    part[w * S + (size_t)by * m + k] = v;
   }
 }
 
@@ -1658,10 +1662,10 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
                                                                V, it, a);
     } else {
       if (any_planar)
-        k_mh_field32<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc,
+        k_mh_field32<RD_GEOM_PLANAR><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc,
                                                                       c->d_state, V, it, a);
       if (any_tip)
-        k_mh_field32<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
+        k_mh_field32<RD_GEOM_TIP><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
                                                                    V, it, a);
       k_mh_settle<RD_PREC_MIXED><<<ne, kSettle, 0, c->stream>>>(dt, c->d_desc, c->d_state,
                                                                 c->d_wf, V, it, a);
