@@ -777,6 +777,17 @@ __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, con
   fz += (double)cz;
 }
 
+// Layout of a simulation's chains in the mixed jump field, chosen from its
+// own chain count only: with few chains one wavefront per chain (its lanes
+// stride over the sources), otherwise one lane per chain (the wavefronts
+// take slices of every chunk).
+constexpr int kWaveChains = 32;  // at most this many chains: a wavefront per chain
+__host__ __device__ inline bool jump_by_wave(int m) { return m <= kWaveChains; }
+__host__ __device__ inline int jump_groups32(int m) {
+  return jump_by_wave(m) ? (m + kJumpWaves - 1) / kJumpWaves
+                         : (m + kJumpTargets - 1) / kJumpTargets;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, int ne,
                                                                 const SimDesc* desc,
@@ -788,19 +799,21 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != KIND || it > s.ndim) return;
   const int m = e.nch;
-  const int gx = (m + kJumpTargets - 1) / kJumpTargets;
+  const bool by_wave = jump_by_wave(m);
+  const int gx = jump_groups32(m);
   const int local = blockIdx.x - e.wg0;
   const int bx = local % gx, by = local / gx;  // chain group, source segment
   const DevGeom& g = s.g;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int k = bx * kJumpTargets + lane;
+  // the chain of this lane (uniform across a wavefront when by_wave)
+  const int k = by_wave ? bx * kJumpWaves + w : bx * kJumpTargets + lane;
   const bool act = k < m;
   Proposal P{};
   if (act) {
     P = jump_proposal(e, s, st, ch, it, k, a);
-    if (by == 0 && w == 0) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
+    if (by == 0 && (by_wave ? lane == 0 : w == 0)) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
   }
   const int c0 = by * e.cps;
   const int c1 = min(nchunks, c0 + e.cps);
@@ -825,13 +838,18 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
       slo[t] = lo[jb + t];
     }
     __syncthreads();
-    const int j0 = w * kJumpSlice, j1 = min(min(kChunk, n - jb), j0 + kJumpSlice);
+    const int cnt = min(kChunk, n - jb);
+    // sources of this lane in the chunk: a slice per wavefront, or a stride
+    // of 64 over the chunk per lane
+    const int j0 = by_wave ? lane : w * kJumpSlice;
+    const int j1 = by_wave ? cnt : min(cnt, j0 + kJumpSlice);
+    const int dj = by_wave ? 64 : 1;
     if (act) {
       if (KIND == RD_GEOM_PLANAR) {
-        for (int jj = j0; jj < j1; ++jj)
+        for (int jj = j0; jj < j1; jj += dj)
           mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, fx, fy, fz);
       } else {
-        for (int jj = j0; jj < j1; ++jj) {
+        for (int jj = j0; jj < j1; jj += dj) {
           float cx, cy, cz;
           tip_pair32_target(h1, l1, ih1, il1, shi[jj], slo[jj], kt, Rn, cx, cy, cz);
           fx += (double)cx;
@@ -845,9 +863,17 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
   red[1][w][lane] = fy;
   red[2][w][lane] = fz;
   __syncthreads();
-  if (w < 3 && act) {  // wavefront q adds component q over the wavefronts, in order
-    double* part = a.part + e.part_off;
-    const size_t S = (size_t)e.segs * m;
+  double* part = a.part + e.part_off;
+  const size_t S = (size_t)e.segs * m;
+  if (by_wave) {
+    // lane q of wavefront w adds component q over the 64 lanes, in lane order
+    if (lane < 3 && act) {
+      double v = red[lane][w][0];
+      for (int u = 1; u < 64; ++u) v += red[lane][w][u];
+      part[lane * S + (size_t)by * m + k] = v;
+    }
+  } else if (w < 3 && act) {
+    // wavefront q adds component q over the wavefronts, in wavefront order
     double v = red[w][0][lane];
     for (int u = 1; u < kJumpWaves; ++u) v += red[w][u][lane];
     part[w * S + (size_t)by * m + k] = v;
@@ -1622,7 +1648,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   int nwg = 0;  // workgroups of the compact field launch
   for (Ent& e : ents) {
     const int nch = (e.n_ub + kChunk - 1) / kChunk;
-    const int gx = (e.nch + per - 1) / per;
+    const int gx = fp64 ? (e.nch + per - 1) / per : jump_groups32(e.nch);
     e.nchunks_ub = nch;
     segment(nch, seg_limit(gx, kWantJump), e.segs, e.cps);
     e.part_off = off;
