@@ -99,6 +99,9 @@ def stream_requests(n: int = 256, rate: float = 4.0, seed: int = 2024, steps: in
     return reqs
 
 
+MH_JUMPS = 200  # Metropolis-Hastings jumps per chain of the planar FE sampler
+
+
 class Server:
     def __init__(self, total_particles: int = 1 << 22, max_slots: int = 512,
                  hbm_budget: int = 64 << 30, precision: str = "mixed", device: int = 0):
@@ -130,6 +133,8 @@ class Server:
         t0 = time.perf_counter()
         loop_steps = 0
         occupancy = []
+        mh_pairs = 0
+        force_pairs = 0
         while pending or active:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
@@ -148,7 +153,7 @@ class Server:
                 res = Result(rid=r.rid, arrival=r.arrival if realtime else 0.0, admitted=now,
                              finished=0.0,
                              steps=r.steps, capacity=cap)
-                active[slot] = [r, res, fp, 0]
+                active[slot] = [r, res, fp, 0, 0]  # ..., steps done, electrons in flight
             if not active:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - now)))
@@ -158,10 +163,17 @@ class Server:
             occupancy.append(len(active))
             now = time.perf_counter() - t0
             for slot in list(active):
-                r, res, fp, done = active[slot]
+                r, res, fp, done, n_prev = active[slot]
                 rec = recs[slot]
                 done += 1
                 active[slot][3] = done
+                active[slot][4] = rec.n
+                # work of the step: every chain's field on each of the 200
+                # jumps sees the electrons of the start of the step; the
+                # forces see those plus the new ones
+                mh_pairs += rec.n_round * n_prev * MH_JUMPS
+                n_mid = n_prev + rec.emitted
+                force_pairs += n_mid * max(n_mid - 1, 0)
                 res.emitted += rec.emitted
                 res.absorbed += rec.absorbed
                 res.dropped = rec.dropped
@@ -191,5 +203,8 @@ class Server:
             "pool_steps": loop_steps,
             "mean_active": float(np.mean(occupancy)) if occupancy else 0.0,
             "max_active": int(max(occupancy)) if occupancy else 0,
+            "mh_field_pair_evals": mh_pairs,
+            "force_pair_evals": force_pairs,
+            "pair_evals_per_s": (mh_pairs + force_pairs) / makespan if makespan > 0 else 0.0,
         }
         return sorted(results, key=lambda r: r.rid), stats
