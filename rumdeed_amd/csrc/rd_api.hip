@@ -376,7 +376,7 @@ int rd_ctx_destroy(rd_ctx* c) {
   if (c->h_state) hipHostFree(c->h_state);
   if (c->h_tab) hipHostFree(c->h_tab);
   Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
-                 &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo};
+                 &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo, &c->cnt};
   for (Buf* b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);

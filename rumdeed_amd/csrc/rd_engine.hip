@@ -568,19 +568,38 @@ __device__ __forceinline__ void mh_record(const Ent& e, const SimDesc& s, const 
   ptz[p] = P.z;
 }
 
-// Accept/reject of jump `it` for every chain of entry e, by one block (a
-// power of two of lanes; ra, rr: LDS arrays of blockDim.x entries for the
-// batch acceptance counts; lr: the log-uniforms of the jump, by chain).
+// The planar FE width update of one jump from the batch counts
+// (MH_std_update, src/mod_field_emission_v2.F90:603-612).
+__device__ __forceinline__ void fe_width_update(SimState& S, int it, int first, int na, int nr) {
+  S.it_a = na;
+  S.it_r = nr;
+  if ((it > first) && (na + nr > 0)) {
+    S.a_rate = (double)na / (double)(na + nr);
+    S.MH_std = S.MH_std * rd_exp(0.025 * (S.a_rate - 0.35));
+    if (S.MH_std > 0.1250)
+      S.MH_std = 0.1250;
+    else if (S.MH_std < 0.00005)
+      S.MH_std = 0.00005;
+  }
+}
+
+// Accept/reject of jump `it` for the chains [k_begin, k_end) of entry e, by
+// one block (a power of two of lanes; ra, rr: LDS arrays of blockDim.x
+// entries for the batch acceptance counts; lr: the log-uniforms of the jump,
+// by chain).  Planar FE: with counts == nullptr the block holds every chain
+// and updates the width itself; otherwise it stores its (accepted, rejected)
+// counts there for k_mh_width.
 __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, const double* wf,
                                 const Chains& ch, int it, const double* ptx, const double* pty,
                                 const double* ptz, const double* fx, const double* fy,
-                                const double* fz, const double* lr, int* ra, int* rr) {
+                                const double* fz, const double* lr, int* ra, int* rr,
+                                int k_begin, int k_end, int* counts) {
   const EmitConsts ec = emit_consts();
   const int first = nint(s.ndim * 0.25);
   const bool tip = s.mode == RD_EMIT_TIP_FE;
   const bool counted = (s.mode == RD_EMIT_PLANAR_GTF) || (it > first);
   int na = 0, nrj = 0;
-  for (int k = threadIdx.x; k < e.nch; k += blockDim.x) {
+  for (int k = k_begin + threadIdx.x; k < k_end; k += blockDim.x) {
     const int c = e.cbase + k;
     if (ch.ok[c] != 1) continue;
     const int p = e.pbase + k;
@@ -638,17 +657,11 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    SimState& S = st[e.slot];
-    S.it_a = ra[0];
-    S.it_r = rr[0];
-    if ((it > first) && (ra[0] + rr[0] > 0)) {
-      // MH_std_update (src/mod_field_emission_v2.F90:603-612)
-      S.a_rate = (double)ra[0] / (double)(ra[0] + rr[0]);
-      S.MH_std = S.MH_std * rd_exp(0.025 * (S.a_rate - 0.35));
-      if (S.MH_std > 0.1250)
-        S.MH_std = 0.1250;
-      else if (S.MH_std < 0.00005)
-        S.MH_std = 0.00005;
+    if (counts) {
+      counts[0] = ra[0];
+      counts[1] = rr[0];
+    } else {
+      fe_width_update(st[e.slot], it, first, ra[0], rr[0]);
     }
   }
 }
@@ -660,9 +673,11 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
 // chains (the proposals are counter-based, so every workgroup of a chain
 // computes the same one; the first source segment's workgroup records it)
 // and writes the field partial sums of one source segment.  k_mh_settle: one
-// workgroup per simulation adds the partials in a fixed order and runs the
-// accept step for all of the simulation's chains.  The kernel boundary is
-// the only synchronisation: no counters, no device-wide fences.
+// workgroup per kSettle chains of a simulation adds the partials in a fixed
+// order and runs the accept step for those chains; k_mh_width then updates
+// the planar FE width of simulations whose chains took several workgroups.
+// Kernel boundaries are the only synchronisation: no counters, no
+// device-wide fences.
 struct JumpArgs {
   double *ptx, *pty, *ptz, *fx, *fy, *fz;
   double* part;
@@ -883,21 +898,27 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
 // Finish and accept: one workgroup per simulation.  FP64 adds the chunk
 // partials of every chain in chunk order (plus the vacuum field); mixed adds
 // the segment partials in segment order.
-constexpr int kSettle = 1024;  // lanes of the per-simulation finish and accept
+constexpr int kSettle = 1024;  // lanes of a finish-and-accept workgroup
+
+// Workgroups of a simulation's finish-and-accept: one per kSettle chains.
+__host__ __device__ inline int settle_groups(int m) { return max(1, (m + kSettle - 1) / kSettle); }
 
 template <int PREC>
-__global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, const SimDesc* desc,
-                                                       SimState* st, const double* wf, Chains ch,
-                                                       int it, JumpArgs a) {
+__global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, int ne,
+                                                       const SimDesc* desc, SimState* st,
+                                                       const double* wf, Chains ch, int it,
+                                                       JumpArgs a, int* counts) {
   __shared__ int ra[kSettle], rr[kSettle];
-  const Ent e = tab[blockIdx.x];
+  const Ent e = tab[find_entry(tab, ne, blockIdx.x, true)];
   const SimDesc& s = desc[e.slot];
   if (it > s.ndim) return;
   const int m = e.nch;
+  const int grp = blockIdx.x - e.wg1;
+  const int k0 = grp * kSettle, k1 = min(m, k0 + kSettle);
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const double* part = a.part + e.part_off;
-  for (int k = threadIdx.x; k < m; k += blockDim.x) {
+  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
     const int p = e.pbase + k;
     if (PREC == RD_PREC_FP64) {
       field_finish64(s.g, part, (size_t)e.nchunks_ub * m, m, k, nchunks, a.ptx[p], a.pty[p],
@@ -913,7 +934,25 @@ __global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, const Sim
   }
   __syncthreads();
   mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, a.fx, a.fy, a.fz,
-                  a.rlr + (size_t)(it - 1) * a.C, ra, rr);
+                  a.rlr + (size_t)(it - 1) * a.C, ra, rr, k0, k1,
+                  settle_groups(m) > 1 ? counts + 2 * (size_t)blockIdx.x : nullptr);
+}
+
+// Planar FE simulations whose accept step took several workgroups: the batch
+// counts summed over them (integers, so in any order) and the width update.
+__global__ void k_mh_width(const Ent* tab, const SimDesc* desc, SimState* st, int it,
+                           const int* __restrict__ counts) {
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  if (threadIdx.x != 0 || it > s.ndim || s.mode != RD_EMIT_PLANAR_FE) return;
+  const int gs = settle_groups(e.nch);
+  if (gs == 1) return;
+  int na = 0, nr = 0;
+  for (int g = 0; g < gs; ++g) {
+    na += counts[2 * (e.wg1 + g)];
+    nr += counts[2 * (e.wg1 + g) + 1];
+  }
+  fe_width_update(st[e.slot], it, nint(s.ndim * 0.25), na, nr);
 }
 
 // Per-chain width updates (thermal-field, tip) in chain order and the escape
@@ -1607,8 +1646,9 @@ static int reset_counters(rd_ctx* c, const std::vector<Ent>& ents) {
 }
 
 // The Metropolis-Hastings jumps 1..maxndim of the chain entries `in` (C
-// chains in all: the stride of the point and field arrays), one fused launch
-// per jump (per geometry for FP64).
+// chains in all: the stride of the point and field arrays): per jump, one
+// field launch per geometry present, then the finish-and-accept launch and,
+// for planar FE simulations with more than kSettle chains, the width update.
 static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision,
                      const Chains& V, int maxndim) {
   const bool fp64 = precision == RD_PREC_FP64;
@@ -1645,8 +1685,14 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   a.HI = (const float4*)c->pack_hi.p;
   a.LO = (const float4*)c->pack_lo.p;
   long long off = 0;
-  int nwg = 0;  // workgroups of the compact field launch
+  int nwg = 0;     // workgroups of the compact field launch
+  int nsettle = 0;  // workgroups of the finish-and-accept launch
+  bool any_split = false;
   for (Ent& e : ents) {
+    e.wg1 = nsettle;
+    nsettle += settle_groups(e.nch);
+    if (settle_groups(e.nch) > 1 && c->slots[e.slot].desc.mode == RD_EMIT_PLANAR_FE)
+      any_split = true;
     const int nch = (e.n_ub + kChunk - 1) / kChunk;
     const int gx = fp64 ? (e.nch + per - 1) / per : jump_groups32(e.nch);
     e.nchunks_ub = nch;
@@ -1662,6 +1708,9 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   const size_t nr = (size_t)maxndim * C;
   r = ensure(c, c->rnd, sizeof(double) * 3 * nr);
   if (r) return r;
+  r = ensure(c, c->cnt, sizeof(int) * 2 * (size_t)nsettle);
+  if (r) return r;
+  int* counts = (int*)c->cnt.p;
   Ent* dt = nullptr;
   r = upload_tab(c, ents, &dt);
   if (r) return r;
@@ -1684,8 +1733,8 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
       if (any_tip)
         k_mh_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
                                                                    V, it, a);
-      k_mh_settle<RD_PREC_FP64><<<ne, kSettle, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_wf,
-                                                               V, it, a);
+      k_mh_settle<RD_PREC_FP64><<<nsettle, kSettle, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a, counts);
     } else {
       if (any_planar)
         k_mh_field32<RD_GEOM_PLANAR><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc,
@@ -1693,9 +1742,11 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
       if (any_tip)
         k_mh_field32<RD_GEOM_TIP><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
                                                                    V, it, a);
-      k_mh_settle<RD_PREC_MIXED><<<ne, kSettle, 0, c->stream>>>(dt, c->d_desc, c->d_state,
-                                                                c->d_wf, V, it, a);
+      k_mh_settle<RD_PREC_MIXED><<<nsettle, kSettle, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a, counts);
     }
+    if (any_split)
+      k_mh_width<<<ne, 64, 0, c->stream>>>(dt, c->d_desc, c->d_state, it, counts);
   }
   RD_HIP(hipGetLastError());
   return RD_OK;

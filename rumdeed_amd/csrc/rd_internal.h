@@ -92,16 +92,16 @@ struct Ent {
   int pbase, m;   // field points [pbase, pbase + m)
   int cbase, nch; // MH chains [cbase, cbase + nch)
   int step, nblk; // time step; per-slot block count (reductions)
-  int wg0, pad;   // first workgroup of the entry in a compact launch
+  int wg0, wg1;   // first workgroup of the entry in a compact launch (two layouts)
 };
 
 // Entry of a compact launch whose workgroups [wg0, next wg0) belong to one
 // entry: the last entry with wg0 <= b (binary search, uniform per block).
-__device__ __forceinline__ int find_entry(const Ent* tab, int ne, int b) {
+__device__ __forceinline__ int find_entry(const Ent* tab, int ne, int b, bool second = false) {
   int lo = 0, hi = ne - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (tab[mid].wg0 <= b)
+    if ((second ? tab[mid].wg1 : tab[mid].wg0) <= b)
       lo = mid;
     else
       hi = mid - 1;
@@ -158,6 +158,7 @@ struct rd_ctx {
   // work buffers (grown on demand, never shrunk)
   rd::Buf part, pack_hi, pack_lo, pts, fld, pts_hi, pts_lo, stage, img64, tab, red, chains;
   rd::Buf rnd;  // random numbers of the Metropolis-Hastings jumps of a step
+  rd::Buf cnt;  // accept / reject counts per accept workgroup of a jump
   rd::Buf img_hi, img_lo;  // packed sphere images of the particles (mixed, tip)
   rd::Ent* h_tab = nullptr;  // pinned staging of launch tables (ring)
   int tab_used = 0;          // ring entries used since the last stream synchronisation
