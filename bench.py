@@ -280,7 +280,8 @@ def main():
 
     # dominant kernel: mixed all-pairs accelerations; HIP events on its stream
     tot_ms, ker_ms = dd.time_accel("mixed", reps=5)
-    flops = n * (n - 1) * W.pair_flops(nic)
+    flops = W.force_flops(n, nic, "mixed")
+    kernel = "k_accel_sym32" if n >= W.SYM_MIN else "k_accel_planar32"
     achieved = flops / (ker_ms * 1e-3) / 1e12
     ftot, fker = dd.time_field_grid(*box, G, G, 0.0, "mixed", reps=3)
     t64, k64 = dd.time_accel("fp64", reps=2)
@@ -307,7 +308,7 @@ def main():
                    "interactions_per_step": interactions},
         "fp64": {"value": interactions / (ms_fp64 * 1e-3), "ms_per_step": ms_fp64,
                  "accel_kernel_ms": k64,
-                 "achieved_tflops": flops / (k64 * 1e-3) / 1e12},
+                 "achieved_tflops": W.force_flops(n, nic, "fp64") / (k64 * 1e-3) / 1e12},
         "mixed_over_fp64": ms_fp64 / ms_mixed,
         "kernels_ms": {"accel_mixed": ker_ms, "accel_mixed_total": tot_ms,
                        "field_grid_mixed": fker, "field_grid_mixed_total": ftot,
@@ -315,11 +316,12 @@ def main():
         "hbm_copy_gbs": 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
-                     "traffic": MEASURED_TRAFFIC.get(("k_accel_planar32", n, nic)),
+                     "traffic": MEASURED_TRAFFIC.get((kernel, n, nic)),
                      "traffic_unit": "bytes per launch (HBM read + write, counter passes)",
-                     "kernel": "k_accel_planar32",
+                     "kernel": kernel,
                      "flops_per_launch": flops,
-                     "flops_per_interaction": W.pair_flops(nic)},
+                     "flops_per_interaction": W.pair_flops(nic),
+                     "pairs_per_launch": n * (n - 1) // (2 if kernel == "k_accel_sym32" else 1)},
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -768,13 +768,18 @@ constexpr int kJumpTargets = 64;
 constexpr int kJumpWaves = 16;
 constexpr int kJumpSlice = kChunk / kJumpWaves;
 
+// One mixed pair term; `surf`: the wavefront's targets lie on z = 0.
 __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, const float4 h2,
                                            const float4 l2, int ic, int nic, float two_d,
-                                           double& fx, double& fy, double& fz) {
+                                           bool surf, double& fx, double& fy, double& fz) {
   float dx = (h1.x - h2.x) + (l1.x - l2.x);
   float dy = (h1.y - h2.y) + (l1.y - l2.y);
-  float dz = (h1.z - h2.z) + (l1.z - l2.z);
   float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+  if (surf) {
+    fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, 0.5f * two_d));
+    return;
+  }
+  float dz = (h1.z - h2.z) + (l1.z - l2.z);
   float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
   float ri = rsq(r2);
   float r3 = ri * ri * ri;
@@ -845,6 +850,7 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
   const float two_d = (float)(2.0 * g.d * 1e9);
   const float kt = g.ic ? (float)g.ktip : 0.0f;
   const float Rn = (float)(g.R * 1e9);
+  const bool surf = KIND == RD_GEOM_PLANAR && wave_on_surface(ic, act, h1, l1);
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
@@ -862,7 +868,7 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
     if (act) {
       if (KIND == RD_GEOM_PLANAR) {
         for (int jj = j0; jj < j1; jj += dj)
-          mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, fx, fy, fz);
+          mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, surf, fx, fy, fz);
       } else {
         for (int jj = j0; jj < j1; jj += dj) {
           float cx, cy, cz;
@@ -895,9 +901,9 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
   }
 }
 
-// Finish and accept: one workgroup per simulation.  FP64 adds the chunk
-// partials of every chain in chunk order (plus the vacuum field); mixed adds
-// the segment partials in segment order.
+// Finish and accept: one workgroup per kSettle chains of a simulation.  FP64
+// adds the chunk partials of every chain in chunk order (plus the vacuum
+// field); mixed adds the segment partials in segment order.
 constexpr int kSettle = 1024;  // lanes of a finish-and-accept workgroup
 
 // Workgroups of a simulation's finish-and-accept: one per kSettle chains.

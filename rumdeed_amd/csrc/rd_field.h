@@ -199,6 +199,47 @@ __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, 
   wz = wzs - wzo;
 }
 
+// Direct plus image-charge weight of one pair whose target lies on the
+// cathode plane z = 0 (the emission surface: surface-field grids and
+// Metropolis-Hastings sampling points).  There every partner has a twin at
+// the same distance -- the n = 0 image with the source itself, the opposite
+// partner at 2nd - z_b with the same-charge partner at z_b - 2nd, the one at
+// -2nd - z_b with the one at 2nd + z_b -- whose lateral terms cancel and whose
+// normal terms are equal, so the pair adds
+//   cz = -2 sum_{k=-nic..nic} u_k / (dxy2 + u_k^2)^(3/2),  u_k = z_b + 2kd,
+//   cxy = 0,
+// the same series as ic_planar32 plus the direct term, with 2 nic + 1 distances
+// instead of 4 nic + 2.  zb = z_b, wb = d - z_b and dnm = d (nm); the partners
+// below the cathode are formed from wb, as ic_planar32 forms the anode image.
+__device__ __forceinline__ float surface_pair32(float zb, float wb, float dxy2, int nic,
+                                                float dnm) {
+  const float two_d = dnm + dnm;
+  float w = 0.f;
+  {
+    float r2 = fmaxf(__builtin_fmaf(zb, zb, dxy2), kR2Min);
+    float ri = rsq(r2);
+    w = zb * (ri * ri * ri);
+  }
+  float up = zb, um = -(wb + dnm);
+  for (int n = 1; n <= nic; ++n) {
+    up += two_d;
+    float r2 = __builtin_fmaf(up, up, dxy2);
+    float ri = rsq(r2);
+    w = __builtin_fmaf(up, ri * ri * ri, w);
+    r2 = __builtin_fmaf(um, um, dxy2);
+    ri = rsq(r2);
+    w = __builtin_fmaf(um, ri * ri * ri, w);
+    um -= two_d;
+  }
+  return -2.0f * w;
+}
+
+// True when every active target of the wavefront lies on the plane z = 0 and
+// the image series is on: the wavefront may then use surface_pair32.
+__device__ __forceinline__ bool wave_on_surface(int ic, bool act, float4 h1, float4 l1) {
+  return ic && __all(!act || (h1.z == 0.0f && l1.z == 0.0f));
+}
+
 // ------------------------------------------------------------------------
 // Field at one target per lane from the sources of chunks [c0, c1).  Every
 // lane of the workgroup takes part in the LDS staging whether or not it has
@@ -303,6 +344,8 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
                                             double& fz) {
   const int ic = g.ic, nic = g.nic;
   const float two_d = (float)(2.0 * g.d * 1e9);
+  const float dnm = (float)(g.d * 1e9);
+  const bool surf = wave_on_surface(ic, act, h1, l1);
   const int t = threadIdx.x;
   fx = 0.0;
   fy = 0.0;
@@ -316,7 +359,15 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
     }
     __syncthreads();
     const int cnt = min(kChunk, n - jb);
-    if (act) {
+    if (act && surf) {
+      for (int jj = 0; jj < cnt; ++jj) {
+        const float4 h2 = shi[jj], l2 = slo[jj];
+        float dx = (h1.x - h2.x) + (l1.x - l2.x);
+        float dy = (h1.y - h2.y) + (l1.y - l2.y);
+        float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+        fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, dnm));
+      }
+    } else if (act) {
       for (int jj = 0; jj < cnt; ++jj) {
         const float4 h2 = shi[jj], l2 = slo[jj];
         float dx = (h1.x - h2.x) + (l1.x - l2.x);

@@ -397,6 +397,167 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
   }
 }
 
+// ------------------------------------------------------------------------
+// Mixed accelerations (planar), pair-symmetric form for large simulations.
+//
+// Calculate_Acceleration_Particles_Planar (src/mod_verlet.F90:805-872)
+// evaluates every unordered pair once: the image series of the pair (i < j)
+// at z_i from j's partners, +F on i, and on j the Coulomb part reversed and
+// the image part with x and y reversed and z kept.  This kernel does the
+// same on the GPU.  Particles form blocks of kSymBlock; workgroup (X, Y),
+// X <= Y, takes the block pair.
+//   X < Y: wavefront w holds 64 targets of block X (one per lane) and, in
+//     phase p, the 64 sources of tile (w + p) mod kSymWaves of block Y.  Over
+//     64 steps lane l pairs its target with every source of the tile; the
+//     sources' FP64 reaction sums ride along the wavefront (rotated one lane
+//     per step), so every source's sum is added in a fixed order without
+//     atomics.  Each tile belongs to one wavefront per phase, and the phases
+//     are separated by barriers.
+//   X = Y: the block's pairs by gather, each lane over the whole block,
+//     with the reference's lower-index roles (as k_accel_planar32).
+// Partials: for a particle of block X, slot Y holds its sum from the block
+// pair (X, Y) for Y > X, from (Y, X) for Y < X, and the diagonal for Y = X:
+// the [3][segs][n_ub] layout of the gather kernels with segs = blocks, so
+// k_accel_finish32 adds them in block order.
+constexpr int kSymWaves = 16;
+constexpr int kSymBlock = 64 * kSymWaves;
+constexpr int kSymMin = 16384;
+
+// One lane to the next: lane l takes lane l + 1's value (wave_rol:1).
+__device__ __forceinline__ int wave_rotate(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x134, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double wave_rotate(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = wave_rotate((int)(unsigned)b);
+  const int hi = wave_rotate((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int ne,
+                                                           const SimDesc* desc, const SimState* st,
+                                                           const float4* __restrict__ HI,
+                                                           const float4* __restrict__ LO,
+                                                           double* __restrict__ part) {
+  __shared__ float4 shi[kSymBlock], slo[kSymBlock];
+  __shared__ double jsum[3][kSymBlock];
+  const Ent e = tab[find_entry(tab, ne, blockIdx.x, true)];
+  const int nb = e.segs;
+  int local = blockIdx.x - e.wg1;
+  int X = 0;
+  while (local >= nb - X) {
+    local -= nb - X;
+    ++X;
+  }
+  const int Y = X + local;
+  const int n = st[e.slot].n;
+  if (Y * kSymBlock >= n) return;  // X <= Y: both blocks hold particles
+  const DevGeom& gg = desc[e.slot].g;
+  const int ic = gg.ic, nic = gg.nic;
+  const float two_d = (float)(2.0 * gg.d * 1e9);
+  const float dnm = (float)(gg.d * 1e9);
+  const float4* hi = HI + e.base;
+  const float4* lo = LO + e.base;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // padding (beyond n): charge 0 at the origin, so every term it enters is 0
+  const float4 h0 = make_float4(0, 0, 0, 0), l0 = make_float4(0, 0, 0, dnm);
+  const int i = X * kSymBlock + t;
+  const float4 h1 = i < n ? hi[i] : h0, l1 = i < n ? lo[i] : l0;
+  const int js = Y * kSymBlock + t;
+  shi[t] = js < n ? hi[js] : h0;
+  slo[t] = js < n ? lo[js] : l0;
+  jsum[0][t] = 0.0;
+  jsum[1][t] = 0.0;
+  jsum[2][t] = 0.0;
+  __syncthreads();
+  double* P = part + e.part_off;
+  const size_t S = (size_t)e.segs * e.n_ub;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (X == Y) {
+    const int cnt = min(kSymBlock, n - Y * kSymBlock);
+    for (int jj = 0; jj < cnt; ++jj) {
+      const int j = Y * kSymBlock + jj;
+      const float4 h2 = shi[jj], l2 = slo[jj];
+      float dx = (h1.x - h2.x) + (l1.x - l2.x);
+      float dy = (h1.y - h2.y) + (l1.y - l2.y);
+      float dz = (h1.z - h2.z) + (l1.z - l2.z);
+      float qj = (j == i) ? 0.0f : h2.w;
+      float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+      float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
+      float ri = rsq(r2);
+      float r3 = ri * ri * ri;
+      float cxy = r3, cz = dz * r3;
+      if (ic) {
+        float wxy, wz;
+        float dzab = (j > i) ? dz : -dz;
+        ic_planar32(h1.z + h2.z, l1.w + l2.w, dzab, dxy2, nic, two_d, wxy, wz);
+        cxy += wxy;
+        cz += wz;
+      }
+      cxy *= qj;
+      cz *= qj;
+      fx += (double)(cxy * dx);
+      fy += (double)(cxy * dy);
+      fz += (double)cz;
+    }
+    if (i < n) {
+      P[0 * S + (size_t)X * e.n_ub + i] = fx;
+      P[1 * S + (size_t)X * e.n_ub + i] = fy;
+      P[2 * S + (size_t)X * e.n_ub + i] = fz;
+    }
+    return;
+  }
+  for (int ph = 0; ph < kSymWaves; ++ph) {
+    const int tb = ((w + ph) & (kSymWaves - 1)) * 64;  // the tile of this wavefront
+    int jl = lane;  // the source whose reaction sum this lane holds
+    double gx = jsum[0][tb + lane], gy = jsum[1][tb + lane], gz = jsum[2][tb + lane];
+    for (int step = 0; step < 64; ++step) {
+      const float4 h2 = shi[tb + jl], l2 = slo[tb + jl];
+      float dx = (h1.x - h2.x) + (l1.x - l2.x);
+      float dy = (h1.y - h2.y) + (l1.y - l2.y);
+      float dz = (h1.z - h2.z) + (l1.z - l2.z);
+      float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+      float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
+      float ri = rsq(r2);
+      float r3 = ri * ri * ri;
+      float cxy = r3, czi = dz * r3, czj = -czi;
+      if (ic) {
+        float wxy, wz;
+        ic_planar32(h1.z + h2.z, l1.w + l2.w, dz, dxy2, nic, two_d, wxy, wz);
+        cxy += wxy;
+        czi += wz;
+        czj += wz;
+      }
+      const float ci = cxy * h2.w, cj = cxy * h1.w;
+      fx += (double)(ci * dx);
+      fy += (double)(ci * dy);
+      fz += (double)(czi * h2.w);
+      gx -= (double)(cj * dx);
+      gy -= (double)(cj * dy);
+      gz += (double)(czj * h1.w);
+      gx = wave_rotate(gx);
+      gy = wave_rotate(gy);
+      gz = wave_rotate(gz);
+      jl = wave_rotate(jl);
+    }
+    // 64 rotations: every lane holds its own source's sum again
+    jsum[0][tb + lane] = gx;
+    jsum[1][tb + lane] = gy;
+    jsum[2][tb + lane] = gz;
+    __syncthreads();
+  }
+  if (i < n) {
+    P[0 * S + (size_t)Y * e.n_ub + i] = fx;
+    P[1 * S + (size_t)Y * e.n_ub + i] = fy;
+    P[2 * S + (size_t)Y * e.n_ub + i] = fz;
+  }
+  if (js < n) {
+    P[0 * S + (size_t)X * e.n_ub + js] = jsum[0][t];
+    P[1 * S + (size_t)X * e.n_ub + js] = jsum[1][t];
+    P[2 * S + (size_t)X * e.n_ub + js] = jsum[2][t];
+  }
+}
+
 __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimState* st,
                                  const double* __restrict__ part, const double* __restrict__ X,
                                  const double* __restrict__ Y, const double* __restrict__ Z,
@@ -688,6 +849,15 @@ int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab) {
   return RD_OK;
 }
 
+// Whether a simulation's mixed planar accelerations take the pair-symmetric
+// form: from its own particle bound alone, so that its result does not depend
+// on the other simulations of a pool.  Below kSymMin particles the gather
+// form, whose source segments spread one simulation over more workgroups,
+// finishes sooner.
+static bool sym_form(const rd_ctx* c, const Ent& e) {
+  return c->slots[e.slot].desc.g.kind == RD_GEOM_PLANAR && e.n_ub >= kSymMin;
+}
+
 int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_t k0,
                  hipEvent_t k1) {
   std::vector<Ent> ents;
@@ -706,16 +876,25 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
     return fail(RD_ERR_ARG, "unknown precision");
   long long off = 0;
-  int nwg = 0;  // workgroups of the compact pair launch
+  int nwg = 0;   // workgroups of the compact gather launch
+  int nsym = 0;  // workgroups of the pair-symmetric launch
   for (Ent& e : ents) {
     int nch = (e.n_ub + kChunk - 1) / kChunk;
     const int gx = std::max(1, (e.n_ub + kBlock - 1) / kBlock);
     e.nchunks_ub = nch;
-    segment(nch, seg_limit(gx, kWantPair), e.segs, e.cps);
+    e.wg0 = nwg;
+    e.wg1 = nsym;
+    if (precision == RD_PREC_MIXED && sym_form(c, e)) {
+      const int nb = (e.n_ub + kSymBlock - 1) / kSymBlock;
+      e.segs = nb;
+      e.cps = kSymBlock / kChunk;
+      nsym += nb * (nb + 1) / 2;
+    } else {
+      segment(nch, seg_limit(gx, kWantPair), e.segs, e.cps);
+      nwg += gx * e.segs;
+    }
     e.part_off = off;
     off += 3LL * (precision == RD_PREC_FP64 ? nch : e.segs) * e.n_ub;
-    e.wg0 = nwg;
-    nwg += gx * e.segs;
   }
   int r = ensure(c, c->part, sizeof(double) * (size_t)off);
   if (r) return r;
@@ -760,7 +939,11 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
                                                (float4*)c->img_hi.p, (float4*)c->img_lo.p);
   }
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-  if (any_planar)
+  if (nsym > 0)
+    k_accel_sym32<<<nsym, kSymBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
+                                                      (const float4*)c->pack_hi.p,
+                                                      (const float4*)c->pack_lo.p, part);
+  if (any_planar && grid > 0)
     k_accel_planar32<<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
                                                       (const float4*)c->pack_hi.p,
                                                       (const float4*)c->pack_lo.p, part);

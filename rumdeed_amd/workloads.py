@@ -43,6 +43,20 @@ def grid_points(nx=SNAPSHOT_GRID, ny=SNAPSHOT_GRID, box=SNAPSHOT_BOX, z=0.0):
     return np.stack([gx.ravel(), gy.ravel(), np.full(nx * ny, z)], axis=1)
 
 
+SYM_MIN = 16384  # particles from which the mixed planar force is pair-symmetric
+SYM_REACTION_FLOPS = 9  # reaction on the source: z terms 2, charge products 4, sums 3
+
+
+def force_flops(n: int, n_ic_max: int, precision: str = "mixed") -> int:
+    """Algorithmic flops of one planar acceleration evaluation of n
+    particles: the gather form evaluates n (n - 1) ordered pairs; the
+    pair-symmetric form (mixed, n >= SYM_MIN) evaluates n (n - 1) / 2 pairs,
+    each with the reaction on the source added."""
+    if precision == "mixed" and n >= SYM_MIN:
+        return n * (n - 1) // 2 * (pair_flops(n_ic_max) + SYM_REACTION_FLOPS)
+    return n * (n - 1) * pair_flops(n_ic_max)
+
+
 def pair_flops(n_ic_max: int) -> int:
     """Algorithmic flops of one planar pair interaction (FMA = 2, sqrt = 1,
     divide = 1), counted from the reference op sequence

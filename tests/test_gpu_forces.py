@@ -130,6 +130,55 @@ def test_mixed_error_vs_long_double(Diode, oracle_mod, n, nic):
     assert ef.max() < MIXED_TOL
 
 
+@pytest.mark.parametrize("n,ic,nic", [(16384, 1, 10), (16500, 0, 0), (16500, 1, 1), (40000, 1, 10)])
+def test_mixed_pair_symmetric_form(Diode, n, ic, nic):
+    """From 16384 particles the mixed planar force evaluates each unordered
+    pair once (blocks of 1024, the last one partial here for n != 16384).
+    Against the FP64 kernel, which the other tests pin bit-exactly to the
+    oracle: per-particle relative error <= 1e-5; repeated calls are
+    bit-identical."""
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(n, seed=21)
+    q = q.copy()
+    q[7::97] = 0.0  # absorbed particles: no force on them, none from them
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, ic, nic)
+    a64 = dd.calculate_acceleration_particles("fp64")
+    am = dd.calculate_acceleration_particles("mixed")
+    keep = q != 0.0
+    e = rel_err(am[keep], a64[keep])
+    print(f"symmetric mixed n={n} ic={ic} nic={nic}: max {e.max():.2e} p99 {np.percentile(e, 99):.2e}")
+    assert e.max() < MIXED_TOL
+    assert np.all(am[~keep] == 0.0)
+    assert np.array_equal(dd.calculate_acceleration_particles("mixed"), am)
+
+
+@pytest.mark.parametrize("nic", [0, 1, 10])
+def test_mixed_field_on_and_off_the_cathode(Diode, oracle_mod, nic):
+    """Points on the cathode plane take the paired-image form of the series
+    (lateral image terms cancel there); a wavefront holding any point off the
+    plane takes the general form.  Both within 1e-5 of the long-double
+    oracle, and the lateral field on the plane is zero."""
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(2000, seed=9)
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, 1, nic)
+    g = oracle_mod.geom_planar(W.SNAPSHOT_D, W.SNAPSHOT_V, True, nic)
+    on = W.grid_points(16, 16)  # 256 points, z = 0
+    mixed = on.copy()
+    mixed[5::64, 2] = 1e-12  # one point off the plane in every wavefront
+    above = on.copy()
+    above[:, 2] = np.linspace(1e-9, 900e-9, len(above))
+    for pts in (on, mixed, above):
+        ref = oracle_mod.field(g, pos, q, pts, long_double=True)
+        f = dd.calc_field_at_batch(pts, "mixed")
+        e = rel_err(f, ref)
+        print(f"nic={nic} z in [{pts[:, 2].min():.1e}, {pts[:, 2].max():.1e}]: max {e.max():.2e}")
+        assert e.max() < MIXED_TOL
+    f = dd.calc_field_at_batch(on, "mixed")
+    assert np.all(f[:, :2] == 0.0)
+
+
 def vacuum_accel(oracle_mod, g, p, q, m):
     return q / m * oracle_mod.vacuum_field(g, p)
 
