@@ -583,25 +583,24 @@ __device__ __forceinline__ void fe_width_update(SimState& S, int it, int first, 
   }
 }
 
-// Accept/reject of jump `it` for the chains [k_begin, k_end) of entry e, by
-// one block (a power of two of lanes; ra, rr: LDS arrays of blockDim.x
-// entries for the batch acceptance counts; lr: the log-uniforms of the jump,
-// by chain).  Planar FE: with counts == nullptr the block holds every chain
-// and updates the width itself; otherwise it stores its (accepted, rejected)
-// counts there for k_mh_width.
+// Accept/reject of jump `it` for chain k of entry e (k < 0: no chain), one
+// chain per lane of a block (a power of two of lanes; ra, rr: LDS arrays of
+// blockDim.x entries for the batch acceptance counts; lr: the log-uniforms of
+// the jump, by chain; (fxk, fyk, fzk): the field at the chain's proposal).
+// Planar FE: with counts == nullptr the block holds every chain and updates
+// the width itself; otherwise it stores its (accepted, rejected) counts there
+// for k_mh_width.
 __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, const double* wf,
                                 const Chains& ch, int it, const double* ptx, const double* pty,
-                                const double* ptz, const double* fx, const double* fy,
-                                const double* fz, const double* lr, int* ra, int* rr,
-                                int k_begin, int k_end, int* counts) {
+                                const double* ptz, double fxk, double fyk, double fzk,
+                                const double* lr, int* ra, int* rr, int k, int* counts) {
   const EmitConsts ec = emit_consts();
   const int first = nint(s.ndim * 0.25);
   const bool tip = s.mode == RD_EMIT_TIP_FE;
   const bool counted = (s.mode == RD_EMIT_PLANAR_GTF) || (it > first);
   int na = 0, nrj = 0;
-  for (int k = k_begin + threadIdx.x; k < k_end; k += blockDim.x) {
-    const int c = e.cbase + k;
-    if (ch.ok[c] != 1) continue;
+  const int c = e.cbase + k;
+  if (k >= 0 && ch.ok[c] == 1) {
     const int p = e.pbase + k;
     bool acc = false;
     bool rejected = false;
@@ -609,7 +608,7 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
     if (tip && !ch.pv[c]) {
       rejected = true;
     } else {
-      F = tip ? field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]) : fz[p];
+      F = tip ? field_normal(s.g, ptx[p], pty[p], fxk, fyk, fzk) : fzk;
       bool bad = (s.mode == RD_EMIT_PLANAR_GTF) ? (F > 0.0) : (F >= 0.0);
       if (bad) {
         rejected = true;
@@ -924,23 +923,28 @@ __global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, int ne,
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const double* part = a.part + e.part_off;
-  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+  // one chain per lane: k1 - k0 <= kSettle
+  const int k = k0 + (int)threadIdx.x < k1 ? k0 + (int)threadIdx.x : -1;
+  double fxk = 0.0, fyk = 0.0, fzk = 0.0;
+  if (k >= 0) {
     const int p = e.pbase + k;
     if (PREC == RD_PREC_FP64) {
       field_finish64(s.g, part, (size_t)e.nchunks_ub * m, m, k, nchunks, a.ptx[p], a.pty[p],
-                     a.ptz[p], a.fx[p], a.fy[p], a.fz[p]);
+                     a.ptz[p], fxk, fyk, fzk);
     } else {
       const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
       if (s.g.kind == RD_GEOM_TIP)
         field_finish32_tip(s.g, part, (size_t)e.segs * m, m, k, segs, a.ptx[p], a.pty[p],
-                           a.ptz[p], a.fx[p], a.fy[p], a.fz[p]);
+                           a.ptz[p], fxk, fyk, fzk);
       else
-        field_finish32(s.g, part, (size_t)e.segs * m, m, k, segs, a.fx[p], a.fy[p], a.fz[p]);
+        field_finish32(s.g, part, (size_t)e.segs * m, m, k, segs, fxk, fyk, fzk);
     }
+    a.fx[p] = fxk;
+    a.fy[p] = fyk;
+    a.fz[p] = fzk;
   }
-  __syncthreads();
-  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, a.fx, a.fy, a.fz,
-                  a.rlr + (size_t)(it - 1) * a.C, ra, rr, k0, k1,
+  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, fxk, fyk, fzk,
+                  a.rlr + (size_t)(it - 1) * a.C, ra, rr, k,
                   settle_groups(m) > 1 ? counts + 2 * (size_t)blockIdx.x : nullptr);
 }
 

@@ -96,17 +96,29 @@ struct Ent {
 };
 
 // Entry of a compact launch whose workgroups [wg0, next wg0) belong to one
-// entry: the last entry with wg0 <= b (binary search, uniform per block).
+// entry: the last entry with wg0 <= b (wg1 for the second layout).  The
+// offsets are non-decreasing, so the entries that pass form a prefix: the
+// first wavefront tests 64 entries per round with independent loads and
+// broadcasts the answer through LDS.  Every thread of the block calls it, in
+// uniform control flow (it may hold a barrier), and the block has >= 64
+// lanes.
 __device__ __forceinline__ int find_entry(const Ent* tab, int ne, int b, bool second = false) {
-  int lo = 0, hi = ne - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if ((second ? tab[mid].wg1 : tab[mid].wg0) <= b)
-      lo = mid;
-    else
-      hi = mid - 1;
+  __shared__ int s_entry;
+  if (ne == 1) return 0;  // uniform over the launch
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int best = 0;
+    for (int k0 = 0; k0 < ne; k0 += 64) {
+      const int k = k0 + lane;
+      const bool ok = k < ne && (second ? tab[k].wg1 : tab[k].wg0) <= b;
+      const unsigned long long m = __ballot(ok);
+      if (m) best = k0 + 63 - __clzll((long long)m);
+      if (m != ~0ull) break;
+    }
+    if (lane == 0) s_entry = best;
   }
-  return lo;
+  __syncthreads();
+  return s_entry;
 }
 
 constexpr int kChunk = 256;  // canonical FP64 summation chunk
