@@ -84,13 +84,13 @@ def cpu_baseline(n_sample: int, nic: int, threads: int):
                       f"{dt:.2f} s, {threads} OpenMP threads on {cpu}"}
 
 
-def serve_bench(n, precision, realtime, steps, rate):
+def serve_bench(n, precision, realtime, steps, rate, graphs=False):
     """The batched serving stream (Poisson arrivals, continuous batching)."""
     from rumdeed_amd.serve import Server, stream_requests
 
     reqs = stream_requests(n=n, rate=rate, steps=steps, regime="intended")
     srv = Server(total_particles=1 << 23, max_slots=512, hbm_budget=96 << 30,
-                 precision=precision)
+                 precision=precision, graphs=graphs)
     done = []
 
     def progress(res, k):
@@ -103,7 +103,8 @@ def serve_bench(n, precision, realtime, steps, rate):
     srv.close()
     emitted = np.array([r.emitted for r in results])
     nfin = np.array([r.n_final for r in results])
-    stats.update({"precision": precision, "realtime_arrivals": realtime, "steps_per_request": steps,
+    stats.update({"precision": precision, "graphs": bool(graphs), "realtime_arrivals": realtime,
+                  "steps_per_request": steps,
                   "arrival_rate_per_s": rate,
                   "emitted_total_median": float(np.median(emitted)),
                   "emitted_total_max": int(emitted.max()),
@@ -124,13 +125,14 @@ E2E = {
 }
 
 
-def e2e_bench(name, steps, precision):
+def e2e_bench(name, steps, precision, graphs=False):
     from rumdeed_amd.diode import Diode
 
     cfg = dict(E2E[name])
     if steps:
         cfg["steps"] = steps
     dd = Diode(1 << 18)
+    dd.set_graphs(graphs)
     if cfg["kind"] == "planar":
         dd.set_planar(cfg["V"], cfg["d"], True, cfg["nic"])
     else:
@@ -169,7 +171,8 @@ def e2e_bench(name, steps, precision):
         rel = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
         force_err = {"n": int(dd.n), "max_rel": float(rel.max()),
                      "rms_rel": float(np.sqrt(np.mean(rel ** 2)))}
-    out = {"workload": name, "precision": precision, "steps": cfg["steps"],
+    out = {"workload": name, "precision": precision, "graphs": bool(graphs),
+           "steps": cfg["steps"],
            "wall_s": wall, "ms_per_step": wall / cfg["steps"] * 1e3,
            "force_pair_interactions_per_s": pairs / wall,
            "electrons_final": recs[-1].n, "electrons_max": max(r.n for r in recs),
@@ -216,6 +219,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-precision", default="mixed")
     ap.add_argument("--no-realtime", action="store_true")
+    ap.add_argument("--graphs", action="store_true",
+                    help="serving and end-to-end runs: each step's jump loop as one HIP graph")
     ap.add_argument("--e2e", default="", help="end-to-end workload: " + ", ".join(E2E))
     ap.add_argument("--e2e-steps", type=int, default=0)
     ap.add_argument("--e2e-precision", default="mixed")
@@ -242,11 +247,11 @@ def main():
     if args.serve:
         print(json.dumps({"serving": serve_bench(args.serve, args.serve_precision,
                                                  not args.no_realtime, args.serve_steps,
-                                                 args.serve_rate)}), flush=True)
+                                                 args.serve_rate, args.graphs)}), flush=True)
         return
     if args.e2e:
-        print(json.dumps({"e2e": e2e_bench(args.e2e, args.e2e_steps, args.e2e_precision)}),
-              flush=True)
+        print(json.dumps({"e2e": e2e_bench(args.e2e, args.e2e_steps, args.e2e_precision,
+                                           args.graphs)}), flush=True)
         return
 
     from rumdeed_amd import workloads as W

@@ -196,6 +196,13 @@ int rd_slot_step(rd_ctx* ctx, int32_t slot, int32_t* step);
  * (max_slots entries, may be NULL); slots that are not open are left untouched. */
 int rd_pool_step(rd_ctx* ctx, int32_t precision, rd_step_record* records);
 
+/* Launch the Metropolis-Hastings jump loop of each emission step (about
+ * 400 kernels at 200 jumps) as one HIP graph (on != 0) or as separate
+ * launches (0, the default).  The graph is captured when the loop's launch
+ * parameters change and replayed while they do not; results are identical
+ * either way.  Library control, no reference counterpart. */
+int rd_set_graphs(rd_ctx* ctx, int32_t on);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* Average device time (ms) of the accelerations / grid field over `reps`
  * launches, from HIP events on the context's stream; `kernel_ms` receives

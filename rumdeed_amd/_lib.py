@@ -114,6 +114,7 @@ _SIGS = {
     "rd_time_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double,
                                      c_int32, c_int32, c_double, c_int32, c_int32, _dp, _dp]),
     "rd_time_copy": (c_int32, [c_void_p, c_int64, c_int32, _dp]),
+    "rd_set_graphs": (c_int32, [c_void_p, c_int32]),
 }
 
 _lib = None

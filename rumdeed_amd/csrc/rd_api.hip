@@ -373,6 +373,7 @@ int rd_ctx_destroy(rd_ctx* c) {
   if (c->d_state) hipFree(c->d_state);
   if (c->d_life) hipFree(c->d_life);
   if (c->d_wf) hipFree(c->d_wf);
+  if (c->jump_exec) hipGraphExecDestroy(c->jump_exec);
   if (c->h_state) hipHostFree(c->h_state);
   if (c->h_tab) hipHostFree(c->h_tab);
   Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
@@ -867,6 +868,17 @@ static int copy_body(rd_ctx* c, void* a, hipEvent_t k0, hipEvent_t k1) {
 }
 
 extern "C" {
+
+int rd_set_graphs(rd_ctx* c, int32_t on) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  RD_HIP(hipSetDevice(c->device));
+  RD_HIP(hipStreamSynchronize(c->stream));
+  if (c->jump_exec) RD_HIP(hipGraphExecDestroy(c->jump_exec));
+  c->jump_exec = nullptr;
+  c->jump_key.clear();
+  c->graphs = on != 0;
+  return RD_OK;
+}
 
 int rd_time_copy(rd_ctx* c, int64_t bytes, int32_t reps, double* kernel_ms) {
   if (!c) return fail(RD_ERR_ARG, "null context");

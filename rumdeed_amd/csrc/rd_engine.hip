@@ -1675,6 +1675,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   }
   if (ents.empty()) return RD_OK;
   JumpArgs a;
+  std::memset(&a, 0, sizeof(a));  // its bytes form part of the graph key
   double* ptx = (double*)c->pts.p;
   double* f = (double*)c->fld.p;
   a.ptx = ptx;
@@ -1724,17 +1725,42 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   Ent* dt = nullptr;
   r = upload_tab(c, ents, &dt);
   if (r) return r;
-  {
-    double* rb = (double*)c->rnd.p;
-    k_mh_random<<<dim3((C + 255) / 256, maxndim), 256, 0, c->stream>>>(dt, ne, c->d_desc, C, rb,
-                                                                         rb + nr, rb + 2 * nr);
-    a.ru1 = rb;
-    a.ru2 = rb + nr;
-    a.rlr = rb + 2 * nr;
-    a.C = C;
-  }
+  double* rb = (double*)c->rnd.p;
+  a.ru1 = rb;
+  a.ru2 = rb + nr;
+  a.rlr = rb + 2 * nr;
+  a.C = C;
   a.part = (double*)c->part.p;
   const int grid = nwg;
+  // Everything the launches below depend on besides device data.  With graphs
+  // on, an unchanged key replays the captured loop (the kernels read the
+  // launch table and the chains from device memory, so new contents under
+  // the same pointers are picked up).
+  std::vector<unsigned char> key;
+  auto put = [&key](const void* v, size_t bytes) {
+    const unsigned char* b = (const unsigned char*)v;
+    key.insert(key.end(), b, b + bytes);
+  };
+  const int ints[] = {ne, grid, nsettle, (int)any_split, (int)any_planar, (int)any_tip, (int)fp64,
+                      maxndim, C};
+  const void* ptrs[] = {dt, c->d_desc, c->d_state, c->d_wf, counts};
+  put(ints, sizeof(ints));
+  put(ptrs, sizeof(ptrs));
+  put(&a, sizeof(a));
+  put(&V, sizeof(V));
+  if (c->graphs && c->jump_exec && key == c->jump_key) {
+    RD_HIP(hipGraphLaunch(c->jump_exec, c->stream));
+    return RD_OK;
+  }
+  if (c->graphs) {
+    // the stream was synchronised after the previous launch of the old graph
+    if (c->jump_exec) RD_HIP(hipGraphExecDestroy(c->jump_exec));
+    c->jump_exec = nullptr;
+    c->jump_key.clear();
+    RD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  }
+  k_mh_random<<<dim3((C + 255) / 256, maxndim), 256, 0, c->stream>>>(dt, ne, c->d_desc, C, rb,
+                                                                       rb + nr, rb + 2 * nr);
   for (int it = 1; it <= maxndim; ++it) {
     if (fp64) {
       if (any_planar)
@@ -1757,6 +1783,25 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     }
     if (any_split)
       k_mh_width<<<ne, 64, 0, c->stream>>>(dt, c->d_desc, c->d_state, it, counts);
+  }
+  if (c->graphs) {
+    hipGraph_t g = nullptr;
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (le != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return hip_fail(le, "jump loop launch (capture)");
+    }
+    RD_HIP(ce);
+    const hipError_t ie = hipGraphInstantiate(&c->jump_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      c->jump_exec = nullptr;
+      return hip_fail(ie, "hipGraphInstantiate (jump loop)");
+    }
+    c->jump_key = key;
+    RD_HIP(hipGraphLaunch(c->jump_exec, c->stream));
+    return RD_OK;
   }
   RD_HIP(hipGetLastError());
   return RD_OK;

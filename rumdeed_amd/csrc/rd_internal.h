@@ -179,6 +179,13 @@ struct rd_ctx {
   int cur = 0;               // slot the single-simulation entry points act on
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+
+  // Metropolis-Hastings jump loop as a HIP graph (rd_set_graphs): the last
+  // captured loop and the launch parameters it was captured with; replayed
+  // while they are unchanged.
+  int graphs = 0;
+  hipGraphExec_t jump_exec = nullptr;
+  std::vector<unsigned char> jump_key;
 };
 
 namespace rd {

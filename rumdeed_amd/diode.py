@@ -68,6 +68,10 @@ class Diode:
     def set_time_step(self, dt: float):
         check(self._lib.rd_set_time_step(self._h, c_double(dt)))
 
+    def set_graphs(self, on: bool = True):
+        """Launch each step's Metropolis-Hastings jump loop as one HIP graph."""
+        check(self._lib.rd_set_graphs(self._h, int(bool(on))))
+
     def set_seed(self, seed: int, stream: int = 0):
         check(self._lib.rd_set_seed(self._h, int(seed), int(stream)))
 

@@ -104,8 +104,10 @@ MH_JUMPS = 200  # Metropolis-Hastings jumps per chain of the planar FE sampler
 
 class Server:
     def __init__(self, total_particles: int = 1 << 22, max_slots: int = 512,
-                 hbm_budget: int = 64 << 30, precision: str = "mixed", device: int = 0):
+                 hbm_budget: int = 64 << 30, precision: str = "mixed", device: int = 0,
+                 graphs: bool = False):
         self.pool = Pool(total_particles, max_slots, device)
+        self.pool.set_graphs(graphs)  # jump loop of each step as one HIP graph
         self.budget = int(hbm_budget)
         self.precision = precision
         self.max_slots = max_slots

@@ -266,6 +266,57 @@ def test_mixed_precision_many_chains_statistics(Diode):
     assert abs(r64[1].n_sup - r32[1].n_sup) < 1e-3 * r64[1].n_sup
 
 
+def test_jump_graph_equals_separate_launches(oracle_mod):
+    """Each step's Metropolis-Hastings jump loop launched as one HIP graph
+    (captured when its launch parameters change, replayed while they do not,
+    with the rest of the step launched eagerly in between): FP64 records equal
+    the oracle's at every step, and mixed records and state equal those of
+    separate launches."""
+    from rumdeed_amd.diode import Pool
+
+    cfgs = [dict(seed=5, V=1600.0, d=200 * NM), dict(seed=6, V=1500.0, d=180 * NM),
+            dict(seed=7, V=1700.0, d=220 * NM, wf=((4.6, 4.8),))]
+
+    def make(graphs):
+        pool = Pool(3 * 2048, 4)
+        pool.set_graphs(graphs)
+        slots = []
+        for cfg in cfgs:
+            slots.append(pool.open(2048))
+            _fe(pool, oracle_mod, **cfg)
+        return pool, slots
+
+    pool, slots = make(True)
+    oracles = []
+    for cfg in cfgs:
+        so = oracle_mod.Sim(2048, nthreads=8)
+        _fe(so, oracle_mod, **cfg)
+        oracles.append(so)
+    for k in range(1, 13):
+        recs = pool.step("fp64")
+        print(f"graph fp64 step {k}: n_round {[recs[s].n_round for s in slots]}", flush=True)
+        for s, so in zip(slots, oracles):
+            assert_records_equal(recs[s], so.step(k), f"slot {s} step {k}")
+    for s, so in zip(slots, oracles):
+        pool.select(s)
+        assert_state_equal(pool.get_state(), so.state(), f"slot {s}")
+    pool.close()
+
+    runs = []
+    for graphs in (True, False):
+        pool, slots = make(graphs)
+        recs = [[pool.step("mixed")[s].as_dict() for s in slots] for _ in range(12)]
+        states = []
+        for s in slots:
+            pool.select(s)
+            states.append(pool.get_state())
+        runs.append((recs, states))
+        pool.close()
+    assert runs[0][0] == runs[1][0]
+    for sg, se in zip(runs[0][1], runs[1][1]):
+        assert_state_equal(sg, se, "mixed graph vs separate launches")
+
+
 def test_mixed_pool_alone_equals_in_the_mix():
     """Mixed precision in a pool: a many-chain diode (jumps split over chain
     groups and source segments) and two small ones advance together; each
