@@ -570,6 +570,16 @@ __device__ __forceinline__ void mh_record(const Ent& e, const SimDesc& s, const 
 
 // The planar FE width update of one jump from the batch counts
 // (MH_std_update, src/mod_field_emission_v2.F90:603-612).
+__device__ __forceinline__ double fe_width_after(double mh, int it, int first, int na, int nr) {
+  if ((it > first) && (na + nr > 0)) {
+    mh = mh * rd_exp(0.025 * ((double)na / (double)(na + nr) - 0.35));
+    if (mh > 0.1250)
+      mh = 0.1250;
+    else if (mh < 0.00005)
+      mh = 0.00005;
+  }
+  return mh;
+}
 __device__ __forceinline__ void fe_width_update(SimState& S, int it, int first, int na, int nr) {
   S.it_a = na;
   S.it_r = nr;
@@ -589,7 +599,7 @@ __device__ __forceinline__ void fe_width_update(SimState& S, int it, int first, 
 // the jump, by chain; (fxk, fyk, fzk): the field at the chain's proposal).
 // Planar FE: with counts == nullptr the block holds every chain and updates
 // the width itself; otherwise it stores its (accepted, rejected) counts there
-// for k_mh_width.
+// (see split_width).
 __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, const double* wf,
                                 const Chains& ch, int it, const double* ptx, const double* pty,
                                 const double* ptz, double fxk, double fyk, double fzk,
@@ -673,8 +683,8 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
 // computes the same one; the first source segment's workgroup records it)
 // and writes the field partial sums of one source segment.  k_mh_settle: one
 // workgroup per kSettle chains of a simulation adds the partials in a fixed
-// order and runs the accept step for those chains; k_mh_width then updates
-// the planar FE width of simulations whose chains took several workgroups.
+// order and runs the accept step for those chains (the planar FE width of
+// simulations whose chains take several workgroups: see split_width).
 // Kernel boundaries are the only synchronisation: no counters, no
 // device-wide fences.
 struct JumpArgs {
@@ -683,8 +693,37 @@ struct JumpArgs {
   const double *X, *Y, *Z, *Q;    // FP64 sources
   const float4 *HI, *LO;          // packed sources (mixed)
   const double *ru1, *ru2, *rlr;  // per jump and chain: unit normals, log-uniform ([it-1][C])
+  int* counts;                    // [jump parity][accept workgroup][accepted, rejected]
   int C;                          // chains in all
+  int nsettle;                    // accept workgroups of a jump
 };
+
+constexpr int kSettle = 1024;  // lanes of a finish-and-accept workgroup
+
+// Workgroups of a simulation's finish-and-accept: one per kSettle chains.
+__host__ __device__ inline int settle_groups(int m) { return max(1, (m + kSettle - 1) / kSettle); }
+
+// Planar FE simulations whose accept step takes several workgroups ("split"):
+// the workgroups of jump j store their batch counts in counts[j & 1]; the
+// width after jump j, f(width after j - 1, counts of j), is formed by every
+// proposal of jump j + 1 from the stored width and those counts, and stored
+// into the state by the first accept workgroup of jump j + 1 (nothing else
+// reads the stored width during that launch, and nothing writes it during a
+// proposal launch).  The width after the last jump is stored by
+// k_mh_width_last.  No launch is added per jump.
+__device__ __forceinline__ bool split_width(const SimDesc& s, const Ent& e) {
+  return s.mode == RD_EMIT_PLANAR_FE && settle_groups(e.nch) > 1;
+}
+__device__ __forceinline__ void split_counts(const Ent& e, const int* counts, int nsettle, int j,
+                                             int& na, int& nr) {
+  const int* cn = counts + (size_t)(j & 1) * 2 * nsettle + 2 * (size_t)e.wg1;
+  na = 0;
+  nr = 0;
+  for (int g = 0; g < settle_groups(e.nch); ++g) {
+    na += cn[2 * g];
+    nr += cn[2 * g + 1];
+  }
+}
 
 // The random numbers of every jump of every chain, drawn before the jumps so
 // that none of them is on a jump's critical path: the unit normal pair of the
@@ -722,7 +761,13 @@ __device__ __forceinline__ Proposal jump_proposal(const Ent& e, const SimDesc& s
                                                   const SimState* st, const Chains& ch, int it,
                                                   int k, const JumpArgs& a) {
   const size_t o = (size_t)(it - 1) * a.C + e.cbase + k;
-  return mh_proposal(e, s, st[e.slot].MH_std, ch, it, k, a.ru1[o], a.ru2[o]);
+  double mh = st[e.slot].MH_std;
+  if (it >= 2 && split_width(s, e)) {
+    int na, nr;
+    split_counts(e, a.counts, a.nsettle, it - 1, na, nr);
+    mh = fe_width_after(mh, it - 1, nint(s.ndim * 0.25), na, nr);
+  }
+  return mh_proposal(e, s, mh, ch, it, k, a.ru1[o], a.ru2[o]);
 }
 
 // FP64: one lane per chain, 256 chains per workgroup, sources in 256-source
@@ -767,6 +812,15 @@ constexpr int kJumpTargets = 64;
 constexpr int kJumpWaves = 16;
 constexpr int kJumpSlice = kChunk / kJumpWaves;
 
+// p[j] for a wavefront-uniform j of memory that no launch in flight writes:
+// read through the constant address space, so the load goes through the
+// scalar cache into SGPRs.
+__device__ __forceinline__ float4 load_uniform(const float4* p, int j) {
+  typedef const __attribute__((address_space(4))) float cfloat;
+  const cfloat* q = (const cfloat*)(const float*)(p + j);
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+
 // One mixed pair term; `surf`: the wavefront's targets lie on z = 0.
 __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, const float4 h2,
                                            const float4 l2, int ic, int nic, float two_d,
@@ -808,7 +862,7 @@ __host__ __device__ inline int jump_groups32(int m) {
 }
 
 template <int KIND>
-__global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, int ne,
+__global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* tab, int ne,
                                                                 const SimDesc* desc,
                                                                 const SimState* st, Chains ch,
                                                                 int it, JumpArgs a) {
@@ -850,7 +904,42 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
   const float kt = g.ic ? (float)g.ktip : 0.0f;
   const float Rn = (float)(g.R * 1e9);
   const bool surf = KIND == RD_GEOM_PLANAR && wave_on_surface(ic, act, h1, l1);
-  for (int c = c0; c < c1; ++c) {
+  // Planar chains on the cathode, one per lane: a wavefront's sources are
+  // the same for all its lanes, so they are read through the scalar cache
+  // into SGPRs (no LDS traffic) in the same order as the staged loop below.
+  // Every wavefront of the workgroup holds the same chains, so the branch is
+  // uniform over the workgroup.
+  const bool scalar_src = KIND == RD_GEOM_PLANAR && !by_wave &&
+                          __syncthreads_or(surf ? 1 : 0) && __syncthreads_and(surf ? 1 : 0);
+  if (scalar_src) {
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const float dnm = 0.5f * two_d;
+    for (int c = c0; c < c1; ++c) {
+      const int jb = c * kChunk + wu * kJumpSlice;
+      const int je = min(n, jb + kJumpSlice);
+      if (act) {
+        auto term = [&](const float4 h2, const float4 l2) {
+          const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+          const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+          const float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+          fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, dnm));
+        };
+        int j = jb;
+        for (; j + 4 <= je; j += 4) {  // four sources' loads in flight together
+          float4 hh[4], ll[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            hh[u] = load_uniform(hi, j + u);
+            ll[u] = load_uniform(lo, j + u);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) term(hh[u], ll[u]);
+        }
+        for (; j < je; ++j) term(load_uniform(hi, j), load_uniform(lo, j));
+      }
+    }
+  }
+  for (int c = scalar_src ? c1 : c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
     if (t < kChunk && jb + t < n) {
@@ -866,8 +955,23 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
     const int dj = by_wave ? 64 : 1;
     if (act) {
       if (KIND == RD_GEOM_PLANAR) {
-        for (int jj = j0; jj < j1; jj += dj)
-          mixed_pair(h1, l1, shi[jj], slo[jj], ic, nic, two_d, surf, fx, fy, fz);
+        // one loop per form, so that no branch sits in the source loop
+        if (surf) {
+          const float dnm = 0.5f * two_d;
+          for (int jj = j0; jj < j1; jj += dj) {
+            const float4 h2 = shi[jj], l2 = slo[jj];
+            const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+            const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+            const float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+            fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, dnm));
+          }
+        } else if (ic) {
+          for (int jj = j0; jj < j1; jj += dj)
+            mixed_pair(h1, l1, shi[jj], slo[jj], 1, nic, two_d, false, fx, fy, fz);
+        } else {
+          for (int jj = j0; jj < j1; jj += dj)
+            mixed_pair(h1, l1, shi[jj], slo[jj], 0, 0, two_d, false, fx, fy, fz);
+        }
       } else {
         for (int jj = j0; jj < j1; jj += dj) {
           float cx, cy, cz;
@@ -903,16 +1007,11 @@ __global__ __launch_bounds__(64 * kJumpWaves) void k_mh_field32(const Ent* tab, 
 // Finish and accept: one workgroup per kSettle chains of a simulation.  FP64
 // adds the chunk partials of every chain in chunk order (plus the vacuum
 // field); mixed adds the segment partials in segment order.
-constexpr int kSettle = 1024;  // lanes of a finish-and-accept workgroup
-
-// Workgroups of a simulation's finish-and-accept: one per kSettle chains.
-__host__ __device__ inline int settle_groups(int m) { return max(1, (m + kSettle - 1) / kSettle); }
-
 template <int PREC>
 __global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, int ne,
                                                        const SimDesc* desc, SimState* st,
                                                        const double* wf, Chains ch, int it,
-                                                       JumpArgs a, int* counts) {
+                                                       JumpArgs a) {
   __shared__ int ra[kSettle], rr[kSettle];
   const Ent e = tab[find_entry(tab, ne, blockIdx.x, true)];
   const SimDesc& s = desc[e.slot];
@@ -943,26 +1042,28 @@ __global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, int ne,
     a.fy[p] = fyk;
     a.fz[p] = fzk;
   }
+  const bool split = split_width(s, e);
+  if (split && grp == 0 && threadIdx.x == 0 && it >= 2) {
+    // store the width after the previous jump (its proposals formed it too)
+    int na, nr;
+    split_counts(e, a.counts, a.nsettle, it - 1, na, nr);
+    fe_width_update(st[e.slot], it - 1, nint(s.ndim * 0.25), na, nr);
+  }
   mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, fxk, fyk, fzk,
                   a.rlr + (size_t)(it - 1) * a.C, ra, rr, k,
-                  settle_groups(m) > 1 ? counts + 2 * (size_t)blockIdx.x : nullptr);
+                  split ? a.counts + (size_t)(it & 1) * 2 * a.nsettle + 2 * (size_t)blockIdx.x
+                        : nullptr);
 }
 
-// Planar FE simulations whose accept step took several workgroups: the batch
-// counts summed over them (integers, so in any order) and the width update.
-__global__ void k_mh_width(const Ent* tab, const SimDesc* desc, SimState* st, int it,
-                           const int* __restrict__ counts) {
+// Split planar FE simulations (split_width): the width after their last jump.
+__global__ void k_mh_width_last(const Ent* tab, const SimDesc* desc, SimState* st,
+                                const int* __restrict__ counts, int nsettle) {
   const Ent e = tab[blockIdx.x];
   const SimDesc& s = desc[e.slot];
-  if (threadIdx.x != 0 || it > s.ndim || s.mode != RD_EMIT_PLANAR_FE) return;
-  const int gs = settle_groups(e.nch);
-  if (gs == 1) return;
-  int na = 0, nr = 0;
-  for (int g = 0; g < gs; ++g) {
-    na += counts[2 * (e.wg1 + g)];
-    nr += counts[2 * (e.wg1 + g) + 1];
-  }
-  fe_width_update(st[e.slot], it, nint(s.ndim * 0.25), na, nr);
+  if (threadIdx.x != 0 || s.ndim < 1 || !split_width(s, e)) return;
+  int na, nr;
+  split_counts(e, counts, nsettle, s.ndim, na, nr);
+  fe_width_update(st[e.slot], s.ndim, nint(s.ndim * 0.25), na, nr);
 }
 
 // Per-chain width updates (thermal-field, tip) in chain order and the escape
@@ -1657,8 +1758,8 @@ static int reset_counters(rd_ctx* c, const std::vector<Ent>& ents) {
 
 // The Metropolis-Hastings jumps 1..maxndim of the chain entries `in` (C
 // chains in all: the stride of the point and field arrays): per jump, one
-// field launch per geometry present, then the finish-and-accept launch and,
-// for planar FE simulations with more than kSettle chains, the width update.
+// field launch per geometry present, then the finish-and-accept launch; after
+// the last jump, the width of split planar FE simulations (split_width).
 static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision,
                      const Chains& V, int maxndim) {
   const bool fp64 = precision == RD_PREC_FP64;
@@ -1699,6 +1800,13 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   int nwg = 0;     // workgroups of the compact field launch
   int nsettle = 0;  // workgroups of the finish-and-accept launch
   bool any_split = false;
+  // A context made for one simulation spreads each jump's sums over up to
+  // kWantJump workgroups (latency); in a pool, whose launches serve many
+  // simulations, every mixed workgroup takes at least kPoolJumpChunks chunks
+  // of sources (throughput; the FP64 kernel, bound by its arithmetic, keeps
+  // the finer split).  Either way the split depends on the simulation's own
+  // size and the kind of context only.
+  const int min_cps = (c->nslots_max > 1 && !fp64) ? kPoolJumpChunks : 1;
   for (Ent& e : ents) {
     e.wg1 = nsettle;
     nsettle += settle_groups(e.nch);
@@ -1708,6 +1816,10 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     const int gx = fp64 ? (e.nch + per - 1) / per : jump_groups32(e.nch);
     e.nchunks_ub = nch;
     segment(nch, seg_limit(gx, kWantJump), e.segs, e.cps);
+    if (e.cps < min_cps) {
+      e.cps = std::min(min_cps, std::max(nch, 1));
+      e.segs = (std::max(nch, 1) + e.cps - 1) / e.cps;
+    }
     e.part_off = off;
     off += 3LL * (fp64 ? std::max(nch, 1) : e.segs) * e.nch;
     e.wg0 = nwg;
@@ -1719,9 +1831,11 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   const size_t nr = (size_t)maxndim * C;
   r = ensure(c, c->rnd, sizeof(double) * 3 * nr);
   if (r) return r;
-  r = ensure(c, c->cnt, sizeof(int) * 2 * (size_t)nsettle);
+  r = ensure(c, c->cnt, sizeof(int) * 4 * (size_t)nsettle);
   if (r) return r;
   int* counts = (int*)c->cnt.p;
+  a.counts = counts;
+  a.nsettle = nsettle;
   Ent* dt = nullptr;
   r = upload_tab(c, ents, &dt);
   if (r) return r;
@@ -1770,7 +1884,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
         k_mh_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
                                                                    V, it, a);
       k_mh_settle<RD_PREC_FP64><<<nsettle, kSettle, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a, counts);
+          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a);
     } else {
       if (any_planar)
         k_mh_field32<RD_GEOM_PLANAR><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc,
@@ -1779,22 +1893,22 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
         k_mh_field32<RD_GEOM_TIP><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
                                                                    V, it, a);
       k_mh_settle<RD_PREC_MIXED><<<nsettle, kSettle, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a, counts);
+          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a);
     }
-    if (any_split)
-      k_mh_width<<<ne, 64, 0, c->stream>>>(dt, c->d_desc, c->d_state, it, counts);
   }
+  if (any_split)
+    k_mh_width_last<<<ne, 64, 0, c->stream>>>(dt, c->d_desc, c->d_state, counts, nsettle);
   if (c->graphs) {
     hipGraph_t g = nullptr;
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(c->stream, &g);
     if (le != hipSuccess) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       return hip_fail(le, "jump loop launch (capture)");
     }
     RD_HIP(ce);
     const hipError_t ie = hipGraphInstantiate(&c->jump_exec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (ie != hipSuccess) {
       c->jump_exec = nullptr;
       return hip_fail(ie, "hipGraphInstantiate (jump loop)");

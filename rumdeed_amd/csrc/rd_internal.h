@@ -218,6 +218,7 @@ inline int seg_limit(int target_blocks, int want) {
 }
 constexpr int kWantPair = 8192;  // accelerations, field at points
 constexpr int kWantJump = 512;   // fields of the Metropolis-Hastings chains
+constexpr int kPoolJumpChunks = 4;  // least source chunks per jump workgroup in a pool
 int launch_grid_points(rd_ctx* c, double x0, double y0, double lx, double ly, int nx, int ny,
                        double zz);
 

@@ -124,6 +124,33 @@ def test_planar_fe_run_bit_exact(Diode, oracle_mod):
     assert np.array_equal(dd.life_time(), so.life_time())
 
 
+def test_planar_fe_many_chains_bit_exact(Diode, oracle_mod):
+    """About 2500 chains per step: the accept step of every jump spans three
+    workgroups, whose batch counts set the shared width.  Records and state
+    equal the oracle's."""
+    def setup(t):
+        if hasattr(t, "set_planar"):
+            t.set_planar(8000.0, 1000 * NM, True, 0)
+        else:
+            t.set_geometry(oracle_mod.geom_planar(1000 * NM, 8000.0, True, 0))
+        t.set_time_step(0.25e-15)
+        t.set_seed(31)
+        t.set_emitter(10, 0, -150 * NM, -150 * NM, 300 * NM, 300 * NM, wf=np.array([[4.7]]))
+
+    dd = Diode(8192)
+    so = oracle_mod.Sim(8192, nthreads=8)
+    setup(dd)
+    setup(so)
+    recs = dd.run(10, "fp64")
+    for k in range(1, 11):
+        ro = so.step(k)
+        print(f"many chains step {k}: n_round={recs[k-1].n_round} mh_std={recs[k-1].mh_std}",
+              flush=True)
+        assert_records_equal(recs[k - 1], ro, f"step {k}")
+    assert recs[0].n_round > 2048
+    assert_state_equal(dd.get_state(), so.state(), "end")
+
+
 def test_planar_gtf_run_bit_exact(Diode, oracle_mod):
     """Thermal-field emission, checkerboard work function, Poisson supply,
     Maxwell-Boltzmann velocities."""
