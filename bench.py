@@ -41,7 +41,7 @@ COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per launch
 # calibration copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot be
 # read from inside this process, so the figures are those of the committed
 # summary profiles/snapshot65536_nic10_hbm_counters.txt, for that workload only.
-MEASURED_TRAFFIC = {("k_accel_planar32", 65536, 10): 16.819e6 + 12.583e6}
+MEASURED_TRAFFIC = {("k_accel_sym32", 65536, 10): 39.169e6 + 168.819e6}
 
 
 def cpu_baseline(n_sample: int, nic: int, threads: int):

@@ -37,6 +37,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "rd_field.h"
 #include "rd_internal.h"
@@ -805,9 +806,10 @@ __global__ __launch_bounds__(kBlock) void k_mh_field64(const Ent* tab, int ne,
 
 // Mixed: 64 chains per workgroup, one per lane; the kJumpWaves wavefronts
 // take an equal slice of every 256-source chunk of the segment each (sources
-// staged in LDS, read as wavefront-wide broadcasts), and their FP64 lane sums
-// are added in wavefront order.  Many short slices keep the per-wavefront
-// dependency chain short: a jump is latency-bound, not throughput-bound.
+// staged in LDS and read as wavefront-wide broadcasts, or read through the
+// scalar cache), and their FP64 lane sums are added in wavefront order.  Many
+// short slices keep the per-wavefront dependency chain short: a lone jump is
+// latency-bound (4 wavefronts per workgroup measured slower in pools too).
 constexpr int kJumpTargets = 64;
 constexpr int kJumpWaves = 16;
 constexpr int kJumpSlice = kChunk / kJumpWaves;
@@ -821,17 +823,13 @@ __device__ __forceinline__ float4 load_uniform(const float4* p, int j) {
   return make_float4(q[0], q[1], q[2], q[3]);
 }
 
-// One mixed pair term; `surf`: the wavefront's targets lie on z = 0.
+// One mixed pair term (general series).
 __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, const float4 h2,
                                            const float4 l2, int ic, int nic, float two_d,
-                                           bool surf, double& fx, double& fy, double& fz) {
+                                           double& fx, double& fy, double& fz) {
   float dx = (h1.x - h2.x) + (l1.x - l2.x);
   float dy = (h1.y - h2.y) + (l1.y - l2.y);
   float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
-  if (surf) {
-    fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, 0.5f * two_d));
-    return;
-  }
   float dz = (h1.z - h2.z) + (l1.z - l2.z);
   float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
   float ri = rsq(r2);
@@ -917,25 +915,29 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
     for (int c = c0; c < c1; ++c) {
       const int jb = c * kChunk + wu * kJumpSlice;
       const int je = min(n, jb + kJumpSlice);
-      if (act) {
-        auto term = [&](const float4 h2, const float4 l2) {
-          const float dx = (h1.x - h2.x) + (l1.x - l2.x);
-          const float dy = (h1.y - h2.y) + (l1.y - l2.y);
-          const float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
-          fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, dnm));
-        };
-        int j = jb;
-        for (; j + 4 <= je; j += 4) {  // four sources' loads in flight together
-          float4 hh[4], ll[4];
+      if (act) {  // fz holds the sum without the factor -2 until the end
+        auto sweep = [&](auto n0) {
+          constexpr bool N0 = decltype(n0)::value;
+          int j = jb;
+          for (; j + 4 <= je; j += 4) {  // four sources' loads in flight together
+            float4 hh[4], ll[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            hh[u] = load_uniform(hi, j + u);
-            ll[u] = load_uniform(lo, j + u);
+            for (int u = 0; u < 4; ++u) {
+              hh[u] = load_uniform(hi, j + u);
+              ll[u] = load_uniform(lo, j + u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              fz += (double)surface_term32<N0>(h1, l1, hh[u], ll[u], nic, dnm);
           }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) term(hh[u], ll[u]);
-        }
-        for (; j < je; ++j) term(load_uniform(hi, j), load_uniform(lo, j));
+          for (; j < je; ++j)
+            fz += (double)surface_term32<N0>(h1, l1, load_uniform(hi, j), load_uniform(lo, j), nic,
+                                             dnm);
+        };
+        if (nic == 0)
+          sweep(std::true_type{});
+        else
+          sweep(std::false_type{});
       }
     }
   }
@@ -956,21 +958,20 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
     if (act) {
       if (KIND == RD_GEOM_PLANAR) {
         // one loop per form, so that no branch sits in the source loop
-        if (surf) {
+        if (surf) {  // fz holds the sum without the factor -2 until the end
           const float dnm = 0.5f * two_d;
-          for (int jj = j0; jj < j1; jj += dj) {
-            const float4 h2 = shi[jj], l2 = slo[jj];
-            const float dx = (h1.x - h2.x) + (l1.x - l2.x);
-            const float dy = (h1.y - h2.y) + (l1.y - l2.y);
-            const float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
-            fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, dnm));
-          }
+          if (nic == 0)
+            for (int jj = j0; jj < j1; jj += dj)
+              fz += (double)surface_term32<true>(h1, l1, shi[jj], slo[jj], 0, dnm);
+          else
+            for (int jj = j0; jj < j1; jj += dj)
+              fz += (double)surface_term32<false>(h1, l1, shi[jj], slo[jj], nic, dnm);
         } else if (ic) {
           for (int jj = j0; jj < j1; jj += dj)
-            mixed_pair(h1, l1, shi[jj], slo[jj], 1, nic, two_d, false, fx, fy, fz);
+            mixed_pair(h1, l1, shi[jj], slo[jj], 1, nic, two_d, fx, fy, fz);
         } else {
           for (int jj = j0; jj < j1; jj += dj)
-            mixed_pair(h1, l1, shi[jj], slo[jj], 0, 0, two_d, false, fx, fy, fz);
+            mixed_pair(h1, l1, shi[jj], slo[jj], 0, 0, two_d, fx, fy, fz);
         }
       } else {
         for (int jj = j0; jj < j1; jj += dj) {
@@ -983,6 +984,7 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
       }
     }
   }
+  if (surf) fz = -2.0 * fz;
   red[0][w][lane] = fx;
   red[1][w][lane] = fy;
   red[2][w][lane] = fz;
@@ -1890,8 +1892,8 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
         k_mh_field32<RD_GEOM_PLANAR><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc,
                                                                       c->d_state, V, it, a);
       if (any_tip)
-        k_mh_field32<RD_GEOM_TIP><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
-                                                                   V, it, a);
+        k_mh_field32<RD_GEOM_TIP><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc,
+                                                                   c->d_state, V, it, a);
       k_mh_settle<RD_PREC_MIXED><<<nsettle, kSettle, 0, c->stream>>>(
           dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a);
     }

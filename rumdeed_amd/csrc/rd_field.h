@@ -209,10 +209,12 @@ __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, 
 //   cz = -2 sum_{k=-nic..nic} u_k / (dxy2 + u_k^2)^(3/2),  u_k = z_b + 2kd,
 //   cxy = 0,
 // the same series as ic_planar32 plus the direct term, with 2 nic + 1 distances
-// instead of 4 nic + 2.  zb = z_b, wb = d - z_b and dnm = d (nm); the partners
-// below the cathode are formed from wb, as ic_planar32 forms the anode image.
-__device__ __forceinline__ float surface_pair32(float zb, float wb, float dxy2, int nic,
-                                                float dnm) {
+// instead of 4 nic + 2.  surface_sum32 returns the sum without the factor -2,
+// which the callers apply once to their FP64 total (exactly: a power of two).
+// zb = z_b, wb = d - z_b and dnm = d (nm); the partners below the cathode are
+// formed from wb, as ic_planar32 forms the anode image.
+__device__ __forceinline__ float surface_sum32(float zb, float wb, float dxy2, int nic,
+                                               float dnm) {
   const float two_d = dnm + dnm;
   float w = 0.f;
   {
@@ -231,11 +233,28 @@ __device__ __forceinline__ float surface_pair32(float zb, float wb, float dxy2, 
     w = __builtin_fmaf(um, ri * ri * ri, w);
     um -= two_d;
   }
-  return -2.0f * w;
+  return w;
+}
+
+// q_j/q_0 times surface_sum32 for the packed target (h1, l1) on the cathode
+// and the packed source (h2, l2); N0: the series stops at n = 0 (nic = 0).
+// The source height is its high part alone (the float sum of both parts
+// rounds to it but for ties).
+template <bool N0>
+__device__ __forceinline__ float surface_term32(const float4 h1, const float4 l1, const float4 h2,
+                                                const float4 l2, int nic, float dnm) {
+  const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+  const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+  const float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+  if (N0) {
+    const float ri = rsq(fmaxf(__builtin_fmaf(h2.z, h2.z, dxy2), kR2Min));
+    return h2.w * (h2.z * (ri * ri * ri));
+  }
+  return h2.w * surface_sum32(h2.z, l2.w, dxy2, nic, dnm);
 }
 
 // True when every active target of the wavefront lies on the plane z = 0 and
-// the image series is on: the wavefront may then use surface_pair32.
+// the image series is on: the wavefront may then use surface_term32.
 __device__ __forceinline__ bool wave_on_surface(int ic, bool act, float4 h1, float4 l1) {
   return ic && __all(!act || (h1.z == 0.0f && l1.z == 0.0f));
 }
@@ -359,14 +378,13 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
     }
     __syncthreads();
     const int cnt = min(kChunk, n - jb);
-    if (act && surf) {
-      for (int jj = 0; jj < cnt; ++jj) {
-        const float4 h2 = shi[jj], l2 = slo[jj];
-        float dx = (h1.x - h2.x) + (l1.x - l2.x);
-        float dy = (h1.y - h2.y) + (l1.y - l2.y);
-        float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
-        fz += (double)(h2.w * surface_pair32(h2.z + l2.z, l2.w, dxy2, nic, dnm));
-      }
+    if (act && surf) {  // fz holds the sum without the factor -2 until the end
+      if (nic == 0)
+        for (int jj = 0; jj < cnt; ++jj)
+          fz += (double)surface_term32<true>(h1, l1, shi[jj], slo[jj], 0, dnm);
+      else
+        for (int jj = 0; jj < cnt; ++jj)
+          fz += (double)surface_term32<false>(h1, l1, shi[jj], slo[jj], nic, dnm);
     } else if (act) {
       for (int jj = 0; jj < cnt; ++jj) {
         const float4 h2 = shi[jj], l2 = slo[jj];
@@ -392,6 +410,7 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
       }
     }
   }
+  if (surf) fz = -2.0 * fz;
 }
 
 // Field (V/m) from the segment sums of a mixed evaluation, in segment order.
