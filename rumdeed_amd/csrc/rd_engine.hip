@@ -594,51 +594,76 @@ __device__ __forceinline__ void fe_width_update(SimState& S, int it, int first, 
   }
 }
 
+// What the accept step of one chain reads, loaded together ahead of the
+// field sums so that the loads overlap.
+struct AcceptIn {
+  int ok, pv;            // start found; proposal valid (tip)
+  double px, py, sup;    // proposal (chain coordinates), current log target
+  double lr;             // log-uniform of the jump
+  double x, y, z;        // proposal point
+};
+__device__ __forceinline__ AcceptIn accept_inputs(const Ent& e, const SimDesc& s,
+                                                  const Chains& ch, const double* lr,
+                                                  const double* ptx, const double* pty,
+                                                  const double* ptz, int k) {
+  AcceptIn in{};
+  if (k < 0) return in;
+  const int c = e.cbase + k, p = e.pbase + k;
+  in.ok = ch.ok[c];
+  in.pv = s.mode == RD_EMIT_TIP_FE ? ch.pv[c] : 1;
+  in.px = ch.px[c];
+  in.py = ch.py[c];
+  in.sup = ch.sup[c];
+  in.lr = lr[c];
+  in.x = ptx[p];
+  in.y = pty[p];
+  in.z = ptz[p];
+  return in;
+}
+
 // Accept/reject of jump `it` for chain k of entry e (k < 0: no chain), one
-// chain per lane of a block (a power of two of lanes; ra, rr: LDS arrays of
-// blockDim.x entries for the batch acceptance counts; lr: the log-uniforms of
-// the jump, by chain; (fxk, fyk, fzk): the field at the chain's proposal).
-// Planar FE: with counts == nullptr the block holds every chain and updates
-// the width itself; otherwise it stores its (accepted, rejected) counts there
-// (see split_width).
+// chain per lane of a block (a multiple of 64 lanes; ra, rr: LDS arrays of
+// blockDim.x / 64 entries for the batch acceptance counts; in: the chain's inputs;
+// (fxk, fyk, fzk): the field at the chain's proposal).  Planar FE: with
+// counts == nullptr the block holds every chain and updates the width
+// itself; otherwise it stores its (accepted, rejected) counts there (see
+// split_width).
 __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, const double* wf,
-                                const Chains& ch, int it, const double* ptx, const double* pty,
-                                const double* ptz, double fxk, double fyk, double fzk,
-                                const double* lr, int* ra, int* rr, int k, int* counts) {
+                                const Chains& ch, int it, const AcceptIn& in, double fxk,
+                                double fyk, double fzk, int* ra, int* rr, int k, int* counts) {
   const EmitConsts ec = emit_consts();
   const int first = nint(s.ndim * 0.25);
   const bool tip = s.mode == RD_EMIT_TIP_FE;
   const bool counted = (s.mode == RD_EMIT_PLANAR_GTF) || (it > first);
   int na = 0, nrj = 0;
   const int c = e.cbase + k;
-  if (k >= 0 && ch.ok[c] == 1) {
-    const int p = e.pbase + k;
+  if (k >= 0 && in.ok == 1) {
     bool acc = false;
     bool rejected = false;
     double F = 0.0;
-    if (tip && !ch.pv[c]) {
+    if (tip && !in.pv) {
       rejected = true;
     } else {
-      F = tip ? field_normal(s.g, ptx[p], pty[p], fxk, fyk, fzk) : fzk;
+      F = tip ? field_normal(s.g, in.x, in.y, fxk, fyk, fzk) : fzk;
       bool bad = (s.mode == RD_EMIT_PLANAR_GTF) ? (F > 0.0) : (F >= 0.0);
       if (bad) {
         rejected = true;
       } else {
-        double sup_new = chain_target(s, wf, ec, F, ch.px[c], ch.py[c], ptx[p], pty[p], ptz[p]);
-        double alpha = sup_new - ch.sup[c];
-        if (sup_new >= ch.sup[c]) {
+        double sup_new = chain_target(s, wf, ec, F, in.px, in.py, in.x, in.y, in.z);
+        double alpha = sup_new - in.sup;
+        if (sup_new >= in.sup) {
           acc = true;
         } else {
-          acc = lr[c] <= alpha;
+          acc = in.lr <= alpha;
         }
         if (acc) {
-          ch.cx[c] = ch.px[c];
-          ch.cy[c] = ch.py[c];
+          ch.cx[c] = in.px;
+          ch.cy[c] = in.py;
           ch.sup[c] = sup_new;
           ch.F[c] = F;
-          ch.ox[c] = ptx[p];
-          ch.oy[c] = pty[p];
-          ch.oz[c] = ptz[p];
+          ch.ox[c] = in.x;
+          ch.oy[c] = in.y;
+          ch.oz[c] = in.z;
         } else {
           rejected = true;
         }
@@ -656,22 +681,25 @@ __device__ void mh_accept_block(const Ent& e, const SimDesc& s, SimState* st, co
     }
   }
   if (s.mode != RD_EMIT_PLANAR_FE) return;  // per-chain schedules: no batch update
-  ra[threadIdx.x] = na;
-  rr[threadIdx.x] = nrj;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      ra[threadIdx.x] += ra[threadIdx.x + w];
-      rr[threadIdx.x] += rr[threadIdx.x + w];
-    }
-    __syncthreads();
+  // batch counts: one chain per lane, so a ballot per wavefront, then the
+  // wavefront counts (integers: any order gives the same sums)
+  const unsigned long long ba = __ballot(na), br = __ballot(nrj);
+  if ((threadIdx.x & 63) == 0) {
+    ra[threadIdx.x >> 6] = __popcll(ba);
+    rr[threadIdx.x >> 6] = __popcll(br);
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    int ta = 0, tr = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      ta += ra[w];
+      tr += rr[w];
+    }
     if (counts) {
-      counts[0] = ra[0];
-      counts[1] = rr[0];
+      counts[0] = ta;
+      counts[1] = tr;
     } else {
-      fe_width_update(st[e.slot], it, first, ra[0], rr[0]);
+      fe_width_update(st[e.slot], it, first, ta, tr);
     }
   }
 }
@@ -1014,7 +1042,7 @@ __global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, int ne,
                                                        const SimDesc* desc, SimState* st,
                                                        const double* wf, Chains ch, int it,
                                                        JumpArgs a) {
-  __shared__ int ra[kSettle], rr[kSettle];
+  __shared__ int ra[kSettle / 64], rr[kSettle / 64];
   const Ent e = tab[find_entry(tab, ne, blockIdx.x, true)];
   const SimDesc& s = desc[e.slot];
   if (it > s.ndim) return;
@@ -1026,17 +1054,19 @@ __global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, int ne,
   const double* part = a.part + e.part_off;
   // one chain per lane: k1 - k0 <= kSettle
   const int k = k0 + (int)threadIdx.x < k1 ? k0 + (int)threadIdx.x : -1;
+  const AcceptIn in =
+      accept_inputs(e, s, ch, a.rlr + (size_t)(it - 1) * a.C, a.ptx, a.pty, a.ptz, k);
   double fxk = 0.0, fyk = 0.0, fzk = 0.0;
   if (k >= 0) {
     const int p = e.pbase + k;
     if (PREC == RD_PREC_FP64) {
-      field_finish64(s.g, part, (size_t)e.nchunks_ub * m, m, k, nchunks, a.ptx[p], a.pty[p],
-                     a.ptz[p], fxk, fyk, fzk);
+      field_finish64(s.g, part, (size_t)e.nchunks_ub * m, m, k, nchunks, in.x, in.y, in.z, fxk,
+                     fyk, fzk);
     } else {
       const int segs = nchunks == 0 ? 1 : min(e.segs, (nchunks + e.cps - 1) / e.cps);
       if (s.g.kind == RD_GEOM_TIP)
-        field_finish32_tip(s.g, part, (size_t)e.segs * m, m, k, segs, a.ptx[p], a.pty[p],
-                           a.ptz[p], fxk, fyk, fzk);
+        field_finish32_tip(s.g, part, (size_t)e.segs * m, m, k, segs, in.x, in.y, in.z, fxk,
+                           fyk, fzk);
       else
         field_finish32(s.g, part, (size_t)e.segs * m, m, k, segs, fxk, fyk, fzk);
     }
@@ -1051,8 +1081,7 @@ __global__ __launch_bounds__(kSettle) void k_mh_settle(const Ent* tab, int ne,
     split_counts(e, a.counts, a.nsettle, it - 1, na, nr);
     fe_width_update(st[e.slot], it - 1, nint(s.ndim * 0.25), na, nr);
   }
-  mh_accept_block(e, s, st, wf, ch, it, a.ptx, a.pty, a.ptz, fxk, fyk, fzk,
-                  a.rlr + (size_t)(it - 1) * a.C, ra, rr, k,
+  mh_accept_block(e, s, st, wf, ch, it, in, fxk, fyk, fzk, ra, rr, k,
                   split ? a.counts + (size_t)(it & 1) * 2 * a.nsettle + 2 * (size_t)blockIdx.x
                         : nullptr);
 }
