@@ -1791,20 +1791,19 @@ static int reset_counters(rd_ctx* c, const std::vector<Ent>& ents) {
 // chains in all: the stride of the point and field arrays): per jump, one
 // field launch per geometry present, then the finish-and-accept launch; after
 // the last jump, the width of split planar FE simulations (split_width).
+//
+// In a pool the simulations are dealt into two halves of about equal work,
+// each with its own launch table on its own stream.  Nothing of one half
+// depends on the other during the jumps, so the latency-bound accept step of
+// one half overlaps the field launch of the other.  Every launch still sees
+// each simulation exactly as before, so results do not change.
 static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision,
                      const Chains& V, int maxndim) {
   const bool fp64 = precision == RD_PREC_FP64;
   const int per = fp64 ? kBlock : kJumpTargets;  // chains per workgroup
   std::vector<Ent> ents;
-  bool any_tip = false, any_planar = false;
-  for (const Ent& e : in) {
-    if (e.nch <= 0) continue;
-    if (c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
-      any_tip = true;
-    else
-      any_planar = true;
-    ents.push_back(e);
-  }
+  for (const Ent& e : in)
+    if (e.nch > 0) ents.push_back(e);
   if (ents.empty()) return RD_OK;
   JumpArgs a;
   std::memset(&a, 0, sizeof(a));  // its bytes form part of the graph key
@@ -1827,10 +1826,34 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   }
   a.HI = (const float4*)c->pack_hi.p;
   a.LO = (const float4*)c->pack_lo.p;
-  long long off = 0;
-  int nwg = 0;     // workgroups of the compact field launch
-  int nsettle = 0;  // workgroups of the finish-and-accept launch
-  bool any_split = false;
+
+  // halves: a prefix and the rest of the table, split at half the work
+  struct Half {
+    std::vector<Ent> ents;
+    int nwg = 0, nsettle = 0;
+    bool any_split = false, any_planar = false, any_tip = false;
+    Ent* dt = nullptr;
+    JumpArgs a;
+    hipStream_t stream = nullptr;
+  };
+  const bool two = c->nslots_max > 1 && !c->graphs && ents.size() >= 2;
+  Half hv[2];
+  int nh = 1;
+  if (two) {
+    double total = 0.0, acc = 0.0;
+    for (const Ent& e : ents) total += (double)e.nch * std::max(e.n_ub, 1);
+    size_t cut = 1;
+    for (size_t k = 0; k + 1 < ents.size(); ++k) {
+      acc += (double)ents[k].nch * std::max(ents[k].n_ub, 1);
+      cut = k + 1;
+      if (acc >= 0.5 * total) break;
+    }
+    hv[0].ents.assign(ents.begin(), ents.begin() + cut);
+    hv[1].ents.assign(ents.begin() + cut, ents.end());
+    nh = 2;
+  } else {
+    hv[0].ents = ents;
+  }
   // A context made for one simulation spreads each jump's sums over up to
   // kWantJump workgroups (latency); in a pool, whose launches serve many
   // simulations, every mixed workgroup takes at least kPoolJumpChunks chunks
@@ -1838,60 +1861,86 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   // the finer split).  Either way the split depends on the simulation's own
   // size and the kind of context only.
   const int min_cps = (c->nslots_max > 1 && !fp64) ? kPoolJumpChunks : 1;
-  for (Ent& e : ents) {
-    e.wg1 = nsettle;
-    nsettle += settle_groups(e.nch);
-    if (settle_groups(e.nch) > 1 && c->slots[e.slot].desc.mode == RD_EMIT_PLANAR_FE)
-      any_split = true;
-    const int nch = (e.n_ub + kChunk - 1) / kChunk;
-    const int gx = fp64 ? (e.nch + per - 1) / per : jump_groups32(e.nch);
-    e.nchunks_ub = nch;
-    segment(nch, seg_limit(gx, kWantJump), e.segs, e.cps);
-    if (e.cps < min_cps) {
-      e.cps = std::min(min_cps, std::max(nch, 1));
-      e.segs = (std::max(nch, 1) + e.cps - 1) / e.cps;
+  long long off = 0;
+  int nsettle_all = 0;
+  for (int h = 0; h < nh; ++h) {
+    Half& H = hv[h];
+    for (Ent& e : H.ents) {
+      if (c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
+        H.any_tip = true;
+      else
+        H.any_planar = true;
+      e.wg1 = H.nsettle;
+      H.nsettle += settle_groups(e.nch);
+      if (settle_groups(e.nch) > 1 && c->slots[e.slot].desc.mode == RD_EMIT_PLANAR_FE)
+        H.any_split = true;
+      const int nch = (e.n_ub + kChunk - 1) / kChunk;
+      const int gx = fp64 ? (e.nch + per - 1) / per : jump_groups32(e.nch);
+      e.nchunks_ub = nch;
+      segment(nch, seg_limit(gx, kWantJump), e.segs, e.cps);
+      if (e.cps < min_cps) {
+        e.cps = std::min(min_cps, std::max(nch, 1));
+        e.segs = (std::max(nch, 1) + e.cps - 1) / e.cps;
+      }
+      e.part_off = off;
+      off += 3LL * (fp64 ? std::max(nch, 1) : e.segs) * e.nch;
+      e.wg0 = H.nwg;
+      H.nwg += gx * e.segs;
     }
-    e.part_off = off;
-    off += 3LL * (fp64 ? std::max(nch, 1) : e.segs) * e.nch;
-    e.wg0 = nwg;
-    nwg += gx * e.segs;
+    nsettle_all += H.nsettle;
   }
-  const int ne = (int)ents.size();
   r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
   if (r) return r;
   const size_t nr = (size_t)maxndim * C;
   r = ensure(c, c->rnd, sizeof(double) * 3 * nr);
   if (r) return r;
-  r = ensure(c, c->cnt, sizeof(int) * 4 * (size_t)nsettle);
+  r = ensure(c, c->cnt, sizeof(int) * 4 * (size_t)nsettle_all);
   if (r) return r;
-  int* counts = (int*)c->cnt.p;
-  a.counts = counts;
-  a.nsettle = nsettle;
-  Ent* dt = nullptr;
-  r = upload_tab(c, ents, &dt);
-  if (r) return r;
+  if (two && !c->stream2) {
+    RD_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    RD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    RD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
   double* rb = (double*)c->rnd.p;
   a.ru1 = rb;
   a.ru2 = rb + nr;
   a.rlr = rb + 2 * nr;
   a.C = C;
   a.part = (double*)c->part.p;
-  const int grid = nwg;
+  Ent* dall = nullptr;  // every entry, for the random numbers of all chains
+  if (nh == 2) {
+    r = upload_tab(c, ents, &dall);
+    if (r) return r;
+  }
+  int cnt_off = 0;
+  for (int h = 0; h < nh; ++h) {
+    Half& H = hv[h];
+    r = upload_tab(c, H.ents, &H.dt);
+    if (r) return r;
+    H.a = a;
+    H.a.counts = (int*)c->cnt.p + cnt_off;
+    H.a.nsettle = H.nsettle;
+    cnt_off += 4 * H.nsettle;
+    H.stream = h == 0 ? c->stream : c->stream2;
+  }
+  if (nh == 1) dall = hv[0].dt;
+  const int ne_all = (int)ents.size();
   // Everything the launches below depend on besides device data.  With graphs
   // on, an unchanged key replays the captured loop (the kernels read the
   // launch table and the chains from device memory, so new contents under
-  // the same pointers are picked up).
+  // the same pointers are picked up).  Graphs use one stream (nh == 1).
+  const Half& G = hv[0];
   std::vector<unsigned char> key;
   auto put = [&key](const void* v, size_t bytes) {
     const unsigned char* b = (const unsigned char*)v;
     key.insert(key.end(), b, b + bytes);
   };
-  const int ints[] = {ne, grid, nsettle, (int)any_split, (int)any_planar, (int)any_tip, (int)fp64,
-                      maxndim, C};
-  const void* ptrs[] = {dt, c->d_desc, c->d_state, c->d_wf, counts};
+  const int ints[] = {(int)G.ents.size(), G.nwg, G.nsettle, (int)G.any_split,
+                      (int)G.any_planar, (int)G.any_tip, (int)fp64, maxndim, C};
+  const void* ptrs[] = {G.dt, dall, c->d_desc, c->d_state, c->d_wf};
   put(ints, sizeof(ints));
   put(ptrs, sizeof(ptrs));
-  put(&a, sizeof(a));
+  put(&G.a, sizeof(G.a));
   put(&V, sizeof(V));
   if (c->graphs && c->jump_exec && key == c->jump_key) {
     RD_HIP(hipGraphLaunch(c->jump_exec, c->stream));
@@ -1904,31 +1953,47 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     c->jump_key.clear();
     RD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   }
-  k_mh_random<<<dim3((C + 255) / 256, maxndim), 256, 0, c->stream>>>(dt, ne, c->d_desc, C, rb,
-                                                                       rb + nr, rb + 2 * nr);
+  k_mh_random<<<dim3((C + 255) / 256, maxndim), 256, 0, c->stream>>>(dall, ne_all, c->d_desc, C,
+                                                                       rb, rb + nr, rb + 2 * nr);
+  if (nh == 2) {
+    RD_HIP(hipEventRecord(c->ev_fork, c->stream));
+    RD_HIP(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+  }
   for (int it = 1; it <= maxndim; ++it) {
-    if (fp64) {
-      if (any_planar)
-        k_mh_field64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc,
-                                                                      c->d_state, V, it, a);
-      if (any_tip)
-        k_mh_field64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
-                                                                   V, it, a);
-      k_mh_settle<RD_PREC_FP64><<<nsettle, kSettle, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a);
-    } else {
-      if (any_planar)
-        k_mh_field32<RD_GEOM_PLANAR><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc,
-                                                                      c->d_state, V, it, a);
-      if (any_tip)
-        k_mh_field32<RD_GEOM_TIP><<<grid, 64 * kJumpWaves, 0, c->stream>>>(dt, ne, c->d_desc,
-                                                                   c->d_state, V, it, a);
-      k_mh_settle<RD_PREC_MIXED><<<nsettle, kSettle, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, a);
+    for (int h = 0; h < nh; ++h) {
+      const Half& H = hv[h];
+      const int ne = (int)H.ents.size();
+      if (fp64) {
+        if (H.any_planar)
+          k_mh_field64<RD_GEOM_PLANAR><<<H.nwg, kBlock, 0, H.stream>>>(
+              H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+        if (H.any_tip)
+          k_mh_field64<RD_GEOM_TIP><<<H.nwg, kBlock, 0, H.stream>>>(H.dt, ne, c->d_desc,
+                                                                     c->d_state, V, it, H.a);
+        k_mh_settle<RD_PREC_FP64><<<H.nsettle, kSettle, 0, H.stream>>>(
+            H.dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, H.a);
+      } else {
+        if (H.any_planar)
+          k_mh_field32<RD_GEOM_PLANAR><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+              H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+        if (H.any_tip)
+          k_mh_field32<RD_GEOM_TIP><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+              H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+        k_mh_settle<RD_PREC_MIXED><<<H.nsettle, kSettle, 0, H.stream>>>(
+            H.dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, H.a);
+      }
     }
   }
-  if (any_split)
-    k_mh_width_last<<<ne, 64, 0, c->stream>>>(dt, c->d_desc, c->d_state, counts, nsettle);
+  for (int h = 0; h < nh; ++h) {
+    const Half& H = hv[h];
+    if (H.any_split)
+      k_mh_width_last<<<(int)H.ents.size(), 64, 0, H.stream>>>(H.dt, c->d_desc, c->d_state,
+                                                               H.a.counts, H.nsettle);
+  }
+  if (nh == 2) {
+    RD_HIP(hipEventRecord(c->ev_join, c->stream2));
+    RD_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  }
   if (c->graphs) {
     hipGraph_t g = nullptr;
     const hipError_t le = hipGetLastError();

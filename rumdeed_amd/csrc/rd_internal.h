@@ -186,6 +186,10 @@ struct rd_ctx {
   int graphs = 0;
   hipGraphExec_t jump_exec = nullptr;
   std::vector<unsigned char> jump_key;
+
+  // second stream of the pool jump loop (default priority) and its fork/join
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace rd {
