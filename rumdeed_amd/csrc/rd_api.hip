@@ -383,8 +383,10 @@ int rd_ctx_destroy(rd_ctx* c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev2) hipEventDestroy(c->ev2);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
-  if (c->ev_join) hipEventDestroy(c->ev_join);
-  if (c->stream2) hipStreamDestroy(c->stream2);
+  for (int h = 0; h < kJumpStreams - 1; ++h) {
+    if (c->ev_join[h]) hipEventDestroy(c->ev_join[h]);
+    if (c->jstream[h]) hipStreamDestroy(c->jstream[h]);
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return RD_OK;

@@ -1792,11 +1792,11 @@ static int reset_counters(rd_ctx* c, const std::vector<Ent>& ents) {
 // field launch per geometry present, then the finish-and-accept launch; after
 // the last jump, the width of split planar FE simulations (split_width).
 //
-// In a pool the simulations are dealt into two halves of about equal work,
-// each with its own launch table on its own stream.  Nothing of one half
-// depends on the other during the jumps, so the latency-bound accept step of
-// one half overlaps the field launch of the other.  Every launch still sees
-// each simulation exactly as before, so results do not change.
+// In a pool the simulations are dealt into up to kJumpStreams parts of about
+// equal work, each with its own launch table on its own stream.  Nothing of
+// one part depends on another during the jumps, so the latency-bound accept
+// step of one part overlaps the field launches of the others.  Every launch
+// still sees each simulation exactly as before, so results do not change.
 static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision,
                      const Chains& V, int maxndim) {
   const bool fp64 = precision == RD_PREC_FP64;
@@ -1827,7 +1827,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   a.HI = (const float4*)c->pack_hi.p;
   a.LO = (const float4*)c->pack_lo.p;
 
-  // halves: a prefix and the rest of the table, split at half the work
+  // parts: consecutive runs of the table of about equal work
   struct Half {
     std::vector<Ent> ents;
     int nwg = 0, nsettle = 0;
@@ -1836,23 +1836,26 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     JumpArgs a;
     hipStream_t stream = nullptr;
   };
-  const bool two = c->nslots_max > 1 && !c->graphs && ents.size() >= 2;
-  Half hv[2];
-  int nh = 1;
-  if (two) {
+  const int nparts = (c->nslots_max > 1 && !c->graphs)
+                         ? std::min<int>(kJumpStreams, (int)ents.size())
+                         : 1;
+  Half hv[kJumpStreams];
+  int nh = 0;
+  {
     double total = 0.0, acc = 0.0;
     for (const Ent& e : ents) total += (double)e.nch * std::max(e.n_ub, 1);
-    size_t cut = 1;
-    for (size_t k = 0; k + 1 < ents.size(); ++k) {
-      acc += (double)ents[k].nch * std::max(ents[k].n_ub, 1);
-      cut = k + 1;
-      if (acc >= 0.5 * total) break;
+    size_t k = 0;
+    for (int h = 0; h < nparts; ++h) {
+      // entries until this part reaches its share; leave one per later part
+      const size_t keep = (size_t)(nparts - 1 - h);
+      while (k < ents.size() - keep) {
+        hv[h].ents.push_back(ents[k]);
+        acc += (double)ents[k].nch * std::max(ents[k].n_ub, 1);
+        ++k;
+        if (h + 1 < nparts && acc >= total * (h + 1) / nparts) break;
+      }
+      if (!hv[h].ents.empty()) nh = h + 1;
     }
-    hv[0].ents.assign(ents.begin(), ents.begin() + cut);
-    hv[1].ents.assign(ents.begin() + cut, ents.end());
-    nh = 2;
-  } else {
-    hv[0].ents = ents;
   }
   // A context made for one simulation spreads each jump's sums over up to
   // kWantJump workgroups (latency); in a pool, whose launches serve many
@@ -1896,10 +1899,12 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   if (r) return r;
   r = ensure(c, c->cnt, sizeof(int) * 4 * (size_t)nsettle_all);
   if (r) return r;
-  if (two && !c->stream2) {
-    RD_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  if (nh > 1 && !c->jstream[0]) {
     RD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    RD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    for (int h = 0; h < kJumpStreams - 1; ++h) {
+      RD_HIP(hipStreamCreateWithFlags(&c->jstream[h], hipStreamNonBlocking));
+      RD_HIP(hipEventCreateWithFlags(&c->ev_join[h], hipEventDisableTiming));
+    }
   }
   double* rb = (double*)c->rnd.p;
   a.ru1 = rb;
@@ -1908,7 +1913,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   a.C = C;
   a.part = (double*)c->part.p;
   Ent* dall = nullptr;  // every entry, for the random numbers of all chains
-  if (nh == 2) {
+  if (nh > 1) {
     r = upload_tab(c, ents, &dall);
     if (r) return r;
   }
@@ -1921,7 +1926,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     H.a.counts = (int*)c->cnt.p + cnt_off;
     H.a.nsettle = H.nsettle;
     cnt_off += 4 * H.nsettle;
-    H.stream = h == 0 ? c->stream : c->stream2;
+    H.stream = h == 0 ? c->stream : c->jstream[h - 1];
   }
   if (nh == 1) dall = hv[0].dt;
   const int ne_all = (int)ents.size();
@@ -1955,9 +1960,9 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   }
   k_mh_random<<<dim3((C + 255) / 256, maxndim), 256, 0, c->stream>>>(dall, ne_all, c->d_desc, C,
                                                                        rb, rb + nr, rb + 2 * nr);
-  if (nh == 2) {
+  if (nh > 1) {
     RD_HIP(hipEventRecord(c->ev_fork, c->stream));
-    RD_HIP(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    for (int h = 1; h < nh; ++h) RD_HIP(hipStreamWaitEvent(hv[h].stream, c->ev_fork, 0));
   }
   for (int it = 1; it <= maxndim; ++it) {
     for (int h = 0; h < nh; ++h) {
@@ -1990,9 +1995,9 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
       k_mh_width_last<<<(int)H.ents.size(), 64, 0, H.stream>>>(H.dt, c->d_desc, c->d_state,
                                                                H.a.counts, H.nsettle);
   }
-  if (nh == 2) {
-    RD_HIP(hipEventRecord(c->ev_join, c->stream2));
-    RD_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  for (int h = 1; h < nh; ++h) {
+    RD_HIP(hipEventRecord(c->ev_join[h - 1], hv[h].stream));
+    RD_HIP(hipStreamWaitEvent(c->stream, c->ev_join[h - 1], 0));
   }
   if (c->graphs) {
     hipGraph_t g = nullptr;

@@ -122,6 +122,7 @@ __device__ __forceinline__ int find_entry(const Ent* tab, int ne, int b, bool se
 }
 
 constexpr int kChunk = 256;  // canonical FP64 summation chunk
+constexpr int kJumpStreams = 2;  // streams of a pool's jump loop (4 measured slower: launch-bound)
 constexpr int kBlock = 256;
 constexpr int kMaxSlots = 4096;
 constexpr int kLife = 1000;  // MAX_LIFE_TIME
@@ -187,9 +188,10 @@ struct rd_ctx {
   hipGraphExec_t jump_exec = nullptr;
   std::vector<unsigned char> jump_key;
 
-  // second stream of the pool jump loop (default priority) and its fork/join
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // further streams of the pool jump loop (default priority) and the events
+  // that fork them from and join them to `stream`
+  hipStream_t jstream[rd::kJumpStreams - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[rd::kJumpStreams - 1] = {};
 };
 
 namespace rd {
