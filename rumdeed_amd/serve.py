@@ -125,9 +125,11 @@ class Server:
                       wf=np.array([[r.phi]]))
 
     def run(self, requests, realtime: bool = True, keep_records: bool = False,
-            progress=None):
+            progress=None, on_step=None):
         """Serve `requests`; arrival times are honoured against the wall clock
-        when `realtime` (otherwise every request is available at once)."""
+        when `realtime` (otherwise every request is available at once).
+        progress(result, n_done) runs as each request completes, on_step(
+        pool_steps, n_active) after every pool step."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         active = {}   # slot -> (request, result, footprint, steps done)
         results = []
@@ -163,6 +165,8 @@ class Server:
             recs = self.pool.step(self.precision)
             loop_steps += 1
             occupancy.append(len(active))
+            if on_step:
+                on_step(loop_steps, len(active))
             now = time.perf_counter() - t0
             for slot in list(active):
                 r, res, fp, done, n_prev = active[slot]
