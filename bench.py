@@ -14,7 +14,10 @@ Prints one JSON line.  The roofline object is for the dominant kernel (the
 mixed all-pairs acceleration kernel), timed with HIP events on the stream it
 is launched on.  cpu_baseline times the reference's own OpenMP pair loop
 (compiled from its sources into oracle/_ref) on a bounded sample on the host
-cores, or the oracle restatement when that library is absent.
+cores, or the oracle restatement when that library is absent.  `serving`
+covers the metric's batched half on a short cut of the serving stream (64
+requests of 200 steps by default): sims/hr and completion latency with
+Poisson arrivals, and the mixed / FP64 throughput ratio.
 """
 from __future__ import annotations
 
@@ -99,7 +102,11 @@ def serve_bench(n, precision, realtime, steps, rate, graphs=False):
                   flush=True)
         done.append(res)
 
-    results, stats = srv.run(reqs, realtime=realtime, progress=progress)
+    def on_step(k, n_active):
+        if k % 200 == 0:
+            print(f"[serve] pool step {k}, {n_active} active", file=sys.stderr, flush=True)
+
+    results, stats = srv.run(reqs, realtime=realtime, progress=progress, on_step=on_step)
     srv.close()
     emitted = np.array([r.emitted for r in results])
     nfin = np.array([r.n_final for r in results])
@@ -185,6 +192,22 @@ def e2e_bench(name, steps, precision, graphs=False):
     return out
 
 
+def serving_summary(n, steps):
+    """The metric's batched half on a short cut of the serving stream: n
+    requests of `steps` steps.  Mixed precision with Poisson arrivals at 4/s
+    (sims/hr and completion latency); then every request admitted at once in
+    both precisions, for the throughput ratio."""
+    rt = serve_bench(n, "mixed", True, steps, 4.0)
+    tm = serve_bench(n, "mixed", False, steps, 4.0)
+    t64 = serve_bench(n, "fp64", False, steps, 4.0)
+    return {"requests": n, "steps_per_request": steps, "arrival_rate_per_s": 4.0,
+            "mixed_poisson": {k: rt[k] for k in ("sims_per_hour", "latency_p50_s",
+                                                  "latency_p99_s", "makespan_s", "max_active")},
+            "sims_per_hour_all_admitted": {"mixed": tm["sims_per_hour"],
+                                           "fp64": t64["sims_per_hour"]},
+            "mixed_over_fp64": tm["sims_per_hour"] / t64["sims_per_hour"]}
+
+
 def profile_pass(n, nic, G, reps):
     """A short run for hardware-counter passes: `reps` launches of each
     snapshot kernel, then a plain 1 GiB device-to-device copy (three times)
@@ -234,6 +257,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-summary", type=int, default=64,
+                    help="requests of the short serving cut in the default line (0: none)")
+    ap.add_argument("--serve-summary-steps", type=int, default=200)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -328,13 +354,15 @@ def main():
                      "flops_per_interaction": W.pair_flops(nic),
                      "pairs_per_launch": n * (n - 1) // (2 if kernel == "k_accel_sym32" else 1)},
     }
+    dd.close()
+    if args.serve_summary > 0:
+        out["serving"] = serving_summary(args.serve_summary, args.serve_summary_steps)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_sample, nic, threads)
         except Exception as e:  # reported, never substituted for the GPU number
             out["cpu_baseline"] = {"value": None, "error": str(e)}
-    dd.close()
     print(json.dumps(out), flush=True)
 
 
