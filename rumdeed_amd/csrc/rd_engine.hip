@@ -1592,28 +1592,43 @@ __global__ __launch_bounds__(256) void k_velocity(const Ent* tab, const SimDesc*
     red[((size_t)blockIdx.z * e.nblk + blockIdx.x) * 16 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
+// Block sums of k_velocity added in block order: lane k < 13 takes sum k,
+// with its loads issued eight blocks at a time.
 __global__ void k_velocity_finish(const Ent* tab, SimState* st, const double* red) {
   const Ent e = tab[blockIdx.x];
-  if (threadIdx.x != 0) return;
+  const int k = threadIdx.x;
+  if (k >= 13) return;
   SimState& S = st[e.slot];
-  const int nb = max(1, (S.n + 255) / 256);
-  double acc[13];
-  for (int k = 0; k < 13; ++k) acc[k] = 0.0;
-  for (int b = 0; b < nb && b < e.nblk; ++b)
-    for (int k = 0; k < 13; ++k) acc[k] = acc[k] + red[((size_t)blockIdx.x * e.nblk + b) * 16 + k];
-  for (int k = 0; k < 3; ++k) S.ramo[k] = acc[k];
-  for (int k = 0; k < 9; ++k) S.vsum[k] = acc[3 + k];
-  S.n_elec = (int)acc[12];  // electrons updated this step
+  const int nb = min(max(1, (S.n + 255) / 256), e.nblk);
+  const double* r = red + (size_t)blockIdx.x * e.nblk * 16 + k;
+  double acc = 0.0;
+  int b = 0;
+  for (; b + 8 <= nb; b += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = r[(size_t)(b + u) * 16];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = acc + v[u];
+  }
+  for (; b < nb; ++b) acc = acc + r[(size_t)b * 16];
+  if (k < 3)
+    S.ramo[k] = acc;
+  else if (k < 12)
+    S.vsum[k - 3] = acc;
+  else
+    S.n_elec = (int)acc;  // electrons updated this step
 }
 
 // ---- order-preserving compaction (one block of 1024 per simulation) ------
+// Chunks of 1024 particles in index order; survivors move down to the running
+// offset, keeping their order (Remove_Particles, src/mod_pair.F90:352-562).
 __global__ __launch_bounds__(1024) void k_compact(
     const Ent* tab, const SimDesc* desc, SimState* st, int32_t* life, double* X, double* Y,
     double* Z, double* PX, double* PY, double* PZ, double* VX, double* VY, double* VZ, double* AX,
     double* AY, double* AZ, double* A1X, double* A1Y, double* A1Z, double* A2X, double* A2Y,
     double* A2Z, double* Q, double* M, int32_t* SP, int32_t* STEP, int32_t* EMIT, int32_t* SEC,
     int32_t* ID, int32_t* ALIVE) {
-  __shared__ int scan[1024];
+  __shared__ int wsum[1024 / 64];
   __shared__ int s_off;
   const Ent e = tab[blockIdx.x];
   SimState& S = st[e.slot];
@@ -1648,16 +1663,17 @@ __global__ __launch_bounds__(1024) void k_compact(
         if (spc >= 1 && spc <= 2) atomicAdd(&life[((size_t)e.slot * 2 + (spc - 1)) * kLife + (lt - 1)], 1);
       }
     }
-    scan[threadIdx.x] = flag;
+    // rank among the chunk's survivors: a ballot per wavefront, then the
+    // survivors of the lower wavefronts
+    const unsigned long long mask = __ballot(flag);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) wsum[wv] = __popcll(mask);
     __syncthreads();
-    for (int w = 1; w < 1024; w <<= 1) {
-      int v = (int)threadIdx.x >= w ? scan[threadIdx.x - w] : 0;
-      __syncthreads();
-      scan[threadIdx.x] += v;
-      __syncthreads();
+    int rank = __popcll(mask & ((1ull << lane) - 1ull)), total = 0;
+    for (int u = 0; u < 1024 / 64; ++u) {
+      if (u < wv) rank += wsum[u];
+      total += wsum[u];
     }
-    const int total = scan[1023];
-    const int rank = scan[threadIdx.x] - flag;
     const int off = s_off;
     __syncthreads();  // every read of this chunk is done before any write
     if (flag) {
@@ -1761,7 +1777,7 @@ int engine_velocity(rd_ctx* c, const std::vector<Ent>& in) {
   k_velocity<<<dim3(nb, 1, ents.size()), 256, 0, c->stream>>>(
       dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az,
       c->a1x, c->a1y, c->a1z, c->a2x, c->a2y, c->a2z, c->q, c->species, (double*)c->red.p);
-  k_velocity_finish<<<ents.size(), 1, 0, c->stream>>>(dt, c->d_state, (const double*)c->red.p);
+  k_velocity_finish<<<ents.size(), 64, 0, c->stream>>>(dt, c->d_state, (const double*)c->red.p);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }

@@ -264,6 +264,34 @@ __device__ __forceinline__ bool wave_on_surface(int ic, bool act, float4 h1, flo
 // lane of the workgroup takes part in the LDS staging whether or not it has
 // a target (`act`), so the calls sit in uniform control flow.
 
+// s_q += P[q * S + k * ld] for q = 0, 1, 2 and k = 0 .. n-1, in k order (the
+// same additions as a plain loop); the loads are issued eight chunks at a
+// time, so that their latencies overlap instead of adding up.
+__device__ __forceinline__ void sum3_in_order(const double* __restrict__ P, size_t S, size_t ld,
+                                              int n, double& s0, double& s1, double& s2) {
+  int k = 0;
+  for (; k + 8 <= n; k += 8) {
+    double a[8], b[8], c[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = P[0 * S + (size_t)(k + u) * ld];
+      b[u] = P[1 * S + (size_t)(k + u) * ld];
+      c[u] = P[2 * S + (size_t)(k + u) * ld];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 = s0 + a[u];
+      s1 = s1 + b[u];
+      s2 = s2 + c[u];
+    }
+  }
+  for (; k < n; ++k) {
+    s0 = s0 + P[0 * S + (size_t)k * ld];
+    s1 = s1 + P[1 * S + (size_t)k * ld];
+    s2 = s2 + P[2 * S + (size_t)k * ld];
+  }
+}
+
 // FP64: one partial per 256-source chunk, P[k * S + c * m + p] for the
 // components k = 0, 1, 2 (the canonical order: sources within a chunk in
 // index order, chunk partials added in chunk order by the finishing step).
@@ -342,11 +370,7 @@ __device__ __forceinline__ void field_finish64(const DevGeom& g, const double* _
                                                double y, double z, double& fx, double& fy,
                                                double& fz) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
-    sx = sx + P[0 * S + (size_t)c * m + p];
-    sy = sy + P[1 * S + (size_t)c * m + p];
-    sz = sz + P[2 * S + (size_t)c * m + p];
-  }
+  sum3_in_order(P + p, S, m, nchunks, sx, sy, sz);
   double ex, ey, ez;
   vacuum64(g, x, y, z, ex, ey, ez);
   fx = ex + sx;
@@ -418,11 +442,7 @@ __device__ __forceinline__ void field_finish32(const DevGeom& g, const double* _
                                                size_t S, int m, int p, int segs, double& fx,
                                                double& fy, double& fz) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int s = 0; s < segs; ++s) {
-    sx += P[0 * S + (size_t)s * m + p];
-    sy += P[1 * S + (size_t)s * m + p];
-    sz += P[2 * S + (size_t)s * m + p];
-  }
+  sum3_in_order(P + p, S, m, segs, sx, sy, sz);
   double pre = g.dfc * Consts::q_0 * 1.0e18;
   fx = pre * sx;
   fy = pre * sy;
@@ -436,11 +456,7 @@ __device__ __forceinline__ void field_finish32_tip(const DevGeom& g, const doubl
                                                    double y, double z, double& fx, double& fy,
                                                    double& fz) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int s = 0; s < segs; ++s) {
-    sx += P[0 * S + (size_t)s * m + p];
-    sy += P[1 * S + (size_t)s * m + p];
-    sz += P[2 * S + (size_t)s * m + p];
-  }
+  sum3_in_order(P + p, S, m, segs, sx, sy, sz);
   double ex, ey, ez;
   tip_vacuum64(g, x, y, z, ex, ey, ez);
   const double pre = g.dfc * Consts::q_0 * 1.0e18;

@@ -212,11 +212,7 @@ __global__ void k_accel_finish64(const Ent* tab, const SimDesc* desc, const SimS
   const double* P = part + e.part_off;
   const size_t S = (size_t)e.nchunks_ub * e.n_ub;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
-    sx = sx + P[0 * S + (size_t)c * e.n_ub + i];
-    sy = sy + P[1 * S + (size_t)c * e.n_ub + i];
-    sz = sz + P[2 * S + (size_t)c * e.n_ub + i];
-  }
+  sum3_in_order(P + i, S, e.n_ub, nchunks, sx, sy, sz);
   const int gi = e.base + i;
   double q1 = Q[gi];
   double im = 1.0 / M[gi];
@@ -574,11 +570,7 @@ __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimS
   const double* P = part + e.part_off;
   const size_t S = (size_t)e.segs * e.n_ub;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int s = 0; s < segs; ++s) {
-    sx += P[0 * S + (size_t)s * e.n_ub + i];
-    sy += P[1 * S + (size_t)s * e.n_ub + i];
-    sz += P[2 * S + (size_t)s * e.n_ub + i];
-  }
+  sum3_in_order(P + i, S, e.n_ub, segs, sx, sy, sz);
   const int gi = e.base + i;
   double q1 = Q[gi];
   double im = 1.0 / M[gi];
