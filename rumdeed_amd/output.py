@@ -1,0 +1,177 @@
+"""RUMDEED's per-step output files, written from the device step records.
+
+The reference's analysis scripts read a run's ``out/`` directory.  This module
+writes the files they read for the emission modes built here, from the
+records every ``rd_run`` / ``rd_pool_step`` returns (and the lifetime
+histogram), with the reference's Fortran edit descriptors:
+
+- ``emitted.dt``: Do_Field_Emission (src/mod_field_emission_v2.F90:202-204),
+  Do_Field_Thermo_Emission (src/mod_field_thermo_emission.F90:130-132),
+  Do_Emission_Tip (src/mod_emission_tip.f90:174-175);
+- ``field.dt``: src/mod_field_emission_v2.F90:389-390 (planar FE) and
+  src/mod_field_thermo_emission.F90:189-190 (GTF); the tip FE writes none;
+- ``absorbed.dt``, ``absorbed_top.dt``, ``absorbed_bot.dt``,
+  ``absorbed_recom.dt``: Write_Absorbed (src/mod_pair.F90:790-805);
+- ``ramo_current.dt``: Write_Ramo_Current (src/mod_pair.F90:812-837), with
+  the mobility of Average_Velocities (src/mod_verlet.F90:425-446);
+- ``lifetime.dt``: Write_Life_Time (src/mod_pair.F90:774-784);
+- ``init.bin``: Write_Initial_Variables (src/main.F90:928-935).
+
+Counts follow the reference's moments of writing: ``emitted.dt`` and
+``ramo_current.dt`` see the particles after emission and before removal
+(n at the end of the step plus the absorbed ones).  Only electrons are
+emitted by these modes, so ion columns are zero and the particle and
+electron columns agree.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+# src/mod_global.F90
+EPSILON_R = 1.0
+M_EEFF = 1.0
+M_IEFF = 1.0
+LENGTH_SCALE = 1.0e-9
+TIME_SCALE = 1.0e-12
+VEL_SCALE = LENGTH_SCALE / TIME_SCALE
+CUR_SCALE = 1.0
+MAX_EMITTERS = 1
+MAX_SECTIONS = 96 * 96
+MAX_LIFE_TIME = 1000
+
+EMIT_TIP_FE, EMIT_PLANAR_GTF, EMIT_PLANAR_FE = 3, 9, 10
+
+
+# ---- Fortran edit descriptors ---------------------------------------------
+def fortran_I(v: int, w: int) -> str:
+    """Iw: right-justified in w columns, asterisks when it does not fit."""
+    s = str(int(v))
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+def _exponent(e: int) -> str:
+    """The exponent field of Ew.d / ESw.d: E+dd, or +ddd without the E when
+    |e| > 99."""
+    if abs(e) <= 99:
+        return f"E{'-' if e < 0 else '+'}{abs(e):02d}"
+    return f"{'-' if e < 0 else '+'}{abs(e):03d}"
+
+
+def fortran_E(x: float, w: int, d: int) -> str:
+    """Ew.d: 0.ddd...E+ee with d significant digits, right-justified."""
+    if not math.isfinite(x):
+        s = "NaN" if math.isnan(x) else ("-Infinity" if x < 0 else "Infinity")
+        return s.rjust(w) if len(s) <= w else "*" * w
+    if x == 0.0:
+        s = ("-" if math.copysign(1.0, x) < 0 else "") + "0." + "0" * d + "E+00"
+    else:
+        m, e = f"{abs(x):.{d - 1}e}".split("e")
+        digits = m.replace(".", "")
+        s = ("-" if x < 0 else "") + "0." + digits + _exponent(int(e) + 1)
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+def fortran_ES(x: float, w: int, d: int) -> str:
+    """ESw.d: d.ddd...E+ee (scientific), right-justified."""
+    if not math.isfinite(x):
+        s = "NaN" if math.isnan(x) else ("-Infinity" if x < 0 else "Infinity")
+        return s.rjust(w) if len(s) <= w else "*" * w
+    m, e = f"{abs(x):.{d}e}".split("e")
+    s = ("-" if math.copysign(1.0, x) < 0 else "") + m + _exponent(int(e))
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+# ---- one run's files -----------------------------------------------------
+class OutputFiles:
+    """The out/ files of one simulation.
+
+    ``mode``: the rd_emitter mode (RD_EMIT_*); ``dt``: time step (s);
+    ``V``: applied voltage (V); ``d``: gap (planar) or box_dim(3) (tip), m.
+    """
+
+    def __init__(self, directory: str, mode: int, dt: float, V: float, d: float,
+                 max_particles: int = 0):
+        self.dir = directory
+        self.mode = mode
+        self.dt = dt
+        self.E_z = -V / d if d else 0.0
+        self.V = V
+        os.makedirs(directory, exist_ok=True)
+        names = ["emitted", "absorbed", "absorbed_top", "absorbed_bot", "absorbed_recom",
+                 "ramo_current"]
+        if mode in (EMIT_PLANAR_FE, EMIT_PLANAR_GTF):
+            names.append("field")
+        self.f = {k: open(os.path.join(directory, k + ".dt"), "w") for k in names}
+        self.write_init_bin(max_particles)
+
+    def close(self, life_time=None):
+        """Close the files; with the lifetime histogram (Diode.life_time(),
+        1000 rows of electron and ion counts), write lifetime.dt too."""
+        for f in self.f.values():
+            f.close()
+        self.f = {}
+        if life_time is not None:
+            self.write_life_time(life_time)
+
+    def write_init_bin(self, max_particles: int):
+        rec = np.zeros(1, dtype=np.dtype([
+            ("epsilon_r", "<f8"), ("m_eeff", "<f8"), ("m_ieff", "<f8"),
+            ("length_scale", "<f8"), ("time_scale", "<f8"), ("vel_scale", "<f8"),
+            ("cur_scale", "<f8"), ("MAX_PARTICLES", "<i4"), ("MAX_EMITTERS", "<i4"),
+            ("MAX_SECTIONS", "<i4"), ("MAX_LIFE_TIME", "<i4")]))
+        rec[0] = (EPSILON_R, M_EEFF, M_IEFF, LENGTH_SCALE, TIME_SCALE, VEL_SCALE, CUR_SCALE,
+                  max_particles, MAX_EMITTERS, MAX_SECTIONS, MAX_LIFE_TIME)
+        rec.tofile(os.path.join(self.dir, "init.bin"))
+
+    def write_life_time(self, hist):
+        hist = np.asarray(hist)
+        hist = hist.T if hist.shape == (MAX_LIFE_TIME, 2) else hist.reshape(2, MAX_LIFE_TIME)
+        with open(os.path.join(self.dir, "lifetime.dt"), "w") as f:
+            for i in range(1, MAX_LIFE_TIME + 1):
+                time = i * self.dt / TIME_SCALE
+                f.write(fortran_ES(time, 12, 4) + "  " + fortran_I(i, 6) + "  " +
+                        fortran_I(hist[0, i - 1], 6) + "  " + fortran_I(hist[1, i - 1], 6) +
+                        "\n")
+
+    def write_step(self, rec):
+        """One step's lines from its record (rd_step_record or its as_dict())."""
+        r = rec if isinstance(rec, dict) else rec.as_dict()
+        step = r["step"]
+        cur_time = self.dt * step / TIME_SCALE
+        n_mid = r["n"] + r["absorbed"]  # after emission, before removal
+        # emitted.dt: (E14.6, *(tr8, i6))
+        ints = [step, r["emitted"], n_mid]
+        if self.mode != EMIT_TIP_FE:
+            ints.append(r["emitted"])  # nrEmitted_emitters(1)
+        self.f["emitted"].write(fortran_E(cur_time, 14, 6) +
+                                "".join(" " * 8 + fortran_I(v, 6) for v in ints) + "\n")
+        # field.dt: (i8, 7(tr2, E16.8))
+        if "field" in self.f:
+            df_avg = r["df_avg"] if self.mode == EMIT_PLANAR_FE else 0.0
+            vals = [*r["f_avg"], r["n_sup"], df_avg, r["a_rate"], r["mh_std"]]
+            self.f["field"].write(fortran_I(step, 8) +
+                                  "".join("  " + fortran_E(v, 16, 8) for v in vals) + "\n")
+        # absorbed*.dt: (ES12.4, *(tr2, i8))
+        def absorbed(name, ints):
+            self.f[name].write(fortran_ES(cur_time, 12, 4) +
+                               "".join("  " + fortran_I(v, 8) for v in [step, *ints]) + "\n")
+
+        absorbed("absorbed", [r["absorbed"], r["absorbed"], 0])
+        absorbed("absorbed_top", [r["absorbed_top"], r["absorbed_top"], 0, r["absorbed_top"]])
+        absorbed("absorbed_bot", [r["absorbed_bot"], r["absorbed_bot"], 0])
+        absorbed("absorbed_recom", [0, 0, 0])
+        # ramo_current.dt
+        v = r["avg_elec_vel"]
+        speed = math.sqrt(v[0] ** 2 + v[1] ** 2 + v[2] ** 2)
+        mob = speed / (-self.E_z) if self.E_z != 0.0 else 0.0
+        ramo = r["ramo"]
+        line = (fortran_ES(cur_time, 12, 4) + "  " + fortran_I(step, 8) + "  " +
+                fortran_ES(sum(ramo) / CUR_SCALE, 12, 4) + "  " + fortran_ES(self.V, 12, 4) +
+                "  " + fortran_I(n_mid, 6) + "  " + fortran_I(n_mid, 6) + "  " +
+                fortran_I(0, 6) + "  " + fortran_ES(mob, 12, 4) + "  " +
+                fortran_ES(speed, 12, 4) + "  " + fortran_ES(speed, 12, 4) + "  " +
+                fortran_ES(0.0, 12, 4) + "".join("  " + fortran_ES(x, 12, 4) for x in ramo))
+        self.f["ramo_current"].write(line + "\n")

@@ -1,6 +1,8 @@
 """The device-resident time step (emission, Beeman, boundaries, Ramo,
 compaction) against the oracle (bit for bit on the FP64 path), the
 reference's own routines (golden steps.npz) and closed forms."""
+import os
+
 import numpy as np
 import pytest
 
@@ -22,6 +24,7 @@ def assert_records_equal(rg, ro, ctx=""):
         assert getattr(rg, k) == getattr(ro, k), (ctx, k, getattr(rg, k), getattr(ro, k))
     assert list(rg.f_avg) == list(ro.f_avg), ctx
     assert list(rg.ramo) == list(ro.ramo), ctx
+    assert list(rg.avg_elec_vel) == list(ro.avg_elec_vel), ctx
 
 
 def assert_state_equal(sg, so, ctx=""):
@@ -149,6 +152,29 @@ def test_planar_fe_many_chains_bit_exact(Diode, oracle_mod):
         assert_records_equal(recs[k - 1], ro, f"step {k}")
     assert recs[0].n_round > 2048
     assert_state_equal(dd.get_state(), so.state(), "end")
+
+
+def test_output_files_equal_oracle(Diode, oracle_mod, tmp_path):
+    """The out/ files written from the device's records are byte-identical to
+    those written from the oracle's (planar FE; the tip's set too)."""
+    from rumdeed_amd import output as O
+
+    dd = Diode(4096)
+    so = oracle_mod.Sim(4096, nthreads=8)
+    _fe(dd, oracle_mod)
+    _fe(so, oracle_mod)
+    fg = O.OutputFiles(str(tmp_path / "gpu"), O.EMIT_PLANAR_FE, 0.5e-15, 1600.0, 200 * NM, 4096)
+    fo = O.OutputFiles(str(tmp_path / "oracle"), O.EMIT_PLANAR_FE, 0.5e-15, 1600.0, 200 * NM,
+                       4096)
+    for k, r in enumerate(dd.run(25, "fp64"), start=1):
+        fg.write_step(r)
+        fo.write_step(oracle_mod.record_dict(so.step(k)))
+    fg.close(dd.life_time())
+    fo.close(so.life_time())
+    names = sorted(os.listdir(tmp_path / "gpu"))
+    assert names == sorted(os.listdir(tmp_path / "oracle")) and "ramo_current.dt" in names
+    for name in names:
+        assert (tmp_path / "gpu" / name).read_bytes() == (tmp_path / "oracle" / name).read_bytes(), name
 
 
 def test_planar_gtf_run_bit_exact(Diode, oracle_mod):
