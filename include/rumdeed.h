@@ -93,6 +93,36 @@ typedef struct {
   double avg_elec_vel[3];
 } rd_step_record;
 
+/* Particle records of one step for RUMDEED's density files (as
+ * `rd_set_logs` collects them; rd_get_logs).  rd_emit_record: every
+ * electron the device emitted, in id order, at its emission point (the
+ * density_emit*.bin writes of Add_Particle, src/mod_pair.F90:63-95).
+ * rd_absorb_record: every particle absorbed, in index order, with its
+ * position and its velocity when it was marked (Mark_Particles_Remove,
+ * src/mod_pair.F90:240-256); where = 1 top (anode), 2 bottom (cathode or
+ * tip).  rd_plane_record: every crossing of a recording plane from below,
+ * by plane and then in index order, with the position after the move and
+ * the velocity before this step's velocity update (Check_Planes,
+ * src/mod_verlet.F90:343-367); plane = 1..RD_MAX_PLANES, the planes-<plane>.bin
+ * file.  Positions in m, velocities in m/s. */
+typedef struct {
+  double x, y, z;
+  int32_t emitter, section, id, species;
+} rd_emit_record;
+
+typedef struct {
+  double x, y, z, vx, vy, vz;
+  int32_t emitter, section, id, where;
+} rd_absorb_record;
+
+typedef struct {
+  double x, y, vx, vy, vz;
+  int32_t emitter, section, id, plane;
+} rd_plane_record;
+
+/* planes_N_max (src/mod_global.F90:337) */
+#define RD_MAX_PLANES 10
+
 /* ---- library and context ------------------------------------------------ */
 const char* rd_version(void);
 const char* rd_last_error(void);
@@ -202,6 +232,27 @@ int rd_pool_step(rd_ctx* ctx, int32_t precision, rd_step_record* records);
  * parameters change and replayed while they do not; results are identical
  * either way.  Library control, no reference counterpart. */
 int rd_set_graphs(rd_ctx* ctx, int32_t on);
+
+/* Collect the particle records of each step (on != 0) or not (0, the
+ * default).  Library control, no reference counterpart: the reference
+ * writes these records to files inside Add_Particle and
+ * Mark_Particles_Remove. */
+int rd_set_logs(rd_ctx* ctx, int32_t on);
+/* The records of the last step of the selected simulation: up to max_emit
+ * emission and max_absorb absorption records; n_emit / n_absorb receive the
+ * step's counts (RD_ERR_CAPACITY when they exceed the maxima). */
+int rd_get_logs(rd_ctx* ctx, int32_t max_emit, rd_emit_record* emit, int32_t* n_emit,
+                int32_t max_absorb, rd_absorb_record* absorb, int32_t* n_absorb);
+/* Recording planes of the selected simulation (planes_N and planes_z of the
+ * input namelist, src/mod_global.F90:337-342, scaled to m as
+ * src/main.F90:375 does): n <= RD_MAX_PLANES heights; a height <= 0 keeps
+ * its number but records nothing, as Check_Planes skips it.  n = 0 (the
+ * default) records none.  Crossings are collected while rd_set_logs is on. */
+int rd_set_planes(rd_ctx* ctx, int32_t n, const double* z);
+/* The plane crossings of the last step of the selected simulation (the
+ * planes-k.bin writes of Check_Planes): up to max_records records, plane 1
+ * first; n receives the count (RD_ERR_CAPACITY when it exceeds max_records). */
+int rd_get_plane_logs(rd_ctx* ctx, int32_t max_records, rd_plane_record* records, int32_t* n);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* Average device time (ms) of the accelerations / grid field over `reps`

@@ -302,6 +302,16 @@ def record_dict(r):
     return d
 
 
+EMIT_RECORD = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("emitter", "<i4"),
+                        ("section", "<i4"), ("id", "<i4"), ("species", "<i4")])
+ABSORB_RECORD = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("vx", "<f8"),
+                          ("vy", "<f8"), ("vz", "<f8"), ("emitter", "<i4"), ("section", "<i4"),
+                          ("id", "<i4"), ("where", "<i4")])
+PLANE_RECORD = np.dtype([("x", "<f8"), ("y", "<f8"), ("vx", "<f8"), ("vy", "<f8"),
+                         ("vz", "<f8"), ("emitter", "<i4"), ("section", "<i4"), ("id", "<i4"),
+                         ("plane", "<i4")])
+
+
 class Sim:
     """CPU restatement of one simulation (emission, Beeman step, removal)."""
 
@@ -384,6 +394,40 @@ class Sim:
     @property
     def n(self):
         return self.L.rdo_sim_count(ctypes.c_void_p(self.h))
+
+    def step_logs(self):
+        """The last step's emission and absorption records (Add_Particle's
+        and Mark_Particles_Remove's density writes), as structured arrays
+        with the fields of rd_emit_record / rd_absorb_record."""
+        L = self.L
+        for name in ("rdo_sim_emit_log", "rdo_sim_absorb_log"):
+            getattr(L, name).restype = c_int
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h = ctypes.c_void_p(self.h)
+        em = np.zeros(L.rdo_sim_emit_log(h, None), dtype=EMIT_RECORD)
+        ab = np.zeros(L.rdo_sim_absorb_log(h, None), dtype=ABSORB_RECORD)
+        L.rdo_sim_emit_log(h, em.ctypes.data_as(ctypes.c_void_p))
+        L.rdo_sim_absorb_log(h, ab.ctypes.data_as(ctypes.c_void_p))
+        return em, ab
+
+    def set_planes(self, z_m):
+        """Recording planes (Check_Planes), heights in m."""
+        z = np.ascontiguousarray(np.asarray(z_m, dtype=np.float64).ravel())
+        self.L.rdo_sim_set_planes.restype = None
+        self.L.rdo_sim_set_planes.argtypes = [ctypes.c_void_p, c_int, ctypes.c_void_p]
+        self.L.rdo_sim_set_planes(ctypes.c_void_p(self.h), int(z.size),
+                                  z.ctypes.data_as(ctypes.c_void_p))
+        self._planes = z
+
+    def plane_logs(self):
+        """The last step's plane crossings, plane 1 first."""
+        L = self.L
+        L.rdo_sim_plane_log.restype = c_int
+        L.rdo_sim_plane_log.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h = ctypes.c_void_p(self.h)
+        rec = np.zeros(L.rdo_sim_plane_log(h, None), dtype=PLANE_RECORD)
+        L.rdo_sim_plane_log(h, rec.ctypes.data_as(ctypes.c_void_p))
+        return rec
 
     def state(self):
         n = self.n

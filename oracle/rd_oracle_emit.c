@@ -324,7 +324,32 @@ typedef struct {
   int start_rounds, n_elec;
   int life[2][NLIFE];
   int nthreads;
+  /* particle records of the last step (the density_emit / density_absorb
+   * writes of Add_Particle and Mark_Particles_Remove) */
+  struct rdo_emit_rec* elog;
+  struct rdo_abs_rec* alog;
+  int n_elog, n_alog;
+  /* recording planes (planes_N, planes_z) and the last step's crossings,
+   * cap records per plane */
+  int nplanes;
+  double planes[10];
+  struct rdo_plane_rec* plog;
+  int n_plog[10];
 } rdo_sim;
+
+/* Same layouts as rd_emit_record / rd_absorb_record (include/rumdeed.h). */
+typedef struct rdo_emit_rec {
+  double x, y, z;
+  int32_t emitter, section, id, species;
+} rdo_emit_rec;
+typedef struct rdo_abs_rec {
+  double x, y, z, vx, vy, vz;
+  int32_t emitter, section, id, where;
+} rdo_abs_rec;
+typedef struct rdo_plane_rec {
+  double x, y, vx, vy, vz;
+  int32_t emitter, section, id, plane;
+} rdo_plane_rec;
 
 /* Per-step record, same layout as rd_step_record (include/rumdeed.h). */
 typedef struct {
@@ -355,6 +380,9 @@ rdo_sim* rdo_sim_create(int cap, int nthreads) {
   s->section = calloc((size_t)cap, 4);
   s->id = calloc((size_t)cap, 4);
   s->alive = calloc((size_t)cap, 4);
+  s->elog = calloc((size_t)cap, sizeof(rdo_emit_rec));
+  s->alog = calloc((size_t)cap, sizeof(rdo_abs_rec));
+  s->plog = calloc(10 * (size_t)cap, sizeof(rdo_plane_rec));
   s->MH_std = 0.0125;
   s->a_rate = 1.0;
   s->quad = 32;
@@ -367,7 +395,8 @@ void rdo_sim_destroy(rdo_sim* s) {
   if (!s) return;
   free(s->pos); free(s->prev); free(s->vel); free(s->acc); free(s->a1); free(s->a2);
   free(s->q); free(s->m); free(s->species); free(s->pstep); free(s->emitter);
-  free(s->section); free(s->id); free(s->alive);
+  free(s->section); free(s->id); free(s->alive); free(s->elog); free(s->alog);
+  free(s->plog);
   free(s);
 }
 
@@ -829,7 +858,18 @@ static void do_emission(rdo_sim* s, int step) {
     if (flag) {
       int before = s->n;
       add_particle(s, p, v, 1, step, 1, sec);
-      if (s->n > before) emitted++;
+      if (s->n > before) {
+        emitted++;
+        /* Add_Particle's density_emit record (src/mod_pair.F90:63-95) */
+        rdo_emit_rec* r = &s->elog[s->n_elog++];
+        r->x = p[0];
+        r->y = p[1];
+        r->z = p[2];
+        r->emitter = 1;
+        r->section = sec;
+        r->id = s->id[before];
+        r->species = 1;
+      }
     }
   }
   s->emitted = emitted;
@@ -848,6 +888,21 @@ static void mark_remove(rdo_sim* s, int i, int reason) {
   if (!s->alive[i]) return;
   s->alive[i] = 0;
   s->q[i] = 0.0;
+  {
+    /* Mark_Particles_Remove's record (src/mod_pair.F90:240-256): position and
+     * the velocity at marking; particles are marked in index order */
+    rdo_abs_rec* a = &s->alog[s->n_alog++];
+    a->x = s->pos[3 * i];
+    a->y = s->pos[3 * i + 1];
+    a->z = s->pos[3 * i + 2];
+    a->vx = s->vel[3 * i];
+    a->vy = s->vel[3 * i + 1];
+    a->vz = s->vel[3 * i + 2];
+    a->emitter = s->emitter[i];
+    a->section = s->section[i];
+    a->id = s->id[i];
+    a->where = reason;
+  }
   s->rem_total++;
   if (s->species[i] == 1) {
     s->rem_elec++;
@@ -861,6 +916,28 @@ static void mark_remove(rdo_sim* s, int i, int reason) {
     s->rem_ion++;
     if (reason == 1) s->rem_top++;
     else s->rem_bot++;
+  }
+}
+
+/* Check_Planes (src/mod_verlet.F90:343-367): a crossing from below of a
+ * plane with height > 0, recorded with the position after the move and the
+ * velocity before the velocity update; particles in index order. */
+static void check_planes(rdo_sim* s, int i) {
+  for (int k = 0; k < s->nplanes; ++k) {
+    const double zp = s->planes[k];
+    if (!(zp > 0.0)) continue;
+    if (s->pos[3 * i + 2] > zp && s->prev[3 * i + 2] < zp) {
+      rdo_plane_rec* r = &s->plog[(size_t)k * s->cap + s->n_plog[k]++];
+      r->x = s->pos[3 * i];
+      r->y = s->pos[3 * i + 1];
+      r->vx = s->vel[3 * i];
+      r->vy = s->vel[3 * i + 1];
+      r->vz = s->vel[3 * i + 2];
+      r->emitter = s->emitter[i];
+      r->section = s->section[i];
+      r->id = s->id[i];
+      r->plane = k + 1;
+    }
   }
 }
 
@@ -884,6 +961,7 @@ static void update_position(rdo_sim* s, int precision_unused) {
       double eta = tip_eta(&s->g, s->pos[3 * i], s->pos[3 * i + 1], z);
       if (eta < s->g.eta_1) mark_remove(s, i, 2);
     }
+    check_planes(s, i);
   }
   rdo_accel(&s->g, s->n, s->pos, s->q, s->m, s->acc, s->nthreads);
   /* velocity + Ramo; the device sums 256-particle blocks with a fixed tree
@@ -960,6 +1038,9 @@ static void remove_particles(rdo_sim* s, int step) {
 }
 
 static void reset_counters(rdo_sim* s) {
+  s->n_elog = 0;
+  s->n_alog = 0;
+  for (int k = 0; k < 10; ++k) s->n_plog[k] = 0;
   s->emitted = 0;
   s->n_round = 0;
   s->rem_total = s->rem_top = s->rem_bot = s->rem_elec = s->rem_ion = s->rem_top_emit = 0;
@@ -1006,6 +1087,31 @@ void rdo_sim_step(rdo_sim* s, int step, int phases, rdo_record* rec) {
 }
 
 int rdo_sim_count(const rdo_sim* s) { return s->n; }
+
+/* The last step's particle records; returns how many (copied when out is
+ * not NULL). */
+int rdo_sim_emit_log(const rdo_sim* s, rdo_emit_rec* out) {
+  if (out) memcpy(out, s->elog, sizeof(rdo_emit_rec) * (size_t)s->n_elog);
+  return s->n_elog;
+}
+void rdo_sim_set_planes(rdo_sim* s, int n, const double* z) {
+  s->nplanes = n < 0 ? 0 : (n > 10 ? 10 : n);
+  for (int k = 0; k < 10; ++k) s->planes[k] = k < s->nplanes ? z[k] : 0.0;
+}
+/* All planes' crossings of the last step, plane 1 first. */
+int rdo_sim_plane_log(const rdo_sim* s, rdo_plane_rec* out) {
+  int at = 0;
+  for (int k = 0; k < 10; ++k) {
+    if (out)
+      memcpy(out + at, s->plog + (size_t)k * s->cap, sizeof(rdo_plane_rec) * (size_t)s->n_plog[k]);
+    at += s->n_plog[k];
+  }
+  return at;
+}
+int rdo_sim_absorb_log(const rdo_sim* s, rdo_abs_rec* out) {
+  if (out) memcpy(out, s->alog, sizeof(rdo_abs_rec) * (size_t)s->n_alog);
+  return s->n_alog;
+}
 
 void rdo_sim_get(const rdo_sim* s, double* pos, double* prev, double* vel, double* acc,
                  double* a1, double* a2, int* ints, double* q) {

@@ -15,6 +15,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "librumdeed.so")
 
 RD_OK = 0
+RD_ERR_ARG, RD_ERR_HIP, RD_ERR_CAPACITY, RD_ERR_STATE = 1, 2, 3, 4
 RD_GEOM_PLANAR = 1
 RD_GEOM_TIP = 2
 RD_PREC_FP64 = 0
@@ -77,6 +78,19 @@ class StepRecord(ctypes.Structure):
             d[k] = list(v) if k in ("f_avg", "ramo", "avg_elec_vel") else v
         return d
 
+# rd_emit_record / rd_absorb_record (include/rumdeed.h)
+import numpy as _np  # noqa: E402
+
+EMIT_RECORD = _np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("emitter", "<i4"),
+                         ("section", "<i4"), ("id", "<i4"), ("species", "<i4")])
+ABSORB_RECORD = _np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("vx", "<f8"),
+                           ("vy", "<f8"), ("vz", "<f8"), ("emitter", "<i4"), ("section", "<i4"),
+                           ("id", "<i4"), ("where", "<i4")])
+PLANE_RECORD = _np.dtype([("x", "<f8"), ("y", "<f8"), ("vx", "<f8"), ("vy", "<f8"),
+                          ("vz", "<f8"), ("emitter", "<i4"), ("section", "<i4"), ("id", "<i4"),
+                          ("plane", "<i4")])
+MAX_PLANES = 10
+
 # name -> (restype, argtypes)
 _SIGS = {
     "rd_version": (c_char_p, []),
@@ -115,6 +129,10 @@ _SIGS = {
                                      c_int32, c_int32, c_double, c_int32, c_int32, _dp, _dp]),
     "rd_time_copy": (c_int32, [c_void_p, c_int64, c_int32, _dp]),
     "rd_set_graphs": (c_int32, [c_void_p, c_int32]),
+    "rd_set_logs": (c_int32, [c_void_p, c_int32]),
+    "rd_get_logs": (c_int32, [c_void_p, c_int32, c_void_p, _ip, c_int32, c_void_p, _ip]),
+    "rd_set_planes": (c_int32, [c_void_p, c_int32, c_void_p]),
+    "rd_get_plane_logs": (c_int32, [c_void_p, c_int32, c_void_p, _ip]),
 }
 
 _lib = None

@@ -377,7 +377,8 @@ int rd_ctx_destroy(rd_ctx* c) {
   if (c->h_state) hipHostFree(c->h_state);
   if (c->h_tab) hipHostFree(c->h_tab);
   Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
-                 &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo, &c->cnt};
+                 &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo, &c->cnt,
+                 &c->mark, &c->elog, &c->alog, &c->plog};
   for (Buf* b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -631,6 +632,78 @@ int rd_get_life_time(rd_ctx* c, int32_t* hist) {
   // (1000, 2) column-major like life_time(1:MAX_LIFE_TIME, 1:nrSpecies)
   for (int s = 0; s < 2; ++s)
     for (int l = 0; l < kLife; ++l) hist[(size_t)s * kLife + l] = tmp[(size_t)s * kLife + l];
+  return RD_OK;
+}
+
+int rd_set_logs(rd_ctx* c, int32_t on) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  RD_HIP(hipSetDevice(c->device));
+  if (on) {
+    int r = ensure(c, c->mark, sizeof(double) * 4 * (size_t)c->cap);
+    if (!r) r = ensure(c, c->elog, sizeof(rd_emit_record) * (size_t)c->cap);
+    if (!r) r = ensure(c, c->alog, sizeof(rd_absorb_record) * (size_t)c->cap);
+    if (!r) r = ensure(c, c->plog, sizeof(rd_plane_record) * RD_MAX_PLANES * (size_t)c->cap);
+    if (r) return r;
+  }
+  c->logs = on != 0;
+  return RD_OK;
+}
+
+int rd_get_logs(rd_ctx* c, int32_t max_emit, rd_emit_record* emit, int32_t* n_emit,
+                int32_t max_absorb, rd_absorb_record* absorb, int32_t* n_absorb) {
+  RD_CUR(c);
+  if (!n_emit || !n_absorb) return fail(RD_ERR_ARG, "null pointer");
+  if (!c->logs) return fail(RD_ERR_STATE, "particle records are off (rd_set_logs)");
+  RD_HIP(hipStreamSynchronize(c->stream));
+  const SimState& S = c->h_state[c->cur];
+  *n_emit = S.emitted;
+  *n_absorb = S.rem_total;
+  if (S.emitted > max_emit || S.rem_total > max_absorb)
+    return fail(RD_ERR_CAPACITY, "record buffers smaller than the step's counts");
+  if ((S.emitted > 0 && !emit) || (S.rem_total > 0 && !absorb))
+    return fail(RD_ERR_ARG, "null record buffer");
+  const size_t base = (size_t)sl->desc.base;
+  if (S.emitted > 0)
+    RD_HIP(hipMemcpy(emit, (rd_emit_record*)c->elog.p + base, sizeof(rd_emit_record) * S.emitted,
+                     hipMemcpyDeviceToHost));
+  if (S.rem_total > 0)
+    RD_HIP(hipMemcpy(absorb, (rd_absorb_record*)c->alog.p + base,
+                     sizeof(rd_absorb_record) * S.rem_total, hipMemcpyDeviceToHost));
+  return RD_OK;
+}
+
+int rd_set_planes(rd_ctx* c, int32_t n, const double* z) {
+  RD_CUR(c);
+  if (n < 0 || n > RD_MAX_PLANES) return fail(RD_ERR_ARG, "plane count outside 0..RD_MAX_PLANES");
+  if (n > 0 && !z) return fail(RD_ERR_ARG, "null plane heights");
+  for (int k = 0; k < n; ++k)
+    if (!std::isfinite(z[k])) return fail(RD_ERR_ARG, "plane height not finite");
+  SimDesc& d = sl->desc;
+  d.nplanes = n;
+  for (int k = 0; k < RD_MAX_PLANES; ++k) d.planes[k] = k < n ? z[k] : 0.0;
+  return put_desc(c, c->cur);
+}
+
+int rd_get_plane_logs(rd_ctx* c, int32_t max_records, rd_plane_record* records, int32_t* n) {
+  RD_CUR(c);
+  if (!n) return fail(RD_ERR_ARG, "null pointer");
+  if (!c->logs) return fail(RD_ERR_STATE, "particle records are off (rd_set_logs)");
+  RD_HIP(hipStreamSynchronize(c->stream));
+  const SimState& S = c->h_state[c->cur];
+  long long total = 0;
+  for (int k = 0; k < RD_MAX_PLANES; ++k) total += S.plane_n[k];
+  *n = (int32_t)total;
+  if (total > max_records) return fail(RD_ERR_CAPACITY, "record buffer smaller than the count");
+  if (total > 0 && !records) return fail(RD_ERR_ARG, "null record buffer");
+  const size_t base = (size_t)sl->desc.base;
+  long long at = 0;
+  for (int k = 0; k < RD_MAX_PLANES; ++k) {
+    const int m = S.plane_n[k];
+    if (m <= 0) continue;
+    RD_HIP(hipMemcpy(records + at, (rd_plane_record*)c->plog.p + (size_t)k * c->cap + base,
+                     sizeof(rd_plane_record) * m, hipMemcpyDeviceToHost));
+    at += m;
+  }
   return RD_OK;
 }
 

@@ -1190,7 +1190,7 @@ __global__ __launch_bounds__(256) void k_emit(const Ent* tab, const SimDesc* des
                                               double* A1Z, double* A2X, double* A2Y, double* A2Z,
                                               double* Q, double* M, int32_t* SP, int32_t* STEP,
                                               int32_t* EMIT, int32_t* SEC, int32_t* ID,
-                                              int32_t* ALIVE) {
+                                              int32_t* ALIVE, rd_emit_record* elog) {
   __shared__ int scan[256];
   __shared__ double dsum[256];
   __shared__ int s_off, s_drop;
@@ -1323,6 +1323,17 @@ __global__ __launch_bounds__(256) void k_emit(const Ent* tab, const SimDesc* des
         SEC[gi] = sec;
         ID[gi] = id0 + s_off + rank;
         ALIVE[gi] = 1;
+        if (elog) {
+          rd_emit_record r;
+          r.x = x;
+          r.y = y;
+          r.z = z;
+          r.emitter = 1;
+          r.section = sec;
+          r.id = id0 + s_off + rank;
+          r.species = RD_SPECIES_ELEC;
+          elog[s.base + (slot_idx - n0)] = r;
+        }
       }
     }
     __syncthreads();
@@ -1429,10 +1440,17 @@ __global__ void k_add_commit(int slot, const SimDesc* desc, SimState* st, int n)
 // ---- Beeman position update + boundary marking ---------------------------
 __device__ __forceinline__ void mark_remove(SimState& S, int32_t* ALIVE, double* Q,
                                             const int32_t* SP, const int32_t* EMIT, int gi,
-                                            int reason) {
+                                            int reason, double* mark, double vx, double vy,
+                                            double vz) {
   if (!ALIVE[gi]) return;
   ALIVE[gi] = 0;
   Q[gi] = 0.0;
+  if (mark) {  // the velocity at marking, for the absorption record
+    mark[4 * (size_t)gi + 0] = vx;
+    mark[4 * (size_t)gi + 1] = vy;
+    mark[4 * (size_t)gi + 2] = vz;
+    mark[4 * (size_t)gi + 3] = reason;
+  }
   atomicAdd(&S.rem_total, 1);
   const int spc = SP[gi];
   if (spc == RD_SPECIES_ELEC) {
@@ -1457,7 +1475,7 @@ __global__ void k_position(const Ent* tab, const SimDesc* desc, SimState* st, do
                            const double* VX, const double* VY, const double* VZ, double* AX,
                            double* AY, double* AZ, double* A1X, double* A1Y, double* A1Z,
                            double* A2X, double* A2Y, double* A2Z, double* Q, const int32_t* SP,
-                           const int32_t* EMIT, int32_t* ALIVE) {
+                           const int32_t* EMIT, int32_t* ALIVE, double* mark) {
   const Ent e = tab[blockIdx.z];
   SimState& S = st[e.slot];
   const int n = S.n;
@@ -1468,6 +1486,7 @@ __global__ void k_position(const Ent* tab, const SimDesc* desc, SimState* st, do
   const double dt = s.dt, dt2 = s.dt2;
   const double c16 = 1.0 / 6.0;
   double x = X[gi], y = Y[gi], z = Z[gi];
+  const double zprev = z;
   PX[gi] = x;
   PY[gi] = y;
   PZ[gi] = z;
@@ -1489,13 +1508,26 @@ __global__ void k_position(const Ent* tab, const SimDesc* desc, SimState* st, do
   AY[gi] = 0.0;
   AZ[gi] = 0.0;
   // ptr_Check_Boundary
+  const double vx = VX[gi], vy = VY[gi], vz = VZ[gi];
+  if (mark) {  // Check_Planes: the velocity of a crossing, for its record
+    bool cross = false;
+    for (int p = 0; p < s.nplanes; ++p) {
+      const double zp = s.planes[p];
+      cross |= zp > 0.0 && z > zp && zprev < zp;
+    }
+    if (cross) {
+      mark[4 * (size_t)gi + 0] = vx;
+      mark[4 * (size_t)gi + 1] = vy;
+      mark[4 * (size_t)gi + 2] = vz;
+    }
+  }
   if (z < 0.0)
-    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2);
+    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2, mark, vx, vy, vz);
   else if (z > s.g.box_top)
-    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 1);
+    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 1, mark, vx, vy, vz);
   if (s.g.kind == RD_GEOM_TIP) {
     double eta = tip_eta(tip_surf(s.g), x, y, z);
-    if (eta < s.g.eta_1) mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2);
+    if (eta < s.g.eta_1) mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2, mark, vx, vy, vz);
   }
 }
 
@@ -1627,15 +1659,18 @@ __global__ __launch_bounds__(1024) void k_compact(
     double* Z, double* PX, double* PY, double* PZ, double* VX, double* VY, double* VZ, double* AX,
     double* AY, double* AZ, double* A1X, double* A1Y, double* A1Z, double* A2X, double* A2Y,
     double* A2Z, double* Q, double* M, int32_t* SP, int32_t* STEP, int32_t* EMIT, int32_t* SEC,
-    int32_t* ID, int32_t* ALIVE) {
-  __shared__ int wsum[1024 / 64];
-  __shared__ int s_off;
+    int32_t* ID, int32_t* ALIVE, const double* mark, rd_absorb_record* alog) {
+  __shared__ int wsum[1024 / 64], rsum[1024 / 64];
+  __shared__ int s_off, s_roff;
   const Ent e = tab[blockIdx.x];
   SimState& S = st[e.slot];
   if (S.rem_total == 0) return;
   const int n = S.n;
   const int base = e.base;
-  if (threadIdx.x == 0) s_off = 0;
+  if (threadIdx.x == 0) {
+    s_off = 0;
+    s_roff = 0;
+  }
   __syncthreads();
   for (int cb = 0; cb < n; cb += 1024) {
     const int i = cb + threadIdx.x;
@@ -1666,15 +1701,41 @@ __global__ __launch_bounds__(1024) void k_compact(
     // rank among the chunk's survivors: a ballot per wavefront, then the
     // survivors of the lower wavefronts
     const unsigned long long mask = __ballot(flag);
+    const bool gone = i < n && !flag;  // removed: an absorption record
+    const unsigned long long gmask = __ballot(gone);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) wsum[wv] = __popcll(mask);
-    __syncthreads();
-    int rank = __popcll(mask & ((1ull << lane) - 1ull)), total = 0;
-    for (int u = 0; u < 1024 / 64; ++u) {
-      if (u < wv) rank += wsum[u];
-      total += wsum[u];
+    if (lane == 0) {
+      wsum[wv] = __popcll(mask);
+      rsum[wv] = __popcll(gmask);
     }
-    const int off = s_off;
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int rank = __popcll(mask & below), total = 0;
+    int rrank = __popcll(gmask & below), rtotal = 0;
+    for (int u = 0; u < 1024 / 64; ++u) {
+      if (u < wv) {
+        rank += wsum[u];
+        rrank += rsum[u];
+      }
+      total += wsum[u];
+      rtotal += rsum[u];
+    }
+    const int off = s_off, roff = s_roff;
+    if (gone && alog) {  // in index order, before any particle moves
+      const int gi = base + i;
+      rd_absorb_record a;
+      a.x = X[gi];
+      a.y = Y[gi];
+      a.z = Z[gi];
+      a.vx = mark[4 * (size_t)gi + 0];
+      a.vy = mark[4 * (size_t)gi + 1];
+      a.vz = mark[4 * (size_t)gi + 2];
+      a.emitter = EMIT[gi];
+      a.section = SEC[gi];
+      a.id = ID[gi];
+      a.where = (int)mark[4 * (size_t)gi + 3];
+      alog[base + roff + rrank] = a;
+    }
     __syncthreads();  // every read of this chunk is done before any write
     if (flag) {
       const int d = base + off + rank;
@@ -1689,10 +1750,80 @@ __global__ __launch_bounds__(1024) void k_compact(
       ALIVE[d] = 1;
     }
     __syncthreads();
-    if (threadIdx.x == 0) s_off = off + total;
+    if (threadIdx.x == 0) {
+      s_off = off + total;
+      s_roff = roff + rtotal;
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) S.n = s_off;
+}
+
+// ---- Check_Planes records ------------------------------------------------
+// One workgroup per simulation walks its particles in index order after the
+// move (removed ones included, as Check_Planes runs before removal) and lists
+// each crossing from below, z_prev < z_p < z, per plane in index order: a
+// ballot per wavefront and plane ranks the crossings.  Plane p's records of
+// the simulation at `base` go to plog[p * cap + base + rank].
+__global__ __launch_bounds__(1024) void k_planes(const Ent* tab, const SimDesc* desc,
+                                                 SimState* st, const double* X, const double* Y,
+                                                 const double* Z, const double* PZ,
+                                                 const int32_t* EMIT, const int32_t* SEC,
+                                                 const int32_t* ID, const double* mark,
+                                                 rd_plane_record* plog, long long cap) {
+  __shared__ int wsum[RD_MAX_PLANES][1024 / 64];
+  __shared__ int s_off[RD_MAX_PLANES];
+  const Ent e = tab[blockIdx.x];
+  SimState& S = st[e.slot];
+  const SimDesc& s = desc[e.slot];
+  const int n = S.n, np = s.nplanes;
+  if (threadIdx.x < RD_MAX_PLANES) s_off[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int cb = 0; cb < n; cb += 1024) {
+    const int i = cb + threadIdx.x;
+    const int gi = e.base + i;
+    double z = 0.0, zprev = 0.0;
+    if (i < n) {
+      z = Z[gi];
+      zprev = PZ[gi];
+    }
+    unsigned long long bm[RD_MAX_PLANES];
+#pragma unroll
+    for (int p = 0; p < RD_MAX_PLANES; ++p) {
+      const double zp = p < np ? s.planes[p] : 0.0;
+      bm[p] = __ballot(i < n && zp > 0.0 && z > zp && zprev < zp);
+      if (lane == 0) wsum[p][wv] = __popcll(bm[p]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < RD_MAX_PLANES; ++p) {
+      if ((bm[p] >> lane) & 1ull) {
+        int rank = s_off[p] + __popcll(bm[p] & below);
+        for (int u = 0; u < wv; ++u) rank += wsum[p][u];
+        rd_plane_record r;
+        r.x = X[gi];
+        r.y = Y[gi];
+        r.vx = mark[4 * (size_t)gi + 0];
+        r.vy = mark[4 * (size_t)gi + 1];
+        r.vz = mark[4 * (size_t)gi + 2];
+        r.emitter = EMIT[gi];
+        r.section = SEC[gi];
+        r.id = ID[gi];
+        r.plane = p + 1;
+        plog[(long long)p * cap + e.base + rank] = r;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < RD_MAX_PLANES) {
+      int t = 0;
+      for (int u = 0; u < 1024 / 64; ++u) t += wsum[threadIdx.x][u];
+      s_off[threadIdx.x] += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < RD_MAX_PLANES) S.plane_n[threadIdx.x] = s_off[threadIdx.x];
 }
 
 // ------------------------------------------------------------------------
@@ -1758,7 +1889,7 @@ int engine_position(rd_ctx* c, const std::vector<Ent>& ents) {
   k_position<<<dim3((mx + 255) / 256, 1, ents.size()), 256, 0, c->stream>>>(
       dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
       c->ax, c->ay, c->az, c->a1x, c->a1y, c->a1z, c->a2x, c->a2y, c->a2z, c->q, c->species,
-      c->emitter, c->alive);
+      c->emitter, c->alive, c->logs ? (double*)c->mark.p : nullptr);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }
@@ -1787,7 +1918,13 @@ int engine_compact(rd_ctx* c, const std::vector<Ent>& ents) {
   Ent* dt;
   int r = upload_tab(c, ents, &dt);
   if (r) return r;
-  k_compact<<<ents.size(), 1024, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->d_life, RD_ARRAYS);
+  if (c->logs)  // plane crossings, before removal reorders the particles
+    k_planes<<<ents.size(), 1024, 0, c->stream>>>(
+        dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->pz, c->emitter, c->section, c->id,
+        (const double*)c->mark.p, (rd_plane_record*)c->plog.p, (long long)c->cap);
+  k_compact<<<ents.size(), 1024, 0, c->stream>>>(
+      dt, c->d_desc, c->d_state, c->d_life, RD_ARRAYS, c->logs ? (const double*)c->mark.p : nullptr,
+      c->logs ? (rd_absorb_record*)c->alog.p : nullptr);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }
@@ -2131,7 +2268,8 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
   r = upload_tab(c, ch, &dc);
   if (r) return r;
   k_mh_finish<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V);
-  k_emit<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V, RD_ARRAYS);
+  k_emit<<<ch.size(), 256, 0, c->stream>>>(dc, c->d_desc, c->d_state, c->d_wf, V, RD_ARRAYS,
+                                            c->logs ? (rd_emit_record*)c->elog.p : nullptr);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }

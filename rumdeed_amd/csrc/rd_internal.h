@@ -60,8 +60,9 @@ struct SimDesc {
   int quad;                   // quadrature points per side (planar)
   int base, cap;
   int ndim;                   // Metropolis-Hastings jumps per chain
-  int pad;
+  int nplanes;                // recording planes (Check_Planes)
   uint64_t seed, stream;
+  double planes[RD_MAX_PLANES];  // plane heights (m); <= 0: unused
 };
 
 // Mutable per-slot state, resident in HBM.
@@ -82,6 +83,7 @@ struct SimState {
   double F_avg[3];
   double ramo[3];   // Ramo current per species (A)
   double vsum[9];   // velocity sums: all, electrons, ions
+  int plane_n[RD_MAX_PLANES];  // plane crossings of the last step
 };
 
 // One launch-table entry (per active slot and launch).
@@ -173,6 +175,10 @@ struct rd_ctx {
   rd::Buf rnd;  // random numbers of the Metropolis-Hastings jumps of a step
   rd::Buf cnt;  // accept / reject counts per accept workgroup of a jump
   rd::Buf img_hi, img_lo;  // packed sphere images of the particles (mixed, tip)
+  // particle records of the step (rd_set_logs): velocity and reason at
+  // marking (4 doubles a particle), emission and absorption records
+  int logs = 0;
+  rd::Buf mark, elog, alog, plog;  // plog: RD_MAX_PLANES regions of cap records
   rd::Ent* h_tab = nullptr;  // pinned staging of launch tables (ring)
   int tab_used = 0;          // ring entries used since the last stream synchronisation
   int tab_last = -1;         // ring index of the last upload

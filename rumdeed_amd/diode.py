@@ -16,6 +16,8 @@ from . import _lib as L
 from ._lib import RD_GEOM_PLANAR, RD_GEOM_TIP, RD_PREC_FP64, RD_PREC_MIXED, check, dptr, f64, iptr
 
 PRECISIONS = {"fp64": RD_PREC_FP64, "mixed": RD_PREC_MIXED}
+# planes_z default of the input namelist, nm (src/mod_global.F90:340-341)
+PLANES_Z_NM = (5.0, 10.0, 25.0, 50.0, 75.0, 100.0, 125.0, 250.0, 500.0, 750.0)
 
 
 def _prec(p):
@@ -67,6 +69,43 @@ class Diode:
 
     def set_time_step(self, dt: float):
         check(self._lib.rd_set_time_step(self._h, c_double(dt)))
+
+    def set_logs(self, on: bool = True):
+        """Collect each step's emission and absorption particle records."""
+        check(self._lib.rd_set_logs(self._h, int(bool(on))))
+
+    def step_logs(self):
+        """The last step's (emission, absorption) records of the selected
+        simulation, as numpy structured arrays (L.EMIT_RECORD,
+        L.ABSORB_RECORD)."""
+        ne, na = ctypes.c_int32(0), ctypes.c_int32(0)
+        rc = self._lib.rd_get_logs(self._h, 0, None, ctypes.byref(ne), 0, None, ctypes.byref(na))
+        if rc not in (0, L.RD_ERR_CAPACITY):
+            check(rc)
+        emit = np.zeros(ne.value, dtype=L.EMIT_RECORD)
+        absorb = np.zeros(na.value, dtype=L.ABSORB_RECORD)
+        check(self._lib.rd_get_logs(self._h, ne.value, emit.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.byref(ne), na.value,
+                                    absorb.ctypes.data_as(ctypes.c_void_p), ctypes.byref(na)))
+        return emit, absorb
+
+    def set_planes(self, z_m):
+        """Recording planes (heights in m; <= 0 keeps the number, records
+        nothing).  RUMDEED's default is PLANES_Z_NM scaled by 1e-9."""
+        z = np.ascontiguousarray(np.asarray(z_m, dtype=np.float64).ravel())
+        check(self._lib.rd_set_planes(self._h, int(z.size),
+                                      z.ctypes.data_as(ctypes.c_void_p) if z.size else None))
+
+    def plane_logs(self):
+        """The last step's plane crossings (L.PLANE_RECORD), plane 1 first."""
+        n = ctypes.c_int32(0)
+        rc = self._lib.rd_get_plane_logs(self._h, 0, None, ctypes.byref(n))
+        if rc not in (0, L.RD_ERR_CAPACITY):
+            check(rc)
+        rec = np.zeros(n.value, dtype=L.PLANE_RECORD)
+        check(self._lib.rd_get_plane_logs(self._h, n.value, rec.ctypes.data_as(ctypes.c_void_p),
+                                          ctypes.byref(n)))
+        return rec
 
     def set_graphs(self, on: bool = True):
         """Launch each step's Metropolis-Hastings jump loop as one HIP graph."""

@@ -15,7 +15,21 @@ histogram), with the reference's Fortran edit descriptors:
 - ``ramo_current.dt``: Write_Ramo_Current (src/mod_pair.F90:812-837), with
   the mobility of Average_Velocities (src/mod_verlet.F90:425-446);
 - ``lifetime.dt``: Write_Life_Time (src/mod_pair.F90:774-784);
-- ``init.bin``: Write_Initial_Variables (src/main.F90:928-935).
+- ``init.bin``: Write_Initial_Variables (src/main.F90:928-935);
+- ``density_emit.bin``, ``density_emit_elec.bin`` (Add_Particle,
+  src/mod_pair.F90:86-89 and 119-123), ``density_absorb_top.bin``,
+  ``density_absorb_bot.bin`` (Mark_Particles_Remove, src/mod_pair.F90:240-256):
+  Fortran stream files (src/main.F90:665-722), packed records with no
+  markers, from the per-step particle records (Diode.step_logs()).  The
+  ion, atom and recombination density files are created empty: these
+  emission modes make none;
+- ``planes-<k>.bin``: Check_Planes (src/mod_verlet.F90:343-367), one stream
+  file per plane with height > 0 (src/main.F90:726-740), from the plane
+  records (Diode.plane_logs());
+- ``position.bin``: Write_Position (src/mod_pair.F90:841-867), written
+  between the position update and the removal (src/main.F90:190-191), so
+  it lists the particles marked for removal too; from Diode.get_state()
+  taken between update_position and remove_particles.
 
 Counts follow the reference's moments of writing: ``emitted.dt`` and
 ``ramo_current.dt`` see the particles after emission and before removal
@@ -43,6 +57,25 @@ MAX_SECTIONS = 96 * 96
 MAX_LIFE_TIME = 1000
 
 EMIT_TIP_FE, EMIT_PLANAR_GTF, EMIT_PLANAR_FE = 3, 9, 10
+SPECIES_ELEC = 1
+WHERE_TOP, WHERE_BOT = 1, 2
+
+# Stream-file record layouts (packed, little-endian as the reference's
+# unformatted writes on x86 and AMD hosts).
+DENSITY_EMIT = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("emitter", "<i4"),
+                         ("section", "<i4"), ("id", "<i4"), ("species", "<i4")])
+DENSITY_EMIT_SPECIES = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("emitter", "<i4"),
+                                 ("id", "<i4")])
+DENSITY_ABSORB_TOP = np.dtype([("x", "<f8"), ("y", "<f8"), ("vx", "<f8"), ("vy", "<f8"),
+                               ("vz", "<f8"), ("emitter", "<i4"), ("section", "<i4"),
+                               ("id", "<i4")])
+DENSITY_ABSORB_BOT = np.dtype([("x", "<f8"), ("y", "<f8"), ("emitter", "<i4"),
+                               ("section", "<i4"), ("id", "<i4")])
+POSITION_PARTICLE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("emitter", "<i4"),
+                              ("section", "<i4"), ("id", "<i4")])
+DENSITY_FILES = ("density_emit", "density_ion", "density_emit_ion", "density_emit_elec",
+                 "density_emit_atom", "density_absorb_top", "density_absorb_bot",
+                 "density_absorb_recom")
 
 
 # ---- Fortran edit descriptors ---------------------------------------------
@@ -90,10 +123,16 @@ class OutputFiles:
 
     ``mode``: the rd_emitter mode (RD_EMIT_*); ``dt``: time step (s);
     ``V``: applied voltage (V); ``d``: gap (planar) or box_dim(3) (tip), m.
+    ``densities``: also write the density_*.bin particle files (from
+    ``write_particles``).  ``planes``: the recording plane heights (m) whose
+    planes-<k>.bin files ``write_planes`` fills.  ``position_steps``: write
+    position.bin (write_position_file), whose header holds this total step
+    count.
     """
 
     def __init__(self, directory: str, mode: int, dt: float, V: float, d: float,
-                 max_particles: int = 0):
+                 max_particles: int = 0, densities: bool = False, planes=None,
+                 position_steps: int | None = None):
         self.dir = directory
         self.mode = mode
         self.dt = dt
@@ -105,14 +144,24 @@ class OutputFiles:
         if mode in (EMIT_PLANAR_FE, EMIT_PLANAR_GTF):
             names.append("field")
         self.f = {k: open(os.path.join(directory, k + ".dt"), "w") for k in names}
+        self.bin = ({k: open(os.path.join(directory, k + ".bin"), "wb") for k in DENSITY_FILES}
+                    if densities else {})
+        self.planes = {}
+        for k, z in enumerate([] if planes is None else list(planes), start=1):
+            if z > 0.0:
+                self.planes[k] = open(os.path.join(directory, f"planes-{k}.bin"), "wb")
+                self.bin[f"planes-{k}"] = self.planes[k]
+        self.position_steps = position_steps
+        if position_steps is not None:
+            self.bin["position"] = open(os.path.join(directory, "position.bin"), "wb")
         self.write_init_bin(max_particles)
 
     def close(self, life_time=None):
         """Close the files; with the lifetime histogram (Diode.life_time(),
         1000 rows of electron and ion counts), write lifetime.dt too."""
-        for f in self.f.values():
+        for f in (*self.f.values(), *self.bin.values()):
             f.close()
-        self.f = {}
+        self.f, self.bin, self.planes = {}, {}, {}
         if life_time is not None:
             self.write_life_time(life_time)
 
@@ -175,3 +224,62 @@ class OutputFiles:
                 fortran_ES(speed, 12, 4) + "  " + fortran_ES(speed, 12, 4) + "  " +
                 fortran_ES(0.0, 12, 4) + "".join("  " + fortran_ES(x, 12, 4) for x in ramo))
         self.f["ramo_current"].write(line + "\n")
+
+    def write_planes(self, records):
+        """One step's plane crossings (Diode.plane_logs(): plane 1 first,
+        each plane in particle index order) appended to planes-<k>.bin."""
+        records = np.asarray(records)
+        for k, f in self.planes.items():
+            r = records[records["plane"] == k]
+            out = np.empty(r.shape[0], DENSITY_ABSORB_TOP)
+            out["x"], out["y"] = r["x"] / LENGTH_SCALE, r["y"] / LENGTH_SCALE
+            for name in ("vx", "vy", "vz", "emitter", "section", "id"):
+                out[name] = r[name]
+            f.write(out.tobytes())
+
+    def write_position(self, step: int, pos, emitter, section, ids):
+        """Write_Position's block for one step: (step, nrPart) and each
+        particle's position (m), emitter, section and id; the header
+        (steps, N_steps = 1) goes before step 1."""
+        f = self.bin["position"]
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        if step == 1:
+            f.write(np.array([self.position_steps, 1], "<i4").tobytes())
+        f.write(np.array([step, pos.shape[0]], "<i4").tobytes())
+        rec = np.empty(pos.shape[0], POSITION_PARTICLE)
+        rec["x"], rec["y"], rec["z"] = pos[:, 0], pos[:, 1], pos[:, 2]
+        rec["emitter"], rec["section"], rec["id"] = emitter, section, ids
+        f.write(rec.tobytes())
+
+    def write_particles(self, emit, absorb):
+        """One step's particle records (the structured arrays of
+        Diode.step_logs(): emission in Add_Particle order, absorption in
+        particle index order) appended to the density_*.bin files, positions
+        divided by length_scale as the reference writes them."""
+        if "density_emit" not in self.bin:
+            raise ValueError("OutputFiles was opened without densities=True")
+        emit = np.asarray(emit)
+        absorb = np.asarray(absorb)
+        rec = np.empty(emit.shape[0], DENSITY_EMIT)
+        for k in ("x", "y", "z"):
+            rec[k] = emit[k] / LENGTH_SCALE
+        for k in ("emitter", "section", "id", "species"):
+            rec[k] = emit[k]
+        elec = rec[emit["species"] == SPECIES_ELEC]
+        sp = np.empty(elec.shape[0], DENSITY_EMIT_SPECIES)
+        for k in DENSITY_EMIT_SPECIES.names:
+            sp[k] = elec[k]
+        self.bin["density_emit"].write(rec.tobytes())
+        self.bin["density_emit_elec"].write(sp.tobytes())
+        top = absorb[absorb["where"] == WHERE_TOP]
+        t = np.empty(top.shape[0], DENSITY_ABSORB_TOP)
+        t["x"], t["y"] = top["x"] / LENGTH_SCALE, top["y"] / LENGTH_SCALE
+        for k in ("vx", "vy", "vz", "emitter", "section", "id"):
+            t[k] = top[k]
+        bot = absorb[absorb["where"] == WHERE_BOT]
+        b = np.empty(bot.shape[0], DENSITY_ABSORB_BOT)
+        b["x"], b["y"] = bot["x"] / LENGTH_SCALE, bot["y"] / LENGTH_SCALE
+        for k in ("emitter", "section", "id"):
+            b[k] = bot[k]
+        self.bin["density_absorb_top"].write(t.tobytes())
+        self.bin["density_absorb_bot"].write(b.tobytes())

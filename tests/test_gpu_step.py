@@ -177,6 +177,112 @@ def test_output_files_equal_oracle(Diode, oracle_mod, tmp_path):
         assert (tmp_path / "gpu" / name).read_bytes() == (tmp_path / "oracle" / name).read_bytes(), name
 
 
+def test_particle_records_equal_oracle(Diode, oracle_mod, tmp_path):
+    """Each step's emission and absorption particle records (the
+    density_*.bin writes) equal the oracle's bit for bit, in the same order,
+    and so do the density files written from them (planar FE with top
+    absorption, then the tip)."""
+    from rumdeed_amd import output as O
+
+    from rumdeed_amd.diode import PLANES_Z_NM
+
+    dd = Diode(4096)
+    so = oracle_mod.Sim(4096, nthreads=8)
+    _fe(dd, oracle_mod)
+    _fe(so, oracle_mod)
+    planes = [z * NM for z in PLANES_Z_NM]  # the input namelist's default planes
+    planes[2] = 199.0 * NM  # one just below the anode (200 nm)
+    dd.set_planes(planes)
+    so.set_planes(planes)
+    dd.set_logs(True)
+    nsteps = 45
+    kw = dict(densities=True, planes=planes, position_steps=nsteps)
+    fg = O.OutputFiles(str(tmp_path / "gpu"), O.EMIT_PLANAR_FE, 0.5e-15, 1600.0, 200 * NM,
+                       4096, **kw)
+    fo = O.OutputFiles(str(tmp_path / "oracle"), O.EMIT_PLANAR_FE, 0.5e-15, 1600.0, 200 * NM,
+                       4096, **kw)
+    absorbed = crossings = 0
+    for k in range(1, nsteps + 1):
+        # the reference's step: emission, move, Write_Position, removal
+        dd.do_emission(k, "fp64")
+        dd.update_position(k, "fp64")
+        sg = dd.get_state()
+        rg = dd.remove_particles(k)
+        so.step(k, phases=1)
+        so.step(k, phases=2)
+        s_o = so.state()
+        ro = so.step(k, phases=4)
+        assert_records_equal(rg, ro, f"step {k}")
+        fg.write_position(k, sg["pos"], sg["emitter"], sg["section"], sg["id"])
+        fo.write_position(k, s_o["pos"], s_o["emitter"], s_o["section"], s_o["id"])
+        eg, ag = dd.step_logs()
+        eo, ao = so.step_logs()
+        pg, po = dd.plane_logs(), so.plane_logs()
+        print(f"records step {k}: emitted={len(eg)} absorbed={len(ag)} crossings={len(pg)}",
+              flush=True)
+        assert eg.tobytes() == eo.tobytes(), f"emission records, step {k}"
+        assert ag.tobytes() == ao.tobytes(), f"absorption records, step {k}"
+        assert pg.tobytes() == po.tobytes(), f"plane records, step {k}"
+        absorbed += len(ag)
+        crossings += len(pg)
+        fg.write_particles(eg, ag)
+        fo.write_particles(eo, ao)
+        fg.write_planes(pg)
+        fo.write_planes(po)
+    fg.close()
+    fo.close()
+    assert absorbed > 0 and crossings > 100
+    names = sorted(os.listdir(tmp_path / "gpu"))
+    assert names == sorted(os.listdir(tmp_path / "oracle")) and "planes-3.bin" in names
+    for name in names:
+        a = (tmp_path / "gpu" / name).read_bytes()
+        assert a == (tmp_path / "oracle" / name).read_bytes(), name
+
+    V, dtip, rb, ht = 1000.0, 1000 * NM, 250 * NM, 500 * NM
+    dt_ = Diode(4096)
+    dt_.set_tip(V, dtip, rb, ht, True, box_top=1000 * NM)
+    dt_.set_time_step(0.25e-16)
+    dt_.set_seed(9)
+    dt_.set_emitter(3, 0, w_tip=4.7)
+    dt_.set_logs(True)
+    st = oracle_mod.Sim(4096, nthreads=8)
+    st.set_geometry(oracle_mod.geom_tip(V, dtip, rb, ht, True), 1000 * NM)
+    st.set_time_step(0.25e-16)
+    st.set_seed(9)
+    st.set_emitter(3, 0, w_tip=4.7)
+    for k in range(1, 7):
+        assert_records_equal(dt_.run(1, "fp64")[0], st.step(k), f"tip step {k}")
+        eg, ag = dt_.step_logs()
+        eo, ao = st.step_logs()
+        assert eg.tobytes() == eo.tobytes() and ag.tobytes() == ao.tobytes(), f"tip step {k}"
+
+
+def test_particle_records_mixed(Diode):
+    """Mixed precision: the records agree with the step's counts, ids run
+    on, and each absorbed particle has left the particle list."""
+    dd = Diode(4096)
+    dd.set_planar(1600.0, 200 * NM, True, 1)
+    dd.set_time_step(0.5e-15)
+    dd.set_seed(5)
+    dd.set_emitter(10, 0, -50 * NM, -50 * NM, 100 * NM, 100 * NM, wf=np.array([[4.7]]))
+    dd.set_logs(True)
+    next_id, absorbed = 0, 0
+    for k in range(1, 46):
+        ids_before = set(dd.get_state()["id"])
+        r = dd.run(1, "mixed")[0]
+        em, ab = dd.step_logs()
+        ids_after = set(dd.get_state()["id"])
+        assert len(em) == r.emitted and len(ab) == r.absorbed
+        assert np.sum(ab["where"] == 1) == r.absorbed_top
+        assert list(em["id"]) == list(range(next_id, next_id + len(em)))
+        next_id += len(em)
+        assert set(ab["id"]) == (ids_before | set(em["id"])) - ids_after
+        absorbed += len(ab)
+    assert absorbed > 0
+    dd.set_logs(False)
+    dd.run(1, "mixed")
+
+
 def test_planar_gtf_run_bit_exact(Diode, oracle_mod):
     """Thermal-field emission, checkerboard work function, Poisson supply,
     Maxwell-Boltzmann velocities."""

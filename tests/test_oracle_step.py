@@ -192,3 +192,66 @@ def test_poisson_counts(oracle_mod):
     z = (np.array(ns) - lam) / np.sqrt(lam)
     assert abs(z.mean()) < 4 / np.sqrt(len(z))
     assert 0.7 < z.var() < 1.4
+
+
+def test_particle_records(oracle_mod):
+    """The per-step particle records (the density_emit / density_absorb
+    writes): counts equal the step record's, ids run on from step to step,
+    planar FE sites sit 1 nm above the cathode, and every absorbed particle
+    is one that vanished, listed in index order with its position after the
+    Beeman update and the velocity it had before this step's velocity
+    update."""
+    s = oracle_mod.Sim(4096, nthreads=4)
+    s.set_geometry(oracle_mod.geom_planar(200 * NM, 1600.0, True, 1))
+    s.set_time_step(0.5e-15)
+    s.set_seed(5)
+    s.set_emitter(10, 0, -50 * NM, -50 * NM, 100 * NM, 100 * NM, wf=np.array([[4.7]]))
+    planes = np.array([5.0, 10.0, -1.0, 50.0, 75.0, 100.0, 125.0, 150.0, 199.0, 250.0]) * NM
+    s.set_planes(planes)
+    dt = 0.5e-15
+    next_id = 0
+    top = bot = crossings = 0
+    for k in range(1, 46):
+        before = s.state()
+        r = s.step(k)
+        em, ab = s.step_logs()
+        after = s.state()
+        # plane crossings from the states on either side of the step:
+        # z_prev < z_p < z for each particle in id (= index) order
+        zold = dict(zip(before["id"].tolist(), before["pos"][:, 2]))
+        zold.update(zip(em["id"].tolist(), em["z"]))
+        vold = dict(zip(before["id"].tolist(), map(tuple, before["vel"])))
+        vold.update({int(i): (0.0, 0.0, 0.0) for i in em["id"]})
+        znew = dict(zip(after["id"].tolist(), after["pos"][:, 2]))
+        znew.update(zip(ab["id"].tolist(), ab["z"]))
+        want = [(p + 1, i) for p, zp in enumerate(planes) if zp > 0
+                for i in sorted(zold) if zold[i] < zp < znew[i]]
+        pl = s.plane_logs()
+        assert [(int(a), int(b)) for a, b in zip(pl["plane"], pl["id"])] == want
+        for rec in pl:
+            assert (rec["vx"], rec["vy"], rec["vz"]) == vold[int(rec["id"])]
+        crossings += len(pl)
+        assert len(em) == r.emitted and len(ab) == r.absorbed
+        assert np.sum(ab["where"] == 1) == r.absorbed_top
+        assert np.sum(ab["where"] == 2) == r.absorbed_bot
+        assert list(em["id"]) == list(range(next_id, next_id + len(em)))
+        next_id += len(em)
+        assert np.all(em["z"] == 1.0 * NM) and np.all(em["species"] == 1)
+        assert np.all(em["emitter"] == 1) and np.all(em["section"] == 1)
+        ids = list(before["id"]) + list(em["id"])
+        gone = [i for i in ids if i not in set(after["id"])]
+        assert list(ab["id"]) == gone
+        idx = {int(i): j for j, i in enumerate(before["id"])}
+        for a in ab:
+            j = idx.get(int(a["id"]))
+            if j is None:  # emitted this step with zero velocity
+                assert a["vx"] == a["vy"] == a["vz"] == 0.0
+                continue
+            assert [a["vx"], a["vy"], a["vz"]] == list(before["vel"][j])
+            x = (before["pos"][j] + before["vel"][j] * dt +
+                 (4 * before["acc"][j] - before["acc_prev"][j]) / 6 * dt * dt)
+            assert abs(a["x"] - x[0]) <= 1e-15 * 200 * NM and abs(a["y"] - x[1]) <= 1e-15 * 200 * NM
+            assert (x[2] > 200 * NM) == (a["where"] == 1)
+        top += r.absorbed_top
+        bot += r.absorbed_bot
+    assert top > 0 and crossings > 100

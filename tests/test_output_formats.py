@@ -95,3 +95,68 @@ def test_run_files(tmp_path):
     tail = np.frombuffer(init[56:].tobytes(), "<i4")
     assert list(head) == [1.0, 1.0, 1.0, 1e-9, 1e-12, 1e-9 / 1e-12, 1.0]
     assert list(tail) == [4096, 1, 96 * 96, 1000]
+
+
+def test_density_files(tmp_path):
+    """density_*.bin: packed stream records in the reference's field order,
+    positions divided by length_scale; top and bottom absorptions go to
+    their own files, species files hold only their species."""
+    import struct
+
+    from rumdeed_amd import _lib as L
+
+    em = np.zeros(3, L.EMIT_RECORD)
+    em["x"], em["y"], em["z"] = [1e-9, 2e-9, -3e-9], [4e-9, 5e-9, 6e-9], 1e-9
+    em["emitter"], em["section"], em["id"], em["species"] = 1, [1, 2, 3], [10, 11, 12], 1
+    ab = np.zeros(3, L.ABSORB_RECORD)
+    ab["x"], ab["y"] = [7e-9, 8e-9, 9e-9], [-1e-9, -2e-9, -3e-9]
+    ab["vx"], ab["vy"], ab["vz"] = [1.0, 2.0, 3.0], [4.0, 5.0, 6.0], [7.0, 8.0, 9.0]
+    ab["emitter"], ab["section"], ab["id"], ab["where"] = 1, [4, 5, 6], [0, 1, 2], [1, 2, 1]
+    f = O.OutputFiles(str(tmp_path), O.EMIT_PLANAR_FE, 1e-15, 100.0, 1e-6, densities=True)
+    f.write_particles(em, ab)
+    f.write_particles(em[:1], ab[:0])
+    f.close()
+    b = (tmp_path / "density_emit.bin").read_bytes()
+    assert len(b) == 4 * 40
+    assert b[:40] == struct.pack("<3d4i", 1e-9 / 1e-9, 4e-9 / 1e-9, 1e-9 / 1e-9, 1, 1, 10, 1)
+    assert b[120:] == b[:40]
+    b = (tmp_path / "density_emit_elec.bin").read_bytes()
+    assert len(b) == 4 * 32 and b[32:64] == struct.pack("<3d2i", 2.0, 5.0, 1.0, 1, 11)
+    b = (tmp_path / "density_absorb_top.bin").read_bytes()
+    assert b == (struct.pack("<5d3i", 7e-9 / 1e-9, -1.0, 1.0, 4.0, 7.0, 1, 4, 0) +
+                 struct.pack("<5d3i", 9e-9 / 1e-9, -3e-9 / 1e-9, 3.0, 6.0, 9.0, 1, 6, 2))
+    b = (tmp_path / "density_absorb_bot.bin").read_bytes()
+    assert b == struct.pack("<2d3i", 8e-9 / 1e-9, -2e-9 / 1e-9, 1, 5, 1)
+    for name in ("density_ion", "density_emit_ion", "density_emit_atom", "density_absorb_recom"):
+        assert (tmp_path / (name + ".bin")).read_bytes() == b""
+
+
+def test_plane_and_position_files(tmp_path):
+    """planes-<k>.bin for each plane with height > 0 (numbered by position
+    in the list), records of the density_absorb_top layout; position.bin
+    with its (steps, 1) header and per-step (step, n) blocks of positions
+    in m."""
+    import struct
+
+    from rumdeed_amd import _lib as L
+
+    pl = np.zeros(3, L.PLANE_RECORD)
+    pl["x"], pl["y"] = [1e-9, 2e-9, 3e-9], [4e-9, 5e-9, 6e-9]
+    pl["vx"], pl["vy"], pl["vz"] = 1.5, 2.5, [3.5, 4.5, 5.5]
+    pl["emitter"], pl["section"], pl["id"], pl["plane"] = 1, 7, [3, 4, 3], [1, 1, 3]
+    f = O.OutputFiles(str(tmp_path), O.EMIT_PLANAR_FE, 1e-15, 100.0, 1e-6,
+                      planes=[5e-9, -1.0, 25e-9], position_steps=20)
+    f.write_planes(pl)
+    f.write_position(1, [[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]], [1, 1], [2, 3], [8, 9])
+    f.write_position(2, np.zeros((0, 3)), [], [], [])
+    f.close()
+    assert sorted(p.name for p in tmp_path.glob("planes-*")) == ["planes-1.bin", "planes-3.bin"]
+    assert (tmp_path / "planes-1.bin").read_bytes() == (
+        struct.pack("<5d3i", 1.0, 4.0, 1.5, 2.5, 3.5, 1, 7, 3) +
+        struct.pack("<5d3i", 2e-9 / 1e-9, 5e-9 / 1e-9, 1.5, 2.5, 4.5, 1, 7, 4))
+    assert (tmp_path / "planes-3.bin").read_bytes() == struct.pack(
+        "<5d3i", 3e-9 / 1e-9, 6e-9 / 1e-9, 1.5, 2.5, 5.5, 1, 7, 3)
+    assert (tmp_path / "position.bin").read_bytes() == (
+        struct.pack("<2i", 20, 1) + struct.pack("<2i", 1, 2) +
+        struct.pack("<3d3i", 1.0, 2.0, 3.0, 1, 2, 8) + struct.pack("<3d3i", 4.0, 5.0, 6.0, 1, 3, 9) +
+        struct.pack("<2i", 2, 0))
