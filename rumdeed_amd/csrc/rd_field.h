@@ -269,26 +269,27 @@ __device__ __forceinline__ bool wave_on_surface(int ic, bool act, float4 h1, flo
 // time, so that their latencies overlap instead of adding up.
 __device__ __forceinline__ void sum3_in_order(const double* __restrict__ P, size_t S, size_t ld,
                                               int n, double& s0, double& s1, double& s2) {
-  int k = 0;
-  for (; k + 8 <= n; k += 8) {
-    double a[8], b[8], c[8];
+  // batches of 16 partials (48 loads in flight), the last one predicated:
+  // the sums are latency-bound, so the number of round trips is what counts
+  constexpr int B = 16;
+  for (int k = 0; k < n; k += B) {
+    double a[B], b[B], c[B];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      a[u] = P[0 * S + (size_t)(k + u) * ld];
-      b[u] = P[1 * S + (size_t)(k + u) * ld];
-      c[u] = P[2 * S + (size_t)(k + u) * ld];
+    for (int u = 0; u < B; ++u) {
+      const bool in = k + u < n;
+      const size_t o = in ? (size_t)(k + u) * ld : 0;
+      a[u] = in ? P[0 * S + o] : 0.0;
+      b[u] = in ? P[1 * S + o] : 0.0;
+      c[u] = in ? P[2 * S + o] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      s0 = s0 + a[u];
-      s1 = s1 + b[u];
-      s2 = s2 + c[u];
+    for (int u = 0; u < B; ++u) {
+      if (k + u < n) {
+        s0 = s0 + a[u];
+        s1 = s1 + b[u];
+        s2 = s2 + c[u];
+      }
     }
-  }
-  for (; k < n; ++k) {
-    s0 = s0 + P[0 * S + (size_t)k * ld];
-    s1 = s1 + P[1 * S + (size_t)k * ld];
-    s2 = s2 + P[2 * S + (size_t)k * ld];
   }
 }
 

@@ -15,6 +15,10 @@ histogram), with the reference's Fortran edit descriptors:
 - ``ramo_current.dt``: Write_Ramo_Current (src/mod_pair.F90:812-837), with
   the mobility of Average_Velocities (src/mod_verlet.F90:425-446);
 - ``lifetime.dt``: Write_Life_Time (src/mod_pair.F90:774-784);
+- ``volt.dt``: Set_Voltage (src/mod_verlet.F90:2050-2057), step 0 at
+  start-up (src/main.F90:172) and then every step; its format has three
+  descriptors for four items, so the zero RF voltage goes on a record of
+  its own (format reversion);
 - ``init.bin``: Write_Initial_Variables (src/main.F90:928-935);
 - ``density_emit.bin``, ``density_emit_elec.bin`` (Add_Particle,
   src/mod_pair.F90:86-89 and 119-123), ``density_absorb_top.bin``,
@@ -117,6 +121,14 @@ def fortran_ES(x: float, w: int, d: int) -> str:
     return s.rjust(w) if len(s) <= w else "*" * w
 
 
+def volt_lines(cur_time: float, step: int, V_d: float, V_rf: float) -> list[str]:
+    """The records of one (ES12.4, tr2, i8, tr2, ES18.8) write of cur_time,
+    step, V_d, V_rf: the fourth item reverts to the start of the format on a
+    new record."""
+    return [fortran_ES(cur_time, 12, 4) + "  " + fortran_I(step, 8) + "  " +
+            fortran_ES(V_d, 18, 8), fortran_ES(V_rf, 12, 4)]
+
+
 # ---- one run's files -----------------------------------------------------
 class OutputFiles:
     """The out/ files of one simulation.
@@ -140,7 +152,7 @@ class OutputFiles:
         self.V = V
         os.makedirs(directory, exist_ok=True)
         names = ["emitted", "absorbed", "absorbed_top", "absorbed_bot", "absorbed_recom",
-                 "ramo_current"]
+                 "ramo_current", "volt"]
         if mode in (EMIT_PLANAR_FE, EMIT_PLANAR_GTF):
             names.append("field")
         self.f = {k: open(os.path.join(directory, k + ".dt"), "w") for k in names}
@@ -155,6 +167,7 @@ class OutputFiles:
         if position_steps is not None:
             self.bin["position"] = open(os.path.join(directory, "position.bin"), "wb")
         self.write_init_bin(max_particles)
+        self.f["volt"].write("\n".join(volt_lines(0.0, 0, V, 0.0)) + "\n")
 
     def close(self, life_time=None):
         """Close the files; with the lifetime histogram (Diode.life_time(),
@@ -191,6 +204,8 @@ class OutputFiles:
         step = r["step"]
         cur_time = self.dt * step / TIME_SCALE
         n_mid = r["n"] + r["absorbed"]  # after emission, before removal
+        # volt.dt (Set_Voltage, at the start of the position update)
+        self.f["volt"].write("\n".join(volt_lines(cur_time, step, self.V, 0.0)) + "\n")
         # emitted.dt: (E14.6, *(tr8, i6))
         ints = [step, r["emitted"], n_mid]
         if self.mode != EMIT_TIP_FE:

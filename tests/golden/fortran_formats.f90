@@ -1,5 +1,6 @@
 ! Prints sample values with the edit descriptors of RUMDEED's output files
-! (emitted.dt, field.dt, absorbed*.dt, ramo_current.dt, lifetime.dt), one
+! (emitted.dt, field.dt, absorbed*.dt, ramo_current.dt, lifetime.dt,
+! volt.dt), one
 ! record per line, so that tests/test_output_formats.py can compare the
 ! Python writer against a Fortran compiler's own formatting.
 program fortran_formats
@@ -13,5 +14,7 @@ program fortran_formats
     write (*, "(i8, tr2, E16.8, tr2, E16.8, tr2, E16.8)") i, x(i), -x(i), x(i)*3.0d0
     write (*, "(ES12.4, *(tr2, i8))") x(i), i, 1000*i, 0
     write (*, "(ES12.4, tr2, i6, tr2, i6)") x(i), i, 7*i
+    ! volt.dt: four items for three descriptors (format reversion)
+    write (*, "(ES12.4, tr2, i8, tr2, ES18.8)") x(i), i, 2.0d0*x(i) + 1.0d0, 0.0d0
   end do
 end program fortran_formats

@@ -27,6 +27,7 @@ def python_lines():
         out.append(O.fortran_ES(x, 12, 4) + "".join("  " + O.fortran_I(v, 8)
                                                     for v in (i, 1000 * i, 0)))
         out.append(O.fortran_ES(x, 12, 4) + "  " + O.fortran_I(i, 6) + "  " + O.fortran_I(7 * i, 6))
+        out.extend(O.volt_lines(x, i, 2.0 * x + 1.0, 0.0))
     return out
 
 
@@ -87,6 +88,10 @@ def test_run_files(tmp_path):
     assert np.allclose(ra[0, [2, 3, 4, 5, 6]], [1e-6, V, 104, 104, 0], rtol=1e-4)
     assert np.allclose(ra[0, 7], speed / (V / d), rtol=1e-4)  # mobility
     assert np.allclose(ra[0, 8:11], [speed, speed, 0.0], rtol=1e-4)
+    vt = (tmp_path / "volt.dt").read_text().splitlines()
+    assert len(vt) == 2 * 6  # step 0 and steps 1-5, two records each
+    assert vt[2] == O.fortran_ES(dt * 1 / 1e-12, 12, 4) + "  " + O.fortran_I(1, 8) + "  " + \
+        O.fortran_ES(V, 18, 8) and vt[3] == "  0.0000E+00"
     lt = np.loadtxt(tmp_path / "lifetime.dt")
     assert lt.shape == (1000, 4) and lt[4, 2] == 9 and lt[4, 1] == 5
     init = np.fromfile(tmp_path / "init.bin", dtype=np.uint8)
