@@ -944,8 +944,10 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
       const int jb = c * kChunk + wu * kJumpSlice;
       const int je = min(n, jb + kJumpSlice);
       if (act) {  // fz holds the sum without the factor -2 until the end
+        static_assert(kJumpSlice <= kSurfBlock, "a slice is one FP32 block");
         auto sweep = [&](auto n0) {
           constexpr bool N0 = decltype(n0)::value;
+          float acc = 0.f;  // N0: the slice's block sum
           int j = jb;
           for (; j + 4 <= je; j += 4) {  // four sources' loads in flight together
             float4 hh[4], ll[4];
@@ -955,12 +957,21 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
               ll[u] = load_uniform(lo, j + u);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-              fz += (double)surface_term32<N0>(h1, l1, hh[u], ll[u], nic, dnm);
+            for (int u = 0; u < 4; ++u) {
+              if (N0)
+                acc = surface_acc32(h1, l1, hh[u], ll[u], acc);
+              else
+                fz += (double)surface_term32<false>(h1, l1, hh[u], ll[u], nic, dnm);
+            }
           }
-          for (; j < je; ++j)
-            fz += (double)surface_term32<N0>(h1, l1, load_uniform(hi, j), load_uniform(lo, j), nic,
-                                             dnm);
+          for (; j < je; ++j) {
+            if (N0)
+              acc = surface_acc32(h1, l1, load_uniform(hi, j), load_uniform(lo, j), acc);
+            else
+              fz += (double)surface_term32<false>(h1, l1, load_uniform(hi, j), load_uniform(lo, j),
+                                                  nic, dnm);
+          }
+          if (N0) fz += (double)acc;
         };
         if (nic == 0)
           sweep(std::true_type{});
@@ -988,10 +999,14 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
         // one loop per form, so that no branch sits in the source loop
         if (surf) {  // fz holds the sum without the factor -2 until the end
           const float dnm = 0.5f * two_d;
-          if (nic == 0)
-            for (int jj = j0; jj < j1; jj += dj)
-              fz += (double)surface_term32<true>(h1, l1, shi[jj], slo[jj], 0, dnm);
-          else
+          if (nic == 0) {
+            // a lane's sources of the chunk: a slice of kJumpSlice, or at most
+            // kChunk / 64 by wavefront: one FP32 block either way
+            static_assert(kChunk / 64 <= kSurfBlock, "one FP32 block per chunk");
+            float acc = 0.f;
+            for (int jj = j0; jj < j1; jj += dj) acc = surface_acc32(h1, l1, shi[jj], slo[jj], acc);
+            fz += (double)acc;
+          } else
             for (int jj = j0; jj < j1; jj += dj)
               fz += (double)surface_term32<false>(h1, l1, shi[jj], slo[jj], nic, dnm);
         } else if (ic) {

@@ -253,6 +253,22 @@ __device__ __forceinline__ float surface_term32(const float4 h1, const float4 l1
   return h2.w * surface_sum32(h2.z, l2.w, dxy2, nic, dnm);
 }
 
+// With N_ic_max = 0: acc + q_j/q_0 z_b / r^3 for one source, in FP32.  The
+// callers add blocks of at most kSurfBlock such terms in FP32 and each block
+// sum to their FP64 total.  Every term of a block has the sign of the
+// charges (z_b >= 0, one species), so a block's FP32 sum is within
+// (kSurfBlock - 1) u = 15 * 2^-24 (< 1e-6) of its exact value; per source it
+// saves the conversion and the FP64 add.
+constexpr int kSurfBlock = 16;
+__device__ __forceinline__ float surface_acc32(const float4 h1, const float4 l1, const float4 h2,
+                                               const float4 l2, float acc) {
+  const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+  const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+  const float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+  const float ri = rsq(fmaxf(__builtin_fmaf(h2.z, h2.z, dxy2), kR2Min));
+  return __builtin_fmaf(h2.w, h2.z * (ri * ri * ri), acc);
+}
+
 // True when every active target of the wavefront lies on the plane z = 0 and
 // the image series is on: the wavefront may then use surface_term32.
 __device__ __forceinline__ bool wave_on_surface(int ic, bool act, float4 h1, float4 l1) {
@@ -405,8 +421,12 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
     const int cnt = min(kChunk, n - jb);
     if (act && surf) {  // fz holds the sum without the factor -2 until the end
       if (nic == 0)
-        for (int jj = 0; jj < cnt; ++jj)
-          fz += (double)surface_term32<true>(h1, l1, shi[jj], slo[jj], 0, dnm);
+        for (int jb2 = 0; jb2 < cnt; jb2 += kSurfBlock) {
+          const int je2 = min(cnt, jb2 + kSurfBlock);
+          float acc = 0.f;
+          for (int jj = jb2; jj < je2; ++jj) acc = surface_acc32(h1, l1, shi[jj], slo[jj], acc);
+          fz += (double)acc;
+        }
       else
         for (int jj = 0; jj < cnt; ++jj)
           fz += (double)surface_term32<false>(h1, l1, shi[jj], slo[jj], nic, dnm);
