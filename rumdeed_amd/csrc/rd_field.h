@@ -150,53 +150,71 @@ __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v);
 // contribution for zero offsets.
 constexpr float kR2Min = 1.0e-20f;
 
-// One image partner: accumulate 1/r^3 and dz/r^3.
-#define RD_PARTNER(DZ, W, WZ)                   \
+// One image partner of a series at dz = DZ: add 1/r^3 to the suffix sum S.
+#define RD_PARTNER_SUM(DZ, S)                   \
   {                                             \
     float _dz = (DZ);                           \
     float _r2 = __builtin_fmaf(_dz, _dz, dxy2); \
     float _ri = rsq(_r2);                       \
-    float _r3 = _ri * _ri * _ri;                \
-    W += _r3;                                   \
-    WZ = __builtin_fmaf(_dz, _r3, WZ);          \
+    S = __builtin_fmaf(_ri * _ri, _ri, S);      \
   }
 
 // Image-charge weights of one pair: returns (sum_same - sum_opp) of 1/r^3 in
 // wxy and of dz/r^3 in wz.  zsum = z_a + z_b, wsum = (d - z_a) + (d - z_b),
 // dzab = z_a - z_b.
+//
+// The partners n >= 1 of a series lie at dz = a + t_n and a - t_n, t_n = 2nd
+// (a = zsum for the opposite charges, dzab for the same charges), so
+//   sum dz/r^3 = a sum 1/r^3 + 2d sum_n n (1/r+^3 - 1/r-^3),
+// and the n-weighted sum is the sum over k of the suffix sums
+// sum_{n >= k} (1/r+^3 - 1/r-^3), built with n descending.  A partner then
+// costs one fma into its suffix sum in place of forming dz/r^3 and adding it
+// twice.  Every partner in that form has |dz| >= t_n / 2, so a r^-3 and
+// t_n r^-3 do not cancel.  The opposite partner across the anode, at
+// zsum - 2d = -wsum, keeps its own term: it is close to the target for
+// particles near the anode.
 __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, float dxy2,
                                             int nic, float two_d, float& wxy, float& wz) {
-  float wo = 0.f, wzo = 0.f, ws = 0.f, wzs = 0.f;
+  float wo, wzo;
   {
-    float dz0 = zsum;
-    float r2 = fmaxf(__builtin_fmaf(dz0, dz0, dxy2), kR2Min);
+    float r2 = fmaxf(__builtin_fmaf(zsum, zsum, dxy2), kR2Min);
     float ri = rsq(r2);
-    float r3 = ri * ri * ri;
-    wo += r3;
-    wzo = __builtin_fmaf(dz0, r3, wzo);
+    wo = ri * ri * ri;
+    wzo = zsum * wo;
   }
-  if (nic >= 1) {
-    // n = 1: the opposite partner across the anode is at distance -(wsum)
+  if (nic < 1) {
+    wxy = -wo;
+    wz = -wzo;
+    return;
+  }
+  {
     float dz1 = -wsum;
     float r2 = fmaxf(__builtin_fmaf(dz1, dz1, dxy2), kR2Min);
     float ri = rsq(r2);
     float r3 = ri * ri * ri;
     wo += r3;
     wzo = __builtin_fmaf(dz1, r3, wzo);
-    RD_PARTNER(zsum + two_d, wo, wzo);
-    RD_PARTNER(dzab - two_d, ws, wzs);
-    RD_PARTNER(dzab + two_d, ws, wzs);
   }
-  float tn = two_d;
-  for (int n = 2; n <= nic; ++n) {
-    tn += two_d;
-    RD_PARTNER(zsum - tn, wo, wzo);
-    RD_PARTNER(zsum + tn, wo, wzo);
-    RD_PARTNER(dzab - tn, ws, wzs);
-    RD_PARTNER(dzab + tn, ws, wzs);
+  // suffix sums of 1/r^3: p (a + t_n) and m (a - t_n), o(pposite) and s(ame)
+  float po = 0.f, mo = 0.f, ps = 0.f, ms = 0.f, ro = 0.f, rs = 0.f;
+  float tn = (float)nic * two_d;
+  for (int n = nic; n >= 2; --n) {
+    RD_PARTNER_SUM(zsum + tn, po);
+    RD_PARTNER_SUM(zsum - tn, mo);
+    RD_PARTNER_SUM(dzab + tn, ps);
+    RD_PARTNER_SUM(dzab - tn, ms);
+    ro = (ro + po) - mo;
+    rs = (rs + ps) - ms;
+    tn -= two_d;
   }
-  wxy = ws - wo;
-  wz = wzs - wzo;
+  RD_PARTNER_SUM(zsum + two_d, po);
+  RD_PARTNER_SUM(dzab + two_d, ps);
+  RD_PARTNER_SUM(dzab - two_d, ms);
+  ro = (ro + po) - mo;
+  rs = (rs + ps) - ms;
+  const float so = po + mo, ss = ps + ms;
+  wxy = ss - (wo + so);
+  wz = __builtin_fmaf(dzab, ss, two_d * rs) - (__builtin_fmaf(zsum, so, wzo) + two_d * ro);
 }
 
 // Direct plus image-charge weight of one pair whose target lies on the
