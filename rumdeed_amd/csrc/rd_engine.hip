@@ -887,13 +887,13 @@ __host__ __device__ inline int jump_groups32(int m) {
                          : (m + kJumpTargets - 1) / kJumpTargets;
 }
 
-template <int KIND>
-__global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* tab, int ne,
+template <int KIND, int PPS>
+__global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* tab, int ne,
                                                                 const SimDesc* desc,
                                                                 const SimState* st, Chains ch,
                                                                 int it, JumpArgs a) {
   __shared__ float4 shi[kChunk], slo[kChunk];
-  __shared__ double red[3][kJumpWaves][kJumpTargets];
+  __shared__ double red[PPS][3][kJumpWaves][kJumpTargets];  // one buffer per partial
   const Ent e = tab[find_entry(tab, ne, blockIdx.x)];
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != KIND || it > s.ndim) return;
@@ -901,7 +901,7 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
   const bool by_wave = jump_by_wave(m);
   const int gx = jump_groups32(m);
   const int local = blockIdx.x - e.wg0;
-  const int bx = local % gx, by = local / gx;  // chain group, source segment
+  const int bx = local % gx, by = local / gx;  // chain group, workgroup segment
   const DevGeom& g = s.g;
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
@@ -914,9 +914,6 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
     P = jump_proposal(e, s, st, ch, it, k, a);
     if (by == 0 && (by_wave ? lane == 0 : w == 0)) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
   }
-  const int c0 = by * e.cps;
-  const int c1 = min(nchunks, c0 + e.cps);
-  double fx = 0.0, fy = 0.0, fz = 0.0;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
   float4 ih1 = make_float4(0, 0, 0, 1), il1 = make_float4(0, 0, 0, 0);
   if (act) {
@@ -937,115 +934,124 @@ __global__ __launch_bounds__(64 * kJumpWaves, 2) void k_mh_field32(const Ent* ta
   // uniform over the workgroup.
   const bool scalar_src = KIND == RD_GEOM_PLANAR && !by_wave &&
                           __syncthreads_or(surf ? 1 : 0) && __syncthreads_and(surf ? 1 : 0);
-  if (scalar_src) {
-    const int wu = __builtin_amdgcn_readfirstlane(w);
-    const float dnm = 0.5f * two_d;
-    for (int c = c0; c < c1; ++c) {
-      const int jb = c * kChunk + wu * kJumpSlice;
-      const int je = min(n, jb + kJumpSlice);
-      if (act) {  // fz holds the sum without the factor -2 until the end
-        static_assert(kJumpSlice <= kSurfBlock, "a slice is one FP32 block");
-        auto sweep = [&](auto n0) {
-          constexpr bool N0 = decltype(n0)::value;
-          float acc = 0.f;  // N0: the slice's block sum
-          int j = jb;
-          for (; j + 4 <= je; j += 4) {  // four sources' loads in flight together
-            float4 hh[4], ll[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              hh[u] = load_uniform(hi, j + u);
-              ll[u] = load_uniform(lo, j + u);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              if (N0)
-                acc = surface_acc32(h1, l1, hh[u], ll[u], acc);
-              else
-                fz += (double)surface_term32<false>(h1, l1, hh[u], ll[u], nic, dnm);
-            }
-          }
-          for (; j < je; ++j) {
-            if (N0)
-              acc = surface_acc32(h1, l1, load_uniform(hi, j), load_uniform(lo, j), acc);
-            else
-              fz += (double)surface_term32<false>(h1, l1, load_uniform(hi, j), load_uniform(lo, j),
-                                                  nic, dnm);
-          }
-          if (N0) fz += (double)acc;
-        };
-        if (nic == 0)
-          sweep(std::true_type{});
-        else
-          sweep(std::false_type{});
-      }
-    }
-  }
-  for (int c = scalar_src ? c1 : c0; c < c1; ++c) {
-    const int jb = c * kChunk;
-    __syncthreads();
-    if (t < kChunk && jb + t < n) {
-      shi[t] = hi[jb + t];
-      slo[t] = lo[jb + t];
-    }
-    __syncthreads();
-    const int cnt = min(kChunk, n - jb);
-    // sources of this lane in the chunk: a slice per wavefront, or a stride
-    // of 64 over the chunk per lane
-    const int j0 = by_wave ? lane : w * kJumpSlice;
-    const int j1 = by_wave ? cnt : min(cnt, j0 + kJumpSlice);
-    const int dj = by_wave ? 64 : 1;
-    if (act) {
-      if (KIND == RD_GEOM_PLANAR) {
-        // one loop per form, so that no branch sits in the source loop
-        if (surf) {  // fz holds the sum without the factor -2 until the end
-          const float dnm = 0.5f * two_d;
-          if (nic == 0) {
-            // a lane's sources of the chunk: a slice of kJumpSlice, or at most
-            // kChunk / 64 by wavefront: one FP32 block either way
-            static_assert(kChunk / 64 <= kSurfBlock, "one FP32 block per chunk");
-            float acc = 0.f;
-            for (int jj = j0; jj < j1; jj += dj) acc = surface_acc32(h1, l1, shi[jj], slo[jj], acc);
-            fz += (double)acc;
-          } else
-            for (int jj = j0; jj < j1; jj += dj)
-              fz += (double)surface_term32<false>(h1, l1, shi[jj], slo[jj], nic, dnm);
-        } else if (ic) {
-          for (int jj = j0; jj < j1; jj += dj)
-            mixed_pair(h1, l1, shi[jj], slo[jj], 1, nic, two_d, fx, fy, fz);
-        } else {
-          for (int jj = j0; jj < j1; jj += dj)
-            mixed_pair(h1, l1, shi[jj], slo[jj], 0, 0, two_d, fx, fy, fz);
-        }
-      } else {
-        for (int jj = j0; jj < j1; jj += dj) {
-          float cx, cy, cz;
-          tip_pair32_target(h1, l1, ih1, il1, shi[jj], slo[jj], kt, Rn, cx, cy, cz);
-          fx += (double)cx;
-          fy += (double)cy;
-          fz += (double)cz;
-        }
-      }
-    }
-  }
-  if (surf) fz = -2.0 * fz;
-  red[0][w][lane] = fx;
-  red[1][w][lane] = fy;
-  red[2][w][lane] = fz;
-  __syncthreads();
   double* part = a.part + e.part_off;
   const size_t S = (size_t)e.segs * m;
-  if (by_wave) {
-    // lane q of wavefront w adds component q over the 64 lanes, in lane order
-    if (lane < 3 && act) {
-      double v = red[lane][w][0];
-      for (int u = 1; u < 64; ++u) v += red[lane][w][u];
-      part[lane * S + (size_t)by * m + k] = v;
+#pragma unroll
+  for (int q = 0; q < PPS; ++q) {
+    const int sg = by * PPS + q;  // the partial
+    if (sg >= e.segs) break;
+    const int c0 = sg * e.cps;
+    const int c1 = min(nchunks, c0 + e.cps);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (scalar_src) {
+      const int wu = __builtin_amdgcn_readfirstlane(w);
+      const float dnm = 0.5f * two_d;
+      for (int c = c0; c < c1; ++c) {
+        const int jb = c * kChunk + wu * kJumpSlice;
+        const int je = min(n, jb + kJumpSlice);
+        if (act) {  // fz holds the sum without the factor -2 until the end
+          static_assert(kJumpSlice <= kSurfBlock, "a slice is one FP32 block");
+          auto sweep = [&](auto n0) {
+            constexpr bool N0 = decltype(n0)::value;
+            float acc = 0.f;  // N0: the slice's block sum
+            int j = jb;
+            for (; j + 4 <= je; j += 4) {  // four sources' loads in flight together
+              float4 hh[4], ll[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                hh[u] = load_uniform(hi, j + u);
+                ll[u] = load_uniform(lo, j + u);
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                if (N0)
+                  acc = surface_acc32(h1, l1, hh[u], ll[u], acc);
+                else
+                  fz += (double)surface_term32<false>(h1, l1, hh[u], ll[u], nic, dnm);
+              }
+            }
+            for (; j < je; ++j) {
+              if (N0)
+                acc = surface_acc32(h1, l1, load_uniform(hi, j), load_uniform(lo, j), acc);
+              else
+                fz += (double)surface_term32<false>(h1, l1, load_uniform(hi, j),
+                                                    load_uniform(lo, j), nic, dnm);
+            }
+            if (N0) fz += (double)acc;
+          };
+          if (nic == 0)
+            sweep(std::true_type{});
+          else
+            sweep(std::false_type{});
+        }
+      }
     }
-  } else if (w < 3 && act) {
-    // wavefront q adds component q over the wavefronts, in wavefront order
-    double v = red[w][0][lane];
-    for (int u = 1; u < kJumpWaves; ++u) v += red[w][u][lane];
-    part[w * S + (size_t)by * m + k] = v;
+    for (int c = scalar_src ? c1 : c0; c < c1; ++c) {
+      const int jb = c * kChunk;
+      __syncthreads();
+      if (t < kChunk && jb + t < n) {
+        shi[t] = hi[jb + t];
+        slo[t] = lo[jb + t];
+      }
+      __syncthreads();
+      const int cnt = min(kChunk, n - jb);
+      // sources of this lane in the chunk: a slice per wavefront, or a stride
+      // of 64 over the chunk per lane
+      const int j0 = by_wave ? lane : w * kJumpSlice;
+      const int j1 = by_wave ? cnt : min(cnt, j0 + kJumpSlice);
+      const int dj = by_wave ? 64 : 1;
+      if (act) {
+        if (KIND == RD_GEOM_PLANAR) {
+          // one loop per form, so that no branch sits in the source loop
+          if (surf) {  // fz holds the sum without the factor -2 until the end
+            const float dnm = 0.5f * two_d;
+            if (nic == 0) {
+              // a lane's sources of the chunk: a slice of kJumpSlice, or at most
+              // kChunk / 64 by wavefront: one FP32 block either way
+              static_assert(kChunk / 64 <= kSurfBlock, "one FP32 block per chunk");
+              float acc = 0.f;
+              for (int jj = j0; jj < j1; jj += dj)
+                acc = surface_acc32(h1, l1, shi[jj], slo[jj], acc);
+              fz += (double)acc;
+            } else
+              for (int jj = j0; jj < j1; jj += dj)
+                fz += (double)surface_term32<false>(h1, l1, shi[jj], slo[jj], nic, dnm);
+          } else if (ic) {
+            for (int jj = j0; jj < j1; jj += dj)
+              mixed_pair(h1, l1, shi[jj], slo[jj], 1, nic, two_d, fx, fy, fz);
+          } else {
+            for (int jj = j0; jj < j1; jj += dj)
+              mixed_pair(h1, l1, shi[jj], slo[jj], 0, 0, two_d, fx, fy, fz);
+          }
+        } else {
+          for (int jj = j0; jj < j1; jj += dj) {
+            float cx, cy, cz;
+            tip_pair32_target(h1, l1, ih1, il1, shi[jj], slo[jj], kt, Rn, cx, cy, cz);
+            fx += (double)cx;
+            fy += (double)cy;
+            fz += (double)cz;
+          }
+        }
+      }
+    }
+    if (surf) fz = -2.0 * fz;
+    red[q][0][w][lane] = fx;
+    red[q][1][w][lane] = fy;
+    red[q][2][w][lane] = fz;
+    __syncthreads();
+    if (by_wave) {
+      // lane q of wavefront w adds component q over the 64 lanes, in lane order
+      if (lane < 3 && act) {
+        double v = red[q][lane][w][0];
+        for (int u = 1; u < 64; ++u) v += red[q][lane][w][u];
+        part[lane * S + (size_t)sg * m + k] = v;
+      }
+    } else if (w < 3 && act) {
+      // wavefront q adds component q over the wavefronts, in wavefront order
+      double v = red[q][w][0][lane];
+      for (int u = 1; u < kJumpWaves; ++u) v += red[q][w][u][lane];
+      part[w * S + (size_t)sg * m + k] = v;
+    }
   }
 }
 
@@ -2053,10 +2059,27 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
       }
       e.part_off = off;
       off += 3LL * (fp64 ? std::max(nch, 1) : e.segs) * e.nch;
-      e.wg0 = H.nwg;
-      H.nwg += gx * e.segs;
     }
     nsettle_all += H.nsettle;
+  }
+  // Partials per mixed workgroup: one, or two when the launches are large
+  // enough to keep every CU busy with half as many workgroups (each then
+  // pays the proposal and set-up once for twice the sources).  The partials
+  // themselves keep their size, so the sums do not depend on this choice.
+  int pps = 1;
+  if (!fp64 && c->nslots_max > 1) {
+    long long wg_all = 0;
+    for (int h = 0; h < nh; ++h)
+      for (const Ent& e : hv[h].ents) wg_all += (long long)jump_groups32(e.nch) * e.segs;
+    if (wg_all >= kJumpPairWgs) pps = 2;
+  }
+  for (int h = 0; h < nh; ++h) {
+    Half& H = hv[h];
+    for (Ent& e : H.ents) {
+      const int gx = fp64 ? (e.nch + per - 1) / per : jump_groups32(e.nch);
+      e.wg0 = H.nwg;
+      H.nwg += gx * ((e.segs + pps - 1) / pps);
+    }
   }
   r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
   if (r) return r;
@@ -2144,12 +2167,22 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
         k_mh_settle<RD_PREC_FP64><<<H.nsettle, kSettle, 0, H.stream>>>(
             H.dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, H.a);
       } else {
-        if (H.any_planar)
-          k_mh_field32<RD_GEOM_PLANAR><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
-              H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
-        if (H.any_tip)
-          k_mh_field32<RD_GEOM_TIP><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
-              H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+        if (H.any_planar) {
+          if (pps == 2)
+            k_mh_field32<RD_GEOM_PLANAR, 2><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+                H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+          else
+            k_mh_field32<RD_GEOM_PLANAR, 1><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+                H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+        }
+        if (H.any_tip) {
+          if (pps == 2)
+            k_mh_field32<RD_GEOM_TIP, 2><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+                H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+          else
+            k_mh_field32<RD_GEOM_TIP, 1><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+                H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
+        }
         k_mh_settle<RD_PREC_MIXED><<<H.nsettle, kSettle, 0, H.stream>>>(
             H.dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, H.a);
       }

@@ -230,7 +230,11 @@ inline int seg_limit(int target_blocks, int want) {
 }
 constexpr int kWantPair = 8192;  // accelerations, field at points
 constexpr int kWantJump = 512;   // fields of the Metropolis-Hastings chains
-constexpr int kPoolJumpChunks = 4;  // least source chunks per jump workgroup in a pool
+constexpr int kPoolJumpChunks = 4;  // least source chunks per jump partial in a pool
+// Jump workgroups (one partial each) from which a pool's mixed jump launches
+// give every workgroup two partials: the two launches of a jump then still
+// hold about two workgroups per CU each.
+constexpr int kJumpPairWgs = 2048;
 int launch_grid_points(rd_ctx* c, double x0, double y0, double lx, double ly, int nx, int ny,
                        double zz);
 
