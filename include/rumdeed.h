@@ -233,6 +233,13 @@ int rd_pool_step(rd_ctx* ctx, int32_t precision, rd_step_record* records);
  * either way.  Library control, no reference counterpart. */
 int rd_set_graphs(rd_ctx* ctx, int32_t on);
 
+/* Pool tuning: the mixed-precision jump launches of a step give each
+ * workgroup two partial sums of the chain fields once they hold at least
+ * min_workgroups workgroups of one partial each (2048 by default; 0: always).
+ * The partials keep their size and order, so results are identical for every
+ * threshold.  Library control, no reference counterpart. */
+int rd_set_jump_pairing(rd_ctx* ctx, int64_t min_workgroups);
+
 /* Collect the particle records of each step (on != 0) or not (0, the
  * default).  Library control, no reference counterpart: the reference
  * writes these records to files inside Add_Particle and

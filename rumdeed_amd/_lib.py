@@ -129,6 +129,7 @@ _SIGS = {
                                      c_int32, c_int32, c_double, c_int32, c_int32, _dp, _dp]),
     "rd_time_copy": (c_int32, [c_void_p, c_int64, c_int32, _dp]),
     "rd_set_graphs": (c_int32, [c_void_p, c_int32]),
+    "rd_set_jump_pairing": (c_int32, [c_void_p, c_int64]),
     "rd_set_logs": (c_int32, [c_void_p, c_int32]),
     "rd_get_logs": (c_int32, [c_void_p, c_int32, c_void_p, _ip, c_int32, c_void_p, _ip]),
     "rd_set_planes": (c_int32, [c_void_p, c_int32, c_void_p]),

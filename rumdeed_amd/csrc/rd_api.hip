@@ -958,6 +958,13 @@ int rd_set_graphs(rd_ctx* c, int32_t on) {
   return RD_OK;
 }
 
+int rd_set_jump_pairing(rd_ctx* c, int64_t min_workgroups) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  if (min_workgroups < 0) return fail(RD_ERR_ARG, "negative workgroup threshold");
+  c->jump_pair_wgs = min_workgroups;
+  return RD_OK;
+}
+
 int rd_time_copy(rd_ctx* c, int64_t bytes, int32_t reps, double* kernel_ms) {
   if (!c) return fail(RD_ERR_ARG, "null context");
   if (bytes < 16 || bytes % 16 != 0 || bytes > (int64_t(1) << 34))

@@ -2071,7 +2071,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     long long wg_all = 0;
     for (int h = 0; h < nh; ++h)
       for (const Ent& e : hv[h].ents) wg_all += (long long)jump_groups32(e.nch) * e.segs;
-    if (wg_all >= kJumpPairWgs) pps = 2;
+    if (wg_all >= c->jump_pair_wgs) pps = 2;
   }
   for (int h = 0; h < nh; ++h) {
     Half& H = hv[h];

@@ -145,6 +145,11 @@ struct Slot {
 
 }  // namespace rd
 
+// Jump workgroups (one partial each) from which a pool's mixed jump launches
+// give every workgroup two partials: the two launches of a jump then still
+// hold about two workgroups per CU each.
+constexpr int kJumpPairWgs = 2048;
+
 struct rd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -192,6 +197,9 @@ struct rd_ctx {
   // while they are unchanged.
   int graphs = 0;
   hipGraphExec_t jump_exec = nullptr;
+  // Jump workgroups from which a pool's mixed jump launches pair their
+  // partials (rd_set_jump_pairing; kJumpPairWgs by default).
+  long long jump_pair_wgs = kJumpPairWgs;
   std::vector<unsigned char> jump_key;
 
   // further streams of the pool jump loop (default priority) and the events
@@ -231,10 +239,6 @@ inline int seg_limit(int target_blocks, int want) {
 constexpr int kWantPair = 8192;  // accelerations, field at points
 constexpr int kWantJump = 512;   // fields of the Metropolis-Hastings chains
 constexpr int kPoolJumpChunks = 4;  // least source chunks per jump partial in a pool
-// Jump workgroups (one partial each) from which a pool's mixed jump launches
-// give every workgroup two partials: the two launches of a jump then still
-// hold about two workgroups per CU each.
-constexpr int kJumpPairWgs = 2048;
 int launch_grid_points(rd_ctx* c, double x0, double y0, double lx, double ly, int nx, int ny,
                        double zz);
 

@@ -111,6 +111,11 @@ class Diode:
         """Launch each step's Metropolis-Hastings jump loop as one HIP graph."""
         check(self._lib.rd_set_graphs(self._h, int(bool(on))))
 
+    def set_jump_pairing(self, min_workgroups: int):
+        """Pool launches of at least min_workgroups jump workgroups pair their
+        partial sums (results unchanged; 0: always)."""
+        check(self._lib.rd_set_jump_pairing(self._h, int(min_workgroups)))
+
     def set_seed(self, seed: int, stream: int = 0):
         check(self._lib.rd_set_seed(self._h, int(seed), int(stream)))
 
