@@ -231,15 +231,10 @@ __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, 
 // which the callers apply once to their FP64 total (exactly: a power of two).
 // zb = z_b, wb = d - z_b and dnm = d (nm); the partners below the cathode are
 // formed from wb, as ic_planar32 forms the anode image.
-__device__ __forceinline__ float surface_sum32(float zb, float wb, float dxy2, int nic,
-                                               float dnm) {
+// surface_far32: w plus the partners k != 0 of that series, one by one.
+__device__ __forceinline__ float surface_far32(float zb, float wb, float dxy2, int nic, float dnm,
+                                               float w) {
   const float two_d = dnm + dnm;
-  float w = 0.f;
-  {
-    float r2 = fmaxf(__builtin_fmaf(zb, zb, dxy2), kR2Min);
-    float ri = rsq(r2);
-    w = zb * (ri * ri * ri);
-  }
   float up = zb, um = -(wb + dnm);
   for (int n = 1; n <= nic; ++n) {
     up += two_d;
@@ -252,6 +247,61 @@ __device__ __forceinline__ float surface_sum32(float zb, float wb, float dxy2, i
     um -= two_d;
   }
   return w;
+}
+
+__device__ __forceinline__ float surface_sum32(float zb, float wb, float dxy2, int nic,
+                                               float dnm) {
+  float r2 = fmaxf(__builtin_fmaf(zb, zb, dxy2), kR2Min);
+  float ri = rsq(r2);
+  return surface_far32(zb, wb, dxy2, nic, dnm, zb * (ri * ri * ri));
+}
+
+// The partners k != 0 of the series above as a power series in s = dxy2.
+// They lie at |u_k| >= d, so
+//   far(s) = sum_{k != 0} u_k / (s + u_k^2)^(3/2) = sum_m c_m s^m,
+//   c_m = binom(-3/2, m) sum_{k != 0} sgn(u_k) (u_k^2)^-(m+1),
+// and each partner's series in s / u_k^2 <= s / d^2 alternates with
+// decreasing terms.  With kFarTerms terms and s <= kFarS d^2 the truncation
+// stays below |binom(-3/2, 8)| 0.1^8 = 3.4e-8 of each partner's term.  The
+// coefficients are FP64 sums over the partners rounded to FP32 once
+// (surface_far_coeffs); pairs beyond kFarS d^2 take surface_far32.
+constexpr int kFarTerms = 8;
+constexpr float kFarS = 0.1f;
+
+// c_0 .. c_7 of one source: zb = z_b, wb = d - z_b, dnm = d (nm).
+__device__ __forceinline__ void surface_far_coeffs(double zb, double wb, double dnm, int nic,
+                                                   float4& c0, float4& c1) {
+  double S[kFarTerms];
+  for (int m = 0; m < kFarTerms; ++m) S[m] = 0.0;
+  for (int k = 1; k <= nic; ++k) {
+    const double up = zb + 2.0 * k * dnm;          // u_k
+    const double um = wb + (2.0 * k - 1.0) * dnm;  // |u_-k| = 2kd - z_b
+    const double ia = 1.0 / (up * up), ib = 1.0 / (um * um);
+    double pa = ia, pb = ib;
+    for (int m = 0; m < kFarTerms; ++m) {
+      S[m] += pa - pb;
+      pa *= ia;
+      pb *= ib;
+    }
+  }
+  float c[kFarTerms];
+  double b = 1.0;  // binom(-3/2, m)
+  for (int m = 0; m < kFarTerms; ++m) {
+    c[m] = (float)(b * S[m]);
+    b *= (-1.5 - m) / (m + 1.0);
+  }
+  c0 = make_float4(c[0], c[1], c[2], c[3]);
+  c1 = make_float4(c[4], c[5], c[6], c[7]);
+}
+
+__device__ __forceinline__ float surface_far_eval(float s, float4 c0, float4 c1) {
+  float v = __builtin_fmaf(c1.w, s, c1.z);
+  v = __builtin_fmaf(v, s, c1.y);
+  v = __builtin_fmaf(v, s, c1.x);
+  v = __builtin_fmaf(v, s, c0.w);
+  v = __builtin_fmaf(v, s, c0.z);
+  v = __builtin_fmaf(v, s, c0.y);
+  return __builtin_fmaf(v, s, c0.x);
 }
 
 // q_j/q_0 times surface_sum32 for the packed target (h1, l1) on the cathode
@@ -417,12 +467,14 @@ __device__ __forceinline__ void field_finish64(const DevGeom& g, const double* _
 // of chunks [c0, c1) at the packed target (h1, l1).
 __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h1, float4 l1,
                                             const float4* __restrict__ hi,
-                                            const float4* __restrict__ lo, int n, int c0, int c1,
-                                            float4* shi, float4* slo, double& fx, double& fy,
-                                            double& fz) {
+                                            const float4* __restrict__ lo,
+                                            const float4* __restrict__ far, int n, int c0,
+                                            int c1, float4* shi, float4* slo, float4* sfar,
+                                            double& fx, double& fy, double& fz) {
   const int ic = g.ic, nic = g.nic;
   const float two_d = (float)(2.0 * g.d * 1e9);
   const float dnm = (float)(g.d * 1e9);
+  const float slim = kFarS * dnm * dnm;
   const bool surf = wave_on_surface(ic, act, h1, l1);
   const int t = threadIdx.x;
   fx = 0.0;
@@ -434,11 +486,29 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
     if (jb + t < n) {
       shi[t] = hi[jb + t];
       slo[t] = lo[jb + t];
+      if (far) {
+        sfar[2 * t] = far[2 * (jb + t)];
+        sfar[2 * t + 1] = far[2 * (jb + t) + 1];
+      }
     }
     __syncthreads();
     const int cnt = min(kChunk, n - jb);
     if (act && surf) {  // fz holds the sum without the factor -2 until the end
-      if (nic == 0)
+      if (nic > 0 && far)
+        for (int jj = 0; jj < cnt; ++jj) {
+          const float4 h2 = shi[jj], l2 = slo[jj];
+          const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+          const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+          const float s = __builtin_fmaf(dy, dy, dx * dx);
+          const float ri = rsq(fmaxf(__builtin_fmaf(h2.z, h2.z, s), kR2Min));
+          float w = h2.z * (ri * ri * ri);
+          if (s <= slim)
+            w += surface_far_eval(s, sfar[2 * jj], sfar[2 * jj + 1]);
+          else
+            w = surface_far32(h2.z, l2.w, s, nic, dnm, w);
+          fz += (double)(h2.w * w);
+        }
+      else if (nic == 0)
         for (int jb2 = 0; jb2 < cnt; jb2 += kSurfBlock) {
           const int je2 = min(cnt, jb2 + kSurfBlock);
           float acc = 0.f;

@@ -177,6 +177,7 @@ struct rd_ctx {
 
   // work buffers (grown on demand, never shrunk)
   rd::Buf part, pack_hi, pack_lo, pts, fld, pts_hi, pts_lo, stage, img64, tab, red, chains;
+  rd::Buf pack_far;  // far-partner series of the particles (surface_far_coeffs)
   rd::Buf rnd;  // random numbers of the Metropolis-Hastings jumps of a step
   rd::Buf cnt;  // accept / reject counts per accept workgroup of a jump
   rd::Buf img_hi, img_lo;  // packed sphere images of the particles (mixed, tip)

@@ -301,6 +301,24 @@ __global__ void k_pack_particles(const Ent* tab, const SimDesc* desc, const SimS
   lo[gi] = l;
 }
 
+// Far-partner coefficients (surface_far_coeffs) of the particles of every
+// planar entry with image partners beyond n = 0: far[2 i], far[2 i + 1].
+__global__ void k_pack_far(const Ent* tab, const SimDesc* desc, const SimState* st,
+                           const double* __restrict__ Z, float4* __restrict__ far) {
+  const Ent e = tab[blockIdx.z];
+  const DevGeom& g = desc[e.slot].g;
+  if (g.kind != RD_GEOM_PLANAR || !g.ic || g.nic < 1) return;
+  const int n = st[e.slot].n;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t gi = (size_t)e.base + i;
+  const double zn = Z[gi] * 1.0e9, dnm = g.d * 1.0e9;
+  float4 c0, c1;
+  surface_far_coeffs(zn, dnm - zn, dnm, g.nic, c0, c1);
+  far[2 * gi] = c0;
+  far[2 * gi + 1] = c1;
+}
+
 // Pack the field points of every entry.
 __global__ void k_pack_points(const Ent* tab, const SimDesc* desc, const double* __restrict__ ptx,
                               const double* __restrict__ pty, const double* __restrict__ ptz,
@@ -728,8 +746,9 @@ __global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, int n
                                                            const float4* __restrict__ PLO,
                                                            const float4* __restrict__ HI,
                                                            const float4* __restrict__ LO,
+                                                           const float4* __restrict__ FAR,
                                                            double* __restrict__ part) {
-  __shared__ float4 shi[kChunk], slo[kChunk];
+  __shared__ float4 shi[kChunk], slo[kChunk], sfar[2 * kChunk];
   const Tile tl = tile_of(tab, ne, true);
   const Ent& e = tl.e;
   const int m = e.m;
@@ -749,7 +768,8 @@ __global__ __launch_bounds__(kBlock) void k_field_planar32(const Ent* tab, int n
     l1 = PLO[e.pbase + p];
   }
   double fx, fy, fz;
-  field_seg32(g, act, h1, l1, HI + e.base, LO + e.base, n, c0, c1, shi, slo, fx, fy, fz);
+  field_seg32(g, act, h1, l1, HI + e.base, LO + e.base, FAR ? FAR + 2 * (size_t)e.base : nullptr,
+              n, c0, c1, shi, slo, sfar, fx, fy, fz);
   if (act) {
     double* P = part + e.part_off;
     const size_t S = (size_t)e.segs * m;
@@ -1022,11 +1042,25 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
     k_pack_particles<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(
         dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (float4*)c->pack_hi.p,
         (float4*)c->pack_lo.p);
+  // far-partner series of the sources, for planar entries with image partners
+  // beyond n = 0 (used where a wavefront's points lie on the cathode)
+  bool any_far = false;
+  for (const Ent& e : ents) {
+    const DevGeom& g = c->slots[e.slot].desc.g;
+    any_far |= g.kind == RD_GEOM_PLANAR && g.ic && g.nic >= 1;
+  }
+  if (any_far && maxn > 0) {
+    r = ensure(c, c->pack_far, 2 * sizeof(float4) * (size_t)c->cap);
+    if (r) return r;
+    k_pack_far<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(
+        dt, c->d_desc, c->d_state, c->z, (float4*)c->pack_far.p);
+  }
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
   if (any_planar)
     k_field_planar32<<<grid, kBlock, 0, c->stream>>>(
         dt, ne, c->d_desc, c->d_state, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p,
-        (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p, part);
+        (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p,
+        any_far ? (const float4*)c->pack_far.p : nullptr, part);
   if (any_tip)
     k_field_tip32<<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state, ptx, pty, ptz,
                                                    (const float4*)c->pack_hi.p,

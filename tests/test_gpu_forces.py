@@ -179,6 +179,34 @@ def test_mixed_field_on_and_off_the_cathode(Diode, oracle_mod, nic):
     assert np.all(f[:, :2] == 0.0)
 
 
+@pytest.mark.parametrize("nic", [1, 10])
+def test_mixed_cathode_field_far_series_and_fallback(Diode, oracle_mod, nic):
+    """On the cathode the image partners k != 0 of a source are summed as a
+    power series in the squared lateral distance s while s <= 0.1 d^2, and
+    one by one beyond it.  Sources spread five gaps wide put the pairs of
+    every wavefront on both sides of that bound; the fields stay within 1e-5
+    of the long-double oracle."""
+    from rumdeed_amd import workloads as W
+
+    d = 200e-9
+    pos, q, m = W.snapshot(1500, seed=13)
+    pos[:, :2] *= 5.0  # lateral +-500 nm
+    pos[:, 2] *= d / W.SNAPSHOT_D  # z in (0, d]
+    dd = _diode_planar(Diode, pos, q, m, d, W.SNAPSHOT_V, 1, nic)
+    g = oracle_mod.geom_planar(d, W.SNAPSHOT_V, True, nic)
+    on = W.grid_points(16, 16, box=(-500e-9, -500e-9, 1000e-9, 1000e-9))
+    rho2 = ((on[:, None, :2] - pos[None, :, :2]) ** 2).sum(-1)
+    near = (rho2 <= 0.1 * d * d).mean()
+    assert 0.001 < near < 0.5, near
+    ref = oracle_mod.field(g, pos, q, on, long_double=True)
+    f = dd.calc_field_at_batch(on, "mixed")
+    e = rel_err(f, ref)
+    print(f"cathode field nic={nic}: {near:.3f} of the pairs in the series range, "
+          f"max {e.max():.2e}")
+    assert e.max() < MIXED_TOL
+    assert np.all(f[:, :2] == 0.0)
+
+
 def vacuum_accel(oracle_mod, g, p, q, m):
     return q / m * oracle_mod.vacuum_field(g, p)
 
