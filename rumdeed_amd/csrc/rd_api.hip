@@ -376,7 +376,7 @@ int rd_ctx_destroy(rd_ctx* c) {
   if (c->jump_exec) hipGraphExecDestroy(c->jump_exec);
   if (c->h_state) hipHostFree(c->h_state);
   if (c->h_tab) hipHostFree(c->h_tab);
-  Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pack_far, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
+  Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pack_far, &c->pack_srf, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
                  &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo, &c->cnt,
                  &c->mark, &c->elog, &c->alog, &c->plog};
   for (Buf* b : bufs) free_buf(*b);

@@ -721,6 +721,7 @@ struct JumpArgs {
   double* part;
   const double *X, *Y, *Z, *Q;    // FP64 sources
   const float4 *HI, *LO;          // packed sources (mixed)
+  const float4* SRF;              // (x, y, z, q z) high parts (mixed, cathode surface)
   const double *ru1, *ru2, *rlr;  // per jump and chain: unit normals, log-uniform ([it-1][C])
   int* counts;                    // [jump parity][accept workgroup][accepted, rejected]
   int C;                          // chains in all
@@ -922,6 +923,7 @@ __global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* ta
   }
   const float4* hi = a.HI + e.base;
   const float4* lo = a.LO + e.base;
+  const float4* srf = a.SRF + e.base;
   const int ic = g.ic, nic = g.nic;
   const float two_d = (float)(2.0 * g.d * 1e9);
   const float kt = g.ic ? (float)g.ktip : 0.0f;
@@ -959,20 +961,20 @@ __global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* ta
               float4 hh[4], ll[4];
 #pragma unroll
               for (int u = 0; u < 4; ++u) {
-                hh[u] = load_uniform(hi, j + u);
+                hh[u] = load_uniform(N0 ? srf : hi, j + u);
                 ll[u] = load_uniform(lo, j + u);
               }
 #pragma unroll
               for (int u = 0; u < 4; ++u) {
                 if (N0)
-                  acc = surface_acc32(h1, l1, hh[u], ll[u], acc);
+                  acc = surface_accq32(h1, l1, hh[u], ll[u], acc);
                 else
                   fz += (double)surface_term32<false>(h1, l1, hh[u], ll[u], nic, dnm);
               }
             }
             for (; j < je; ++j) {
               if (N0)
-                acc = surface_acc32(h1, l1, load_uniform(hi, j), load_uniform(lo, j), acc);
+                acc = surface_accq32(h1, l1, load_uniform(srf, j), load_uniform(lo, j), acc);
               else
                 fz += (double)surface_term32<false>(h1, l1, load_uniform(hi, j),
                                                     load_uniform(lo, j), nic, dnm);
@@ -1993,11 +1995,12 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   a.Q = c->q;
   int r = RD_OK;
   if (!fp64) {
-    r = launch_pack(c, ents);
+    r = launch_pack(c, ents, true);
     if (r) return r;
   }
   a.HI = (const float4*)c->pack_hi.p;
   a.LO = (const float4*)c->pack_lo.p;
+  a.SRF = (const float4*)c->pack_srf.p;
 
   // parts: consecutive runs of the table of about equal work
   struct Half {

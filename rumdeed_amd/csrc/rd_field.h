@@ -337,6 +337,17 @@ __device__ __forceinline__ float surface_acc32(const float4 h1, const float4 l1,
   return __builtin_fmaf(h2.w, h2.z * (ri * ri * ri), acc);
 }
 
+// surface_acc32 for a source s2 = (x, y, z, q_j/q_0 z) of high parts
+// (k_pack_particles' srf) and its low parts l2.
+__device__ __forceinline__ float surface_accq32(const float4 h1, const float4 l1, const float4 s2,
+                                                const float4 l2, float acc) {
+  const float dx = (h1.x - s2.x) + (l1.x - l2.x);
+  const float dy = (h1.y - s2.y) + (l1.y - l2.y);
+  const float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+  const float ri = rsq(fmaxf(__builtin_fmaf(s2.z, s2.z, dxy2), kR2Min));
+  return __builtin_fmaf(s2.w, ri * ri * ri, acc);
+}
+
 // True when every active target of the wavefront lies on the plane z = 0 and
 // the image series is on: the wavefront may then use surface_term32.
 __device__ __forceinline__ bool wave_on_surface(int ic, bool act, float4 h1, float4 l1) {

@@ -178,6 +178,7 @@ struct rd_ctx {
   // work buffers (grown on demand, never shrunk)
   rd::Buf part, pack_hi, pack_lo, pts, fld, pts_hi, pts_lo, stage, img64, tab, red, chains;
   rd::Buf pack_far;  // far-partner series of the particles (surface_far_coeffs)
+  rd::Buf pack_srf;  // (x, y, z, q z) high parts of the particles: cathode-surface sums
   rd::Buf rnd;  // random numbers of the Metropolis-Hastings jumps of a step
   rd::Buf cnt;  // accept / reject counts per accept workgroup of a jump
   rd::Buf img_hi, img_lo;  // packed sphere images of the particles (mixed, tip)
@@ -226,8 +227,9 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& ents, int precision, hipEven
 int launch_field(rd_ctx* c, const std::vector<Ent>& ents, int mtot, int precision, hipEvent_t k0,
                  hipEvent_t k1);
 int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab);
-// Packed mixed-precision copies (c->pack_hi / pack_lo) of every entry's particles.
-int launch_pack(rd_ctx* c, const std::vector<Ent>& ents);
+// Packed mixed-precision copies (c->pack_hi / pack_lo, and c->pack_srf when
+// `surface`) of every entry's particles.
+int launch_pack(rd_ctx* c, const std::vector<Ent>& ents, bool surface = false);
 // Split nchunks source chunks into `segs` (at most max_segs) segments of
 // `cps` chunks each.
 void segment(int nchunks, int max_segs, int& segs, int& cps);

@@ -289,7 +289,8 @@ __global__ void k_field_finish64(const Ent* tab, const SimDesc* desc, const SimS
 __global__ void k_pack_particles(const Ent* tab, const SimDesc* desc, const SimState* st,
                                  const double* __restrict__ X, const double* __restrict__ Y,
                                  const double* __restrict__ Z, const double* __restrict__ Q,
-                                 float4* __restrict__ hi, float4* __restrict__ lo) {
+                                 float4* __restrict__ hi, float4* __restrict__ lo,
+                                 float4* __restrict__ srf) {
   const Ent e = tab[blockIdx.z];
   const int n = st[e.slot].n;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -299,6 +300,7 @@ __global__ void k_pack_particles(const Ent* tab, const SimDesc* desc, const SimS
   pack1(X[gi], Y[gi], Z[gi], Q[gi], desc[e.slot].g.d, h, l);
   hi[gi] = h;
   lo[gi] = l;
+  if (srf) srf[gi] = make_float4(h.x, h.y, h.z, h.w * h.z);
 }
 
 // Far-partner coefficients (surface_far_coeffs) of the particles of every
@@ -942,7 +944,7 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   if (r) return r;
   k_pack_particles<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->x, c->y, c->z,
                                                 c->q, (float4*)c->pack_hi.p,
-                                                (float4*)c->pack_lo.p);
+                                                (float4*)c->pack_lo.p, nullptr);
   if (any_tip) {
     r = ensure(c, c->img_hi, sizeof(float4) * (size_t)c->cap);
     if (!r) r = ensure(c, c->img_lo, sizeof(float4) * (size_t)c->cap);
@@ -1041,7 +1043,7 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   if (maxn > 0)
     k_pack_particles<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(
         dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (float4*)c->pack_hi.p,
-        (float4*)c->pack_lo.p);
+        (float4*)c->pack_lo.p, nullptr);
   // far-partner series of the sources, for planar entries with image partners
   // beyond n = 0 (used where a wavefront's points lie on the cathode)
   bool any_far = false;
@@ -1072,19 +1074,20 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   return RD_OK;
 }
 
-int launch_pack(rd_ctx* c, const std::vector<Ent>& ents) {
+int launch_pack(rd_ctx* c, const std::vector<Ent>& ents, bool surface) {
   int maxn = 0;
   for (const Ent& e : ents) maxn = std::max(maxn, e.n_ub);
   if (ents.empty() || maxn == 0) return RD_OK;
   int r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)c->cap);
   if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)c->cap);
+  if (!r && surface) r = ensure(c, c->pack_srf, sizeof(float4) * (size_t)c->cap);
   if (r) return r;
   Ent* dt = nullptr;
   r = upload_tab(c, ents, &dt);
   if (r) return r;
   k_pack_particles<<<dim3((maxn + 255) / 256, 1, ents.size()), 256, 0, c->stream>>>(
       dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (float4*)c->pack_hi.p,
-      (float4*)c->pack_lo.p);
+      (float4*)c->pack_lo.p, surface ? (float4*)c->pack_srf.p : nullptr);
   RD_HIP(hipGetLastError());
   return RD_OK;
 }
