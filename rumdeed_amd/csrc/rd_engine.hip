@@ -839,6 +839,9 @@ __global__ __launch_bounds__(kBlock) void k_mh_field64(const Ent* tab, int ne,
 // scalar cache), and their FP64 lane sums are added in wavefront order.  Many
 // short slices keep the per-wavefront dependency chain short: a lone jump is
 // latency-bound (4 wavefronts per workgroup measured slower in pools too).
+// A workgroup computes PPS consecutive segment partials (the set-up above the
+// source loop is paid once for all of them); the partials are the same for
+// every PPS.  At most 64 VGPRs (eight waves per SIMD: two workgroups per CU).
 constexpr int kJumpTargets = 64;
 constexpr int kJumpWaves = 16;
 constexpr int kJumpSlice = kChunk / kJumpWaves;
@@ -2034,7 +2037,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   }
   // A context made for one simulation spreads each jump's sums over up to
   // kWantJump workgroups (latency); in a pool, whose launches serve many
-  // simulations, every mixed workgroup takes at least kPoolJumpChunks chunks
+  // simulations, every mixed partial covers at least kPoolJumpChunks chunks
   // of sources (throughput; the FP64 kernel, bound by its arithmetic, keeps
   // the finer split).  Either way the split depends on the simulation's own
   // size and the kind of context only.
