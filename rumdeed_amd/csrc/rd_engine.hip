@@ -898,6 +898,7 @@ __global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* ta
                                                                 int it, JumpArgs a) {
   __shared__ float4 shi[kChunk], slo[kChunk];
   __shared__ double red[PPS][3][kJumpWaves][kJumpTargets];  // one buffer per partial
+  __shared__ float4 tgt[4][kJumpTargets];  // wavefront 0's packed targets
   const Ent e = tab[find_entry(tab, ne, blockIdx.x)];
   const SimDesc& s = desc[e.slot];
   if (s.g.kind != KIND || it > s.ndim) return;
@@ -913,16 +914,28 @@ __global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* ta
   // the chain of this lane (uniform across a wavefront when by_wave)
   const int k = by_wave ? bx * kJumpWaves + w : bx * kJumpTargets + lane;
   const bool act = k < m;
-  Proposal P{};
-  if (act) {
-    P = jump_proposal(e, s, st, ch, it, k, a);
-    if (by == 0 && (by_wave ? lane == 0 : w == 0)) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
-  }
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
   float4 ih1 = make_float4(0, 0, 0, 1), il1 = make_float4(0, 0, 0, 0);
-  if (act) {
+  if (act && (by_wave || w == 0)) {
+    const Proposal P = jump_proposal(e, s, st, ch, it, k, a);
+    if (by == 0 && (!by_wave || lane == 0)) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
     pack1(P.x, P.y, P.z, 0.0, g.d, h1, l1);
     if (KIND == RD_GEOM_TIP) pack_image(P.x, P.y, P.z, g.z0, g.R, ih1, il1);
+  }
+  if (!by_wave) {
+    if (w == 0) {
+      tgt[0][lane] = h1;
+      tgt[1][lane] = l1;
+      tgt[2][lane] = ih1;
+      tgt[3][lane] = il1;
+    }
+    __syncthreads();
+    if (w != 0) {
+      h1 = tgt[0][lane];
+      l1 = tgt[1][lane];
+      ih1 = tgt[2][lane];
+      il1 = tgt[3][lane];
+    }
   }
   const float4* hi = a.HI + e.base;
   const float4* lo = a.LO + e.base;
