@@ -916,6 +916,9 @@ __global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* ta
   const bool act = k < m;
   float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
   float4 ih1 = make_float4(0, 0, 0, 1), il1 = make_float4(0, 0, 0, 0);
+  // One lane per chain: every wavefront holds the same kJumpTargets chains, so
+  // wavefront 0 forms and packs their proposals and the others read the packed
+  // targets from LDS (by_wave depends on m alone: uniform over the workgroup).
   if (act && (by_wave || w == 0)) {
     const Proposal P = jump_proposal(e, s, st, ch, it, k, a);
     if (by == 0 && (!by_wave || lane == 0)) mh_record(e, s, ch, k, P, a.ptx, a.pty, a.ptz);
