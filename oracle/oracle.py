@@ -119,6 +119,21 @@ def accel(g: Geom, pos, q, m, nthreads=None, long_double=False):
     return out
 
 
+def accel_subset(g: Geom, pos, q, m, targets, nthreads=None, long_double=False):
+    """Accelerations (len(targets),3) of the listed particles (0-based) over
+    all sources; row k equals row targets[k] of accel()."""
+    pos = _f64(pos)
+    n = pos.shape[0]
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    assert t.size == 0 or (t.min() >= 0 and t.max() < n)
+    out = np.zeros((t.size, 3))
+    nt = nthreads or default_threads()
+    fn = lib().rdo_accel_subset_ld if long_double else lib().rdo_accel_subset
+    fn(ctypes.byref(g), c_int(n), _d(pos), _d(_f64(q)), _d(_f64(m)), c_int(t.size), _i(t),
+       _d(out), c_int(nt))
+    return out
+
+
 def field(g: Geom, pos, q, pts, nthreads=None, long_double=False):
     """Field (M,3) at points pts (M,3) from particles pos (n,3) with charge q."""
     pos = _f64(pos).reshape(-1, 3)
@@ -129,6 +144,29 @@ def field(g: Geom, pos, q, pts, nthreads=None, long_double=False):
     fn = lib().rdo_field_ld if long_double else lib().rdo_field
     fn(ctypes.byref(g), c_int(pos.shape[0]), _d(pos), _d(q), c_int(pts.shape[0]), _d(pts),
        _d(out), c_int(nt))
+    return out
+
+
+def field_scale(g: Geom, pos, q, pts, nthreads=None):
+    """Per point: sum over sources of the norm of each source's field term
+    plus the norm of the vacuum field."""
+    pos = _f64(pos).reshape(-1, 3)
+    pts = _f64(pts).reshape(-1, 3)
+    out = np.zeros(pts.shape[0])
+    lib().rdo_field_scale(ctypes.byref(g), c_int(pos.shape[0]), _d(pos), _d(_f64(q)),
+                          c_int(pts.shape[0]), _d(pts), _d(out), c_int(nthreads or default_threads()))
+    return out
+
+
+def accel_scale(g: Geom, pos, q, m, targets, nthreads=None):
+    """Per listed target: sum over sources of the norm of each pair's
+    acceleration term plus the norm of the vacuum term."""
+    pos = _f64(pos)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    out = np.zeros(t.size)
+    lib().rdo_accel_scale(ctypes.byref(g), c_int(pos.shape[0]), _d(pos), _d(_f64(q)),
+                          _d(_f64(m)), c_int(t.size), _i(t), _d(out),
+                          c_int(nthreads or default_threads()))
     return out
 
 
@@ -272,6 +310,40 @@ class Ref:
 
     def read_work(self, x0, y0, lx, ly):
         self.L.ref_read_work(c_double(x0), c_double(y0), c_double(lx), c_double(ly))
+
+    def tip_xyz(self, xi, eta, phi):
+        o = np.zeros(3)
+        self.L.ref_tip_xyz(c_double(xi), c_double(eta), c_double(phi), _d(o))
+        return o
+
+    def tip_coords(self, p):
+        o = np.zeros(3)
+        self.L.ref_tip_coords(_d(_f64(p)), _d(o))
+        return o
+
+    def tip_normal(self, p):
+        o = np.zeros(3)
+        self.L.ref_tip_normal(_d(_f64(p)), _d(o))
+        return o
+
+    def field_normal(self, p, E):
+        self.L.ref_field_normal.restype = c_double
+        return self.L.ref_field_normal(_d(_f64(p)), _d(_f64(E)))
+
+    def tip_area(self, xi1, xi2, phi1, phi2):
+        self.L.ref_tip_area.restype = c_double
+        self.L.ref_tip_area.argtypes = [c_double] * 4
+        return self.L.ref_tip_area(xi1, xi2, phi1, phi2)
+
+    def tip_vacuum(self, p):
+        o = np.zeros(3)
+        self.L.ref_tip_vacuum(_d(_f64(p)), _d(o))
+        return o
+
+    def mb_velocity(self, T, seed, n):
+        out = np.zeros((n, 3))
+        self.L.ref_mb_velocity(c_double(T), c_int(seed), c_int(n), _d(out))
+        return out
 
     def work_at(self, pts):
         pts = _f64(pts).reshape(-1, 3)
@@ -480,6 +552,52 @@ def work_checker(table, x0, y0, lx, ly, pts):
                                   ctypes.byref(s))
         sec[k] = s.value
     return w, sec
+
+
+def tip_surface(g: Geom):
+    """The oracle's tip surface functions for geometry g: (xyz(xi, eta, phi),
+    normal(p), area(xi1, xi2, phi1, phi2), eta(p), field_normal(p, E))."""
+    L = lib()
+    L.rdo_tip_xyz.argtypes = [ctypes.c_void_p, c_double, c_double, c_double, _dp]
+    L.rdo_tip_normal.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.rdo_tip_area.argtypes = [ctypes.c_void_p, c_double, c_double, c_double, c_double]
+    L.rdo_tip_area.restype = c_double
+    L.rdo_tip_eta.argtypes = [ctypes.c_void_p, _dp]
+    L.rdo_tip_eta.restype = c_double
+    L.rdo_field_normal.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.rdo_field_normal.restype = c_double
+    gp = ctypes.byref(g)
+
+    def xyz(xi, eta, phi):
+        o = np.zeros(3)
+        L.rdo_tip_xyz(gp, xi, eta, phi, _d(o))
+        return o
+
+    def normal(p):
+        o = np.zeros(3)
+        L.rdo_tip_normal(gp, _d(_f64(p)), _d(o))
+        return o
+
+    def area(xi1, xi2, phi1, phi2):
+        return L.rdo_tip_area(gp, xi1, xi2, phi1, phi2)
+
+    def eta(p):
+        return L.rdo_tip_eta(gp, _d(_f64(p)))
+
+    def field_normal(p, E):
+        return L.rdo_field_normal(gp, _d(_f64(p)), _d(_f64(E)))
+
+    return xyz, normal, area, eta, field_normal
+
+
+def mb_velocity(seed, stream, step, T, n):
+    """Emission velocities (n,3) of candidates 0..n-1 of a thermal-field step."""
+    L = lib()
+    L.rdo_mb_velocity.argtypes = [c_uint64, c_uint64, c_int, c_double, c_int, _dp]
+    L.rdo_mb_velocity.restype = None
+    out = np.zeros((n, 3))
+    L.rdo_mb_velocity(int(seed), int(stream), int(step), float(T), int(n), _d(out))
+    return out
 
 
 def sincos(x):

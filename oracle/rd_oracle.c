@@ -298,15 +298,13 @@ static inline void pair_term(const rdo_geom* g, int i, int j, double x1, double 
   *fz = az;
 }
 
-void rdo_accel(const rdo_geom* g, int n, const double* pos, const double* q, const double* m,
-               double* acc, int nthreads) {
+/* Acceleration of target i over all n sources in the canonical order. */
+static void accel_target(const rdo_geom* g, int n, const double* pos, const double* q,
+                         const double* m, int i, double* out) {
   const double es = eps_soft(), dfc = div_fac_c(), kt = k_tip();
   const double z0 = g->h_tip - g->r_tip, R = g->r_tip;
-#ifdef _OPENMP
-  if (nthreads < 1) nthreads = 1;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
-#endif
-  for (int i = 0; i < n; ++i) {
+  {
+    double* acc = out - 3 * i;
     double x1 = pos[3 * i], y1 = pos[3 * i + 1], z1 = pos[3 * i + 2];
     double q1 = q[i];
     double qd1 = q1 * dfc;
@@ -339,6 +337,24 @@ void rdo_accel(const rdo_geom* g, int n, const double* pos, const double* q, con
       acc[3 * i + 2] = (sz + q1 * fz) * im;
     }
   }
+}
+
+void rdo_accel(const rdo_geom* g, int n, const double* pos, const double* q, const double* m,
+               double* acc, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+  for (int i = 0; i < n; ++i) accel_target(g, n, pos, q, m, i, &acc[3 * i]);
+}
+
+void rdo_accel_subset(const rdo_geom* g, int n, const double* pos, const double* q,
+                      const double* m, int nt, const int* targets, double* acc, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+  for (int k = 0; k < nt; ++k) accel_target(g, n, pos, q, m, targets[k], &acc[3 * k]);
 }
 
 /* Field term of source j at point p (Calc_Field_at, src/mod_verlet.F90:1497-1524,
@@ -420,6 +436,61 @@ void rdo_field(const rdo_geom* g, int n, const double* pos, const double* q, int
   }
 }
 
+/* Conditioning of the sums above: for every point, the sum over the sources
+ * of the norm of each source's term plus the norm of the vacuum field (the
+ * scale against which a sum of rounded terms is accurate), and the same for
+ * listed targets of the accelerations (times q_i / m_i like the sum itself). */
+void rdo_field_scale(const rdo_geom* g, int n, const double* pos, const double* q, int mpts,
+                     const double* pts, double* out, int nthreads) {
+  const double es = eps_soft(), dfc = div_fac_c(), kt = k_tip();
+  const double z0 = g->h_tip - g->r_tip, R = g->r_tip;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+  for (int p = 0; p < mpts; ++p) {
+    double x1 = pts[3 * p], y1 = pts[3 * p + 1], z1 = pts[3 * p + 2];
+    double da = 0, xim = 0, yim = 0, zim = 0;
+    if (g->kind == RDO_GEOM_TIP && g->image_charge)
+      tip_image_point(x1, y1, z1, z0, R, &da, &xim, &yim, &zim);
+    double sc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double fx, fy, fz;
+      field_term(g, x1, y1, z1, j, pos, q, dfc, es, kt, R, da, xim, yim, zim, &fx, &fy, &fz);
+      sc += sqrt(fx * fx + fy * fy + fz * fz);
+    }
+    double e[3];
+    rdo_vacuum_field(g, &pts[3 * p], e);
+    out[p] = sc + sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  }
+}
+
+void rdo_accel_scale(const rdo_geom* g, int n, const double* pos, const double* q,
+                     const double* m, int nt, const int* targets, double* out, int nthreads) {
+  const double es = eps_soft(), dfc = div_fac_c(), kt = k_tip();
+  const double z0 = g->h_tip - g->r_tip, R = g->r_tip;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+  for (int k = 0; k < nt; ++k) {
+    int i = targets[k];
+    double x1 = pos[3 * i], y1 = pos[3 * i + 1], z1 = pos[3 * i + 2];
+    double qd1 = q[i] * dfc;
+    double sc = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      double fx, fy, fz;
+      pair_term(g, i, j, x1, y1, z1, qd1, pos, q, es, kt, z0, R, &fx, &fy, &fz);
+      sc += sqrt(fx * fx + fy * fy + fz * fz);
+    }
+    double e[3];
+    rdo_vacuum_field(g, &pos[3 * i], e);
+    sc += fabs(q[i]) * sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    out[k] = sc / m[i];
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Long-double brute force: same reference semantics, natural order,        */
 /* libm-style transcendental functions (atan2l/cosl/sinl/powl) as written.  */
@@ -489,18 +560,15 @@ static void tip_ic_ld(LD xa, LD ya, LD za, LD xb, LD yb, LD zb, LD z0, LD R, LD 
   *icz = kt * ((za - zb) / ta - (R * (zi - zb)) / (da * tb));
 }
 
-void rdo_accel_ld(const rdo_geom* g, int n, const double* pos, const double* q, const double* m,
-                  double* acc, int nthreads) {
+static void accel_target_ld(const rdo_geom* g, int n, const double* pos, const double* q,
+                            const double* m, int i, double* out) {
   const LD es = (LD)eps_soft();
   const LD dfc = 1.0L / (4.0L * 3.141592653589793238462643383279502884L *
                          (1.0L / ((LD)MU_0 * (LD)C_LIGHT * (LD)C_LIGHT)));
   const LD kt = (LD)Q_0 * dfc;
   const LD z0 = (LD)g->h_tip - (LD)g->r_tip, R = g->r_tip;
-#ifdef _OPENMP
-  if (nthreads < 1) nthreads = 1;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
-#endif
-  for (int i = 0; i < n; ++i) {
+  {
+    double* acc = out - 3 * i;
     LD x1 = pos[3 * i], y1 = pos[3 * i + 1], z1 = pos[3 * i + 2];
     LD q1 = q[i];
     LD ax = 0, ay = 0, az = 0;
@@ -543,6 +611,25 @@ void rdo_accel_ld(const rdo_geom* g, int n, const double* pos, const double* q, 
     acc[3 * i + 1] = (double)((ay + q1 * ey) * im);
     acc[3 * i + 2] = (double)((az + q1 * ez) * im);
   }
+}
+
+void rdo_accel_ld(const rdo_geom* g, int n, const double* pos, const double* q, const double* m,
+                  double* acc, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+  for (int i = 0; i < n; ++i) accel_target_ld(g, n, pos, q, m, i, &acc[3 * i]);
+}
+
+void rdo_accel_subset_ld(const rdo_geom* g, int n, const double* pos, const double* q,
+                         const double* m, int nt, const int* targets, double* acc,
+                         int nthreads) {
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+  for (int k = 0; k < nt; ++k) accel_target_ld(g, n, pos, q, m, targets[k], &acc[3 * k]);
 }
 
 void rdo_field_ld(const rdo_geom* g, int n, const double* pos, const double* q, int mpts,

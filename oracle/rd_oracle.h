@@ -57,6 +57,14 @@ void rdo_accel(const rdo_geom* g, int n, const double* pos, const double* q,
 /* Same physics, long double, natural j order (brute-force error reference). */
 void rdo_accel_ld(const rdo_geom* g, int n, const double* pos, const double* q,
                   const double* m, double* acc, int nthreads);
+/* The accelerations of the nt listed targets (0-based indices) over all n
+ * sources: rows k of acc (3,nt) equal rows targets[k] of rdo_accel /
+ * rdo_accel_ld.  For sampled checks of large snapshots. */
+void rdo_accel_subset(const rdo_geom* g, int n, const double* pos, const double* q,
+                      const double* m, int nt, const int* targets, double* acc, int nthreads);
+void rdo_accel_subset_ld(const rdo_geom* g, int n, const double* pos, const double* q,
+                         const double* m, int nt, const int* targets, double* acc,
+                         int nthreads);
 
 /* Field at M points (src/mod_verlet.F90:1466-1529 / :1635-1911), canonical
  * FP64 order, and its long-double brute-force counterpart. */
@@ -64,6 +72,14 @@ void rdo_field(const rdo_geom* g, int n, const double* pos, const double* q,
                int mpts, const double* pts, double* out, int nthreads);
 void rdo_field_ld(const rdo_geom* g, int n, const double* pos, const double* q,
                   int mpts, const double* pts, double* out, int nthreads);
+
+/* Scale of those sums: per point (target) the sum of the norms of the
+ * sources' terms plus the norm of the vacuum term.  A sum of terms rounded
+ * to relative u is accurate to u times this scale. */
+void rdo_field_scale(const rdo_geom* g, int n, const double* pos, const double* q, int mpts,
+                     const double* pts, double* out, int nthreads);
+void rdo_accel_scale(const rdo_geom* g, int n, const double* pos, const double* q,
+                     const double* m, int nt, const int* targets, double* out, int nthreads);
 
 /* Vacuum field at one point (field_E_planar / field_E_Hyperboloid). */
 void rdo_vacuum_field(const rdo_geom* g, const double* p, double* out);

@@ -306,6 +306,42 @@ static double field_normal(const rdo_geom* g, double x, double y, const double* 
   return nx * E[0] + ny * E[1] + nz * E[2];
 }
 
+/* Public entry points of the tip surface functions, for the tests that pin
+ * them against the reference's own (xyz_corr, surface_normal, Tip_Area,
+ * eta_coor, Field_normal). */
+void rdo_tip_xyz(const rdo_geom* g, double xi, double eta, double phi, double* out) {
+  tip_xyz(g, xi, eta, phi, &out[0], &out[1], &out[2]);
+}
+void rdo_tip_normal(const rdo_geom* g, const double* p, double* out) {
+  tip_normal(g, p[0], p[1], &out[0], &out[1], &out[2]);
+}
+double rdo_tip_area(const rdo_geom* g, double xi1, double xi2, double phi1, double phi2) {
+  return tip_area(g, xi1, xi2, phi1, phi2);
+}
+double rdo_tip_eta(const rdo_geom* g, const double* p) { return tip_eta(g, p[0], p[1], p[2]); }
+double rdo_field_normal(const rdo_geom* g, const double* p, const double* E) {
+  return field_normal(g, p[0], p[1], E);
+}
+
+/* Get_MB_Velocity (src/mod_velocity.f90:53-68): two Box-Muller pairs of
+ * standard deviation sqrt(k_b T / m_0); x from the first pair, y and z from
+ * the second (the reference overwrites element 2), z made positive.  The
+ * draws of candidate k of a step. */
+static void mb_velocity(uint64_t seed, uint64_t stream, int step, int k, double T, double* v) {
+  rdo_consts cc;
+  rdo_get_consts(&cc);
+  double sd = sqrt(cc.k_b * T / cc.m_0);
+  double b1, b2, b3, b4;
+  box_muller(seed, stream, (uint64_t)step, tagv(T_VEL, 0), (uint64_t)k, 0, 0, sd, sd, &b1, &b2);
+  box_muller(seed, stream, (uint64_t)step, tagv(T_VEL, 1), (uint64_t)k, 0, 0, sd, sd, &b3, &b4);
+  v[0] = b1;
+  v[1] = b3;
+  v[2] = fabs(b4);
+}
+void rdo_mb_velocity(uint64_t seed, uint64_t stream, int step, double T, int n, double* out) {
+  for (int k = 0; k < n; ++k) mb_velocity(seed, stream, step, k, T, &out[3 * k]);
+}
+
 /* ==================================================================== */
 typedef struct {
   rdo_geom g;
@@ -836,11 +872,7 @@ static void do_emission(rdo_sim* s, int step) {
       if (ok[k] == 1) {
         flag = 1;
         p[0] = cx[k]; p[1] = cy[k]; p[2] = 1.0 * LS;
-        double sd = sqrt(cc.k_b * s->T / cc.m_0);
-        double b1, b2, b3, b4;
-        box_muller(s->seed, s->stream, (uint64_t)step, tagv(T_VEL, 0), (uint64_t)k, 0, 0, sd, sd, &b1, &b2);
-        box_muller(s->seed, s->stream, (uint64_t)step, tagv(T_VEL, 1), (uint64_t)k, 0, 0, sd, sd, &b3, &b4);
-        v[0] = b1; v[1] = b3; v[2] = fabs(b4);
+        mb_velocity(s->seed, s->stream, step, k, s->T, v);
         work_at(s, p[0], p[1], &sec);
       }
     } else {

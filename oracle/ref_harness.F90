@@ -10,6 +10,9 @@
 !   Add_Particle / Mark_Particles_Remove / Remove_Particles    (src/mod_pair.F90:29-562)
 !   Get_Kevin_Jgtf_v2                                          (src/mod_kevin_rjgtf_v2.f90:56-74)
 !   w_theta_xy after Read_work_function                        (src/mod_work_function.F90:53-194)
+!   xyz_corr / xi,eta,phi_coor / surface_normal / Field_normal /
+!   Tip_Area / field_E_Hyperboloid                             (src/mod_hyperboloid_tip.f90:25-163)
+!   Get_MB_Velocity through Init_Emission_Velocity             (src/mod_velocity.f90:20-68)
 ! Only the glue lives here: array allocation (as main.F90:422-456 does it),
 ! scratch output units, geometry set-up, and the Ramo unit vector of the
 ! planar and tip geometries (E_zunit_planar / E_zunit_tip restated, because
@@ -22,6 +25,7 @@ module rd_ref_harness
   use mod_hyperboloid_tip
   use mod_kevin_rjgtf_v2
   use mod_work_function
+  use mod_velocity
   !$ use omp_lib
   implicit none
 
@@ -361,4 +365,65 @@ contains
       sec(k) = s
     end do
   end subroutine ref_work_at
+
+  ! ---- hyperboloid tip surface (after ref_setup_tip) ----------------------
+  subroutine ref_tip_xyz(xi, eta, phi, out) bind(C, name='ref_tip_xyz')
+    real(c_double), value :: xi, eta, phi
+    real(c_double), intent(out) :: out(3)
+    out = xyz_corr(xi, eta, phi)
+  end subroutine ref_tip_xyz
+
+  subroutine ref_tip_coords(pos, out) bind(C, name='ref_tip_coords')
+    real(c_double), intent(in) :: pos(3)
+    real(c_double), intent(out) :: out(3)
+    out(1) = xi_coor(pos(1), pos(2), pos(3))
+    out(2) = eta_coor(pos(1), pos(2), pos(3))
+    out(3) = phi_coor(pos(1), pos(2), pos(3))
+  end subroutine ref_tip_coords
+
+  subroutine ref_tip_normal(pos, out) bind(C, name='ref_tip_normal')
+    real(c_double), intent(in) :: pos(3)
+    real(c_double), intent(out) :: out(3)
+    out = surface_normal(pos)
+  end subroutine ref_tip_normal
+
+  function ref_field_normal(pos, e) bind(C, name='ref_field_normal') result(f)
+    real(c_double), intent(in) :: pos(3), e(3)
+    real(c_double) :: f
+    f = Field_normal(pos, e)
+  end function ref_field_normal
+
+  function ref_tip_area(xi1, xi2, phi1, phi2) bind(C, name='ref_tip_area') result(a)
+    real(c_double), value :: xi1, xi2, phi1, phi2
+    real(c_double) :: a
+    a = Tip_Area(xi1, xi2, phi1, phi2)
+  end function ref_tip_area
+
+  subroutine ref_tip_vacuum(pos, out) bind(C, name='ref_tip_vacuum')
+    real(c_double), intent(in) :: pos(3)
+    real(c_double), intent(out) :: out(3)
+    out = field_E_Hyperboloid(pos)
+  end subroutine ref_tip_vacuum
+
+  ! ---- Maxwell-Boltzmann emission velocity -------------------------------
+  ! n draws of ptr_Get_Emission_Velocity after Init_Emission_Velocity(VELOCITY_MB)
+  ! at temperature T, with the intrinsic generator seeded from `seed`.
+  subroutine ref_mb_velocity(T, seed, n, out) bind(C, name='ref_mb_velocity')
+    real(c_double), value :: T
+    integer(c_int), value :: seed, n
+    real(c_double), intent(out) :: out(3, n)
+    integer :: k, i
+    integer, allocatable :: sd(:)
+    T_temp = T
+    call Init_Emission_Velocity(VELOCITY_MB)
+    call random_seed(size=k)
+    allocate(sd(k))
+    do i = 1, k
+      sd(i) = seed + 7919 * i
+    end do
+    call random_seed(put=sd)
+    do i = 1, n
+      out(:, i) = ptr_Get_Emission_Velocity()
+    end do
+  end subroutine ref_mb_velocity
 end module rd_ref_harness

@@ -58,11 +58,19 @@ class Result:
     n_final: int = 0
     ramo_second_half: float = 0.0
     capacity: int = 0
+    error: str = ""       # set when the request was refused (never admitted)
     records: list = field(default_factory=list)
 
     @property
     def latency(self):
         return self.finished - self.arrival
+
+    @property
+    def valid(self) -> bool:
+        """Served to the end with every emitted electron kept: a refused
+        request, or one whose slot overflowed (electrons dropped as
+        Add_Particle drops them beyond MAX_PARTICLES), is not a valid result."""
+        return not self.error and self.dropped == 0
 
 
 def capacity_estimate(req: Request) -> int:
@@ -72,8 +80,7 @@ def capacity_estimate(req: Request) -> int:
         return req.capacity
     F = abs(req.V) / req.gap
     n = 3.0 * EPS0 * F * req.side * req.side / Q0
-    cap = int(math.ceil(max(n, 256.0) / 256.0) * 256)
-    return min(cap, 1 << 16)
+    return int(math.ceil(max(n, 256.0) / 256.0) * 256)
 
 
 def stream_requests(n: int = 256, rate: float = 4.0, seed: int = 2024, steps: int = 2000,
@@ -108,6 +115,7 @@ class Server:
                  graphs: bool = False):
         self.pool = Pool(total_particles, max_slots, device)
         self.pool.set_graphs(graphs)  # jump loop of each step as one HIP graph
+        self.total_particles = int(total_particles)
         self.budget = int(hbm_budget)
         self.precision = precision
         self.max_slots = max_slots
@@ -129,7 +137,13 @@ class Server:
         """Serve `requests`; arrival times are honoured against the wall clock
         when `realtime` (otherwise every request is available at once).
         progress(result, n_done) runs as each request completes, on_step(
-        pool_steps, n_active) after every pool step."""
+        pool_steps, n_active) after every pool step.
+
+        A request that can never be admitted -- its particle capacity is
+        larger than the pool's store, its footprint larger than the HBM
+        budget, or it does not fit even when no other simulation is open --
+        is refused at once: it comes back as a Result with `error` set and
+        is not waited for, so it never blocks the requests behind it."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         active = {}   # slot -> (request, result, footprint, steps done)
         results = []
@@ -139,16 +153,33 @@ class Server:
         occupancy = []
         mh_pairs = 0
         force_pairs = 0
+        refused = []
         while pending or active:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 r = pending[0]
                 cap = capacity_estimate(r)
                 fp = Pool.footprint(cap)
+                why = ""
+                if cap > self.total_particles:
+                    why = f"capacity {cap} exceeds the pool's {self.total_particles} particles"
+                elif fp > self.budget:
+                    why = f"footprint {fp} B exceeds the HBM budget of {self.budget} B"
+                if why:
+                    pending.popleft()
+                    refused.append(self._refused(r, cap, now, realtime, why))
+                    continue
                 if used + fp > self.budget or len(active) >= self.max_slots:
                     break
                 slot = self.pool.try_open(cap)
                 if slot is None:
+                    if not active:
+                        # the store is empty and still has no contiguous range
+                        pending.popleft()
+                        refused.append(self._refused(
+                            r, cap, now, realtime,
+                            f"capacity {cap} does not fit the empty pool store"))
+                        continue
                     break
                 self._configure(slot, r)
                 pending.popleft()
@@ -201,6 +232,8 @@ class Server:
         lat = np.array([r.latency for r in results]) if results else np.zeros(1)
         stats = {
             "requests": len(results),
+            "refused": len(refused),
+            "invalid": sum(1 for r in results if not r.valid),
             "makespan_s": makespan,
             "sims_per_hour": len(results) / makespan * 3600.0 if makespan > 0 else 0.0,
             "latency_p50_s": float(np.percentile(lat, 50)),
@@ -213,4 +246,10 @@ class Server:
             "force_pair_evals": force_pairs,
             "pair_evals_per_s": (mh_pairs + force_pairs) / makespan if makespan > 0 else 0.0,
         }
-        return sorted(results, key=lambda r: r.rid), stats
+        return sorted(results + refused, key=lambda r: r.rid), stats
+
+    @staticmethod
+    def _refused(r: Request, cap: int, now: float, realtime: bool, why: str) -> Result:
+        arrival = r.arrival if realtime else 0.0
+        return Result(rid=r.rid, arrival=arrival, admitted=now, finished=now, steps=0,
+                      capacity=cap, error=why)
