@@ -426,6 +426,14 @@ class Sim:
     def set_seed(self, seed, stream=0):
         self.L.rdo_sim_set_seed(ctypes.c_void_p(self.h), int(seed), int(stream))
 
+    def set_serial(self, on=True):
+        """Sample emission sites with the reference's default serial
+        samplers (each chain sees the electrons of the chains before it)
+        instead of the lockstep (batch) semantics the device runs."""
+        self.L.rdo_sim_set_serial.restype = None
+        self.L.rdo_sim_set_serial.argtypes = [ctypes.c_void_p, c_int]
+        self.L.rdo_sim_set_serial(ctypes.c_void_p(self.h), int(bool(on)))
+
     def set_emitter(self, mode, delay=0, x0=0.0, y0=0.0, lx=0.0, ly=0.0, T=300.0, w_tip=4.7,
                     wf=None, quad=0, ndim=0):
         wf = np.array([[4.7]]) if wf is None else np.atleast_2d(np.asarray(wf, dtype=np.float64))

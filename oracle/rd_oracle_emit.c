@@ -360,6 +360,7 @@ typedef struct {
   int start_rounds, n_elec;
   int life[2][NLIFE];
   int nthreads;
+  int serial; /* 1: the reference's serial samplers (emit_serial) */
   /* particle records of the last step (the density_emit / density_absorb
    * writes of Add_Particle and Mark_Particles_Remove) */
   struct rdo_emit_rec* elog;
@@ -562,6 +563,10 @@ static void field_at(const rdo_sim* s, int m, const double* pts, double* out) {
   rdo_field(&s->g, s->n, s->pos, s->q, m, pts, out, s->nthreads);
 }
 
+static void add_particle(rdo_sim* s, const double* p, const double* v, int spc, int step,
+                         int emit, int sec);
+static void emit_serial(rdo_sim* s, int step, int nr);
+
 static void do_emission(rdo_sim* s, int step) {
   if (s->mode == EM_NONE || !(s->delay < step)) return;
   const int tip = s->mode == EM_TIP;
@@ -657,6 +662,10 @@ static void do_emission(rdo_sim* s, int step) {
   free(fld);
   s->n_round = nr;
   if (nr <= 0) return;
+  if (s->serial) {
+    emit_serial(s, step, nr);
+    return;
+  }
 
   /* chains */
   const int C = nr;
@@ -914,6 +923,175 @@ static void do_emission(rdo_sim* s, int step) {
   free(cx); free(cy); free(px); free(py); free(F); free(sup); free(D); free(o);
   free(ok); free(acc); free(rej); free(pv); free(cp); free(cf);
 }
+
+
+/* ---- the reference's default serial samplers --------------------------------
+ * For statistical comparison with the lockstep sampling of do_emission (the
+ * reference's mh_batch semantics, which the device runs).  Each of the step's
+ * nr chains runs to completion in turn against the particles present at that
+ * moment -- so it sees the electrons emitted by the chains before it -- and
+ * the candidate is tested and added before the next chain starts:
+ *   FE:  Metropolis_Hastings_rectangle_J (src/mod_field_emission_v2.F90:1122-1265)
+ *        and the emission loop of Do_Field_Emission_Planar_rectangle (:307-385);
+ *   GTF: Metropolis_Hastings_rectangle_J and Do_Field_Thermo_Emission_Planar_simple
+ *        (src/mod_field_thermo_emission.F90:136-364);
+ *   tip: Metro_algo_tip_v3 and Do_Field_Emission_Tip_OLDCODE
+ *        (src/mod_emission_tip.f90:1241-1390, :488-530).
+ * The width MH_std is fixed within a chain and updated once per chain from the
+ * chain's acceptance rate.  Draws use the same counters as the lockstep form
+ * (chain k, jump it), so the two differ only by their semantics. */
+static double field_one(const rdo_sim* s, const double* p, double* e) {
+  rdo_field(&s->g, s->n, s->pos, s->q, 1, p, e, 1);
+  return s->mode == EM_TIP ? field_normal(&s->g, p[0], p[1], e) : e[2];
+}
+
+static void emit_serial(rdo_sim* s, int step, int nr) {
+  const int tip = s->mode == EM_TIP, gtf = s->mode == EM_GTF;
+  const int ic = s->g.image_charge;
+  const int first = gtf ? 0 : fnint(s->ndim * 0.25);
+  const double lo = tip ? 0.0005 : (gtf ? 0.005 : 0.00005);
+  rdo_consts cc;
+  rdo_get_consts(&cc);
+  int emitted = 0;
+  double dsum = 0.0;
+  for (int k = 0; k < nr; ++k) {
+    if (tip) { /* Metro_algo_tip_v3 clamps the shared width before use */
+      if (s->MH_std > 0.125) s->MH_std = 0.125;
+      else if (s->MH_std < 0.0005) s->MH_std = 0.0005;
+    }
+    double cx = 0.0, cy = 0.0, p[3], e[3], F = 1.0;
+    int ok = 0;
+    for (int round = 1; round <= 10001 && !ok; ++round) {
+      uint64_t r[4];
+      ph((uint64_t)step, tagv(T_START, (uint64_t)round), (uint64_t)k, 0, s->seed, s->stream, r);
+      double u1 = u01(r[0]), u2 = u01(r[1]);
+      if (tip) {
+        cx = 1.0 + (s->g.max_xi - 1.0) * u1;
+        cy = 2.0 * PI_D * u2;
+        tip_xyz(&s->g, cx, s->g.eta_1, cy, &p[0], &p[1], &p[2]);
+      } else {
+        cx = u1 * s->elx + s->ex0;
+        cy = u2 * s->ely + s->ey0;
+        p[0] = cx; p[1] = cy; p[2] = 0.0;
+      }
+      F = field_one(s, p, e);
+      ok = F < 0.0;
+    }
+    if (!ok) {  /* a failed chain emits nothing */
+      if (s->mode == EM_FE) dsum += 0.0;
+      continue;
+    }
+    double sup = chain_target(s, F, cx, cy);
+    const double MH = s->MH_std;
+    int acc = 0, rej = 0;
+    for (int it = 1; it <= s->ndim; ++it) {
+      const int counted = gtf || it > first;
+      const uint64_t tg = tagv(T_JUMP, (uint64_t)it);
+      double nx, ny, q[3];
+      if (tip) {
+        double f = it > first ? MH : 0.10;
+        double d1, d2;
+        box_muller(s->seed, s->stream, (uint64_t)step, tg, (uint64_t)k, 0.0, 0.0,
+                   (s->g.max_xi - 1.0) * f, 2.0 * PI_D * f, &d1, &d2);
+        nx = cx + d1;
+        ny = fmodulo(cy + d2, 2.0 * PI_D);
+        if (nx > s->g.max_xi) nx = 2.0 * s->g.max_xi - nx;
+        if (nx < 1.0) nx = 2.0 - nx;
+        if ((nx < 1.0) || (nx > s->g.max_xi)) {
+          if (counted) rej++;
+          continue;
+        }
+        tip_xyz(&s->g, nx, s->g.eta_1, ny, &q[0], &q[1], &q[2]);
+      } else {
+        double f = gtf ? MH : (it > first ? MH : 0.10);
+        box_muller(s->seed, s->stream, (uint64_t)step, tg, (uint64_t)k, cx, cy, s->elx * f,
+                   s->ely * f, &nx, &ny);
+        if (gtf) {
+          double sxs = (nx - s->ex0) / s->elx, sys = (ny - s->ey0) / s->ely;
+          if ((sxs > 1.0) || (sxs < 0.0)) sxs = 1.0 - (sxs - floor(sxs));
+          if ((sys > 1.0) || (sys < 0.0)) sys = 1.0 - (sys - floor(sys));
+          nx = sxs * s->elx + s->ex0;
+          ny = sys * s->ely + s->ey0;
+        } else {
+          double x_max = s->ex0 + s->elx, x_min = s->ex0, y_max = s->ey0 + s->ely, y_min = s->ey0;
+          if (nx > x_max) nx = x_max - (nx - x_max);
+          else if (nx < x_min) nx = (x_min - nx) + x_min;
+          if (ny > y_max) ny = y_max - (ny - y_max);
+          else if (ny < y_min) ny = (y_min - ny) + y_min;
+        }
+        q[0] = nx; q[1] = ny; q[2] = 0.0;
+      }
+      double Fn = field_one(s, q, e);
+      if (gtf ? (Fn > 0.0) : (Fn >= 0.0)) {
+        if (counted) rej++;
+        continue;
+      }
+      double sn = chain_target(s, Fn, nx, ny);
+      int accd = sn >= sup;
+      if (!accd) {
+        double rnd = uniform(s->seed, s->stream, (uint64_t)step, tagv(T_ACCEPT, (uint64_t)it),
+                             (uint64_t)k, 0);
+        accd = rdo_log(rnd) <= sn - sup;
+      }
+      if (accd) {
+        cx = nx; cy = ny; sup = sn; F = Fn;
+        p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+        if (counted) acc++;
+      } else if (counted) {
+        rej++;
+      }
+    }
+    if (acc + rej > 0) {
+      s->a_rate = (double)acc / (double)(acc + rej);
+      s->MH_std = s->MH_std * rdo_exp(0.025 * (s->a_rate - 0.35));
+      if (s->MH_std > 0.1250) s->MH_std = 0.1250;
+      else if (s->MH_std < lo) s->MH_std = lo;
+    }
+    double pe[3], v[3] = {0.0, 0.0, 0.0};
+    int sec = 1, flag = 0;
+    if (s->mode == EM_FE) {
+      double Df = rdo_escape_log(ic, F, work_at(s, cx, cy, &sec));
+      dsum += rdo_exp(Df);
+      double rnd = uniform(s->seed, s->stream, (uint64_t)step, tagv(T_EMIT, 0), (uint64_t)k, 0);
+      if (rdo_log(rnd) <= Df) {
+        flag = 1;
+        pe[0] = cx; pe[1] = cy; pe[2] = 1.0 * LS;
+        work_at(s, cx, cy, &sec);
+      }
+    } else if (gtf) {
+      flag = 1;
+      pe[0] = cx; pe[1] = cy; pe[2] = 1.0 * LS;
+      mb_velocity(s->seed, s->stream, step, k, s->T, v);
+      work_at(s, cx, cy, &sec);
+    } else {
+      double sw = sqrt(s->w_tip);
+      double D = rdo_exp(b_FN_() * (sw * sw * sw) * fn_v(ic, l_const_(), F, s->w_tip) / fabs(F));
+      double rnd = uniform(s->seed, s->stream, (uint64_t)step, tagv(T_EMIT, 0), (uint64_t)k, 0);
+      if (rnd <= D) {
+        flag = 1;
+        double nx, ny, nz;
+        tip_normal(&s->g, p[0], p[1], &nx, &ny, &nz);
+        pe[0] = p[0] + nx * LS;
+        pe[1] = p[1] + ny * LS;
+        pe[2] = p[2] + nz * LS;
+      }
+    }
+    if (flag) {
+      int before = s->n;
+      add_particle(s, pe, v, 1, step, 1, sec);
+      if (s->n > before) {
+        emitted++;
+        rdo_emit_rec* r = &s->elog[s->n_elog++];
+        r->x = pe[0]; r->y = pe[1]; r->z = pe[2];
+        r->emitter = 1; r->section = sec; r->id = s->id[before]; r->species = 1;
+      }
+    }
+  }
+  s->emitted = emitted;
+  if (s->mode == EM_FE) s->df_avg = dsum / s->N_sup;
+}
+
+void rdo_sim_set_serial(rdo_sim* s, int on) { s->serial = on; }
 
 /* ---- Beeman update, boundaries, Ramo ---------------------------------------- */
 static void mark_remove(rdo_sim* s, int i, int reason) {
