@@ -240,6 +240,18 @@ int rd_set_graphs(rd_ctx* ctx, int32_t on);
  * threshold.  Library control, no reference counterpart. */
 int rd_set_jump_pairing(rd_ctx* ctx, int64_t min_workgroups);
 
+/* Forms of the mixed-precision planar accelerations (flags, 0 = both on,
+ * the default).  RD_FORCE_NO_SYMMETRIC: every simulation takes the gather
+ * form (each ordered pair evaluated at its target) instead of the
+ * pair-symmetric form from 16384 particles.  RD_FORCE_NO_FAR_EXPANSION: the
+ * image partners n >= 2 of the planar series are summed pair by pair in
+ * FP32 instead of as one FP64 expansion of their field over all particles.
+ * Library control for measurement and testing, no reference counterpart;
+ * FP64 accelerations are not affected. */
+#define RD_FORCE_NO_SYMMETRIC 1
+#define RD_FORCE_NO_FAR_EXPANSION 2
+int rd_set_force_options(rd_ctx* ctx, int32_t flags);
+
 /* Collect the particle records of each step (on != 0) or not (0, the
  * default).  Library control, no reference counterpart: the reference
  * writes these records to files inside Add_Particle and

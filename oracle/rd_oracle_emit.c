@@ -361,6 +361,7 @@ typedef struct {
   int life[2][NLIFE];
   int nthreads;
   int serial; /* 1: the reference's serial samplers (emit_serial) */
+  int emit_step; /* step of the last emission phase */
   /* particle records of the last step (the density_emit / density_absorb
    * writes of Add_Particle and Mark_Particles_Remove) */
   struct rdo_emit_rec* elog;
@@ -401,6 +402,7 @@ typedef struct {
 
 rdo_sim* rdo_sim_create(int cap, int nthreads) {
   rdo_sim* s = (rdo_sim*)calloc(1, sizeof(rdo_sim));
+  s->emit_step = -1;
   s->cap = cap;
   s->nthreads = nthreads < 1 ? 1 : nthreads;
   s->pos = calloc(3 * (size_t)cap, 8);
@@ -1266,9 +1268,12 @@ void rdo_sim_step(rdo_sim* s, int step, int phases, rdo_record* rec) {
   if (phases & 1) {
     reset_counters(s);
     do_emission(s, step);
+    s->emit_step = step;
   }
   if (phases & 2) {
-    if (!(phases & 1)) reset_counters(s);
+    /* a step's counters start at zero with its first phase: the emission
+       phase of the same step when it ran on its own, else this update */
+    if (!(phases & 1) && s->emit_step != step) reset_counters(s);
     update_position(s, 0);
   }
   if (phases & 4) remove_particles(s, step);

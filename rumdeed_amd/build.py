@@ -18,7 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "librumdeed.so")
 
-SOURCES = ["rd_api.hip", "rd_pair.hip", "rd_engine.hip"]
+SOURCES = ["rd_api.hip", "rd_pair.hip", "rd_engine.hip", "rd_tail.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -66,16 +66,23 @@ def flang() -> str | None:
     return None
 
 
+# The drop-in Fortran modules in dependency order; mod_global.F90 is the
+# mirror of the reference's module variables that they use.
+FORTRAN_MODULES = ["mod_global.F90", "rumdeed_hip.f90", "rd_device.f90", "mod_verlet.F90",
+                   "mod_pair.F90", "mod_field_emission_v2.F90", "mod_field_thermo_emission.F90",
+                   "mod_emission_tip.f90"]
+
+
 def build_fortran(verbose: bool = False) -> str | None:
-    """Compile the ISO_C_BINDING layer and its test driver with flang and
-    link them against librumdeed.so (found next to the package at run time)."""
+    """Compile the drop-in modules and their test driver with flang and link
+    them against librumdeed.so (found next to the package at run time)."""
     fc = flang()
     if fc is None:
         return None
     bdir = os.path.join(FORTRAN_DIR, "_build")
     os.makedirs(bdir, exist_ok=True)
-    cmd = [fc, "-O2", "-J", bdir, os.path.join(FORTRAN_DIR, "rumdeed_hip.f90"),
-           os.path.join(FORTRAN_DIR, "rd_fortran_tests.f90"), "-L", PKG, "-lrumdeed",
+    srcs = [os.path.join(FORTRAN_DIR, f) for f in FORTRAN_MODULES + ["rd_fortran_tests.f90"]]
+    cmd = [fc, "-O2", "-J", bdir, *srcs, "-L", PKG, "-lrumdeed",
            "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_TESTS + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -378,7 +378,8 @@ int rd_ctx_destroy(rd_ctx* c) {
   if (c->h_tab) hipHostFree(c->h_tab);
   Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pack_far, &c->pack_srf, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
                  &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo, &c->cnt,
-                 &c->mark, &c->elog, &c->alog, &c->plog};
+                 &c->mark, &c->elog, &c->alog, &c->plog, &c->tail_plan, &c->tail_work,
+                 &c->tail_box};
   for (Buf* b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -955,6 +956,14 @@ int rd_set_graphs(rd_ctx* c, int32_t on) {
   c->jump_exec = nullptr;
   c->jump_key.clear();
   c->graphs = on != 0;
+  return RD_OK;
+}
+
+int rd_set_force_options(rd_ctx* c, int32_t flags) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  if (flags & ~(RD_FORCE_NO_SYMMETRIC | RD_FORCE_NO_FAR_EXPANSION))
+    return fail(RD_ERR_ARG, "unknown force option");
+  c->force_opts = flags;
   return RD_OK;
 }
 

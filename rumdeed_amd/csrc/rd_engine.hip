@@ -2386,6 +2386,7 @@ int engine_phases(rd_ctx* c, const std::vector<int>& slots, int precision, int p
   if (phases & 1) {
     r = reset_counters(c, ents);
     if (r) return r;
+    for (const Ent& e : ents) c->slots[e.slot].emit_step = e.step;
   }
   std::vector<int> nr;
   if (phases & 1) {
@@ -2398,9 +2399,16 @@ int engine_phases(rd_ctx* c, const std::vector<int>& slots, int precision, int p
   }
   if (phases & 2) {
     if (!(phases & 1)) {
-      // counters of the removal phase start at zero in a stand-alone update
-      r = reset_counters(c, ents);
-      if (r) return r;
+      // A step's counters start at zero with its first phase: the emission
+      // phase of the same step when the caller ran it on its own, else this
+      // update.
+      std::vector<Ent> fresh;
+      for (const Ent& e : ents)
+        if (c->slots[e.slot].emit_step != e.step) fresh.push_back(e);
+      if (!fresh.empty()) {
+        r = reset_counters(c, fresh);
+        if (r) return r;
+      }
     }
     r = engine_position(c, ents);
     if (!r) r = launch_accel(c, ents, precision, nullptr, nullptr);

@@ -173,6 +173,9 @@ constexpr float kR2Min = 1.0e-20f;
 // t_n r^-3 do not cancel.  The opposite partner across the anode, at
 // zsum - 2d = -wsum, keeps its own term: it is close to the target for
 // particles near the anode.
+// NEAR: the caller guarantees nic <= 1 (no partners n >= 2), so the
+// suffix-sum loop is compiled out.
+template <bool NEAR = false>
 __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, float dxy2,
                                             int nic, float two_d, float& wxy, float& wz) {
   float wo, wzo;
@@ -198,6 +201,7 @@ __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, 
   // suffix sums of 1/r^3: p (a + t_n) and m (a - t_n), o(pposite) and s(ame)
   float po = 0.f, mo = 0.f, ps = 0.f, ms = 0.f, ro = 0.f, rs = 0.f;
   float tn = (float)nic * two_d;
+  if (!NEAR)
   for (int n = nic; n >= 2; --n) {
     RD_PARTNER_SUM(zsum + tn, po);
     RD_PARTNER_SUM(zsum - tn, mo);

@@ -95,6 +95,8 @@ struct Ent {
   int cbase, nch; // MH chains [cbase, cbase + nch)
   int step, nblk; // time step; per-slot block count (reductions)
   int wg0, wg1;   // first workgroup of the entry in a compact launch (two layouts)
+  int nic_pair;   // image partners n <= nic_pair in the mixed pair kernels (the
+                  // rest, when fewer than N_ic_max, come from rd_tail.hip)
 };
 
 // Entry of a compact launch whose workgroups [wg0, next wg0) belong to one
@@ -139,6 +141,7 @@ struct Slot {
   SimDesc desc{};
   int n = 0;         // host mirror of state.n after the last sync
   int step = 0;      // last completed step
+  int emit_step = -1;  // step of the last emission phase (its counters stand)
   rd_geometry geom{};
   bool geom_set = false;
 };
@@ -182,6 +185,12 @@ struct rd_ctx {
   rd::Buf rnd;  // random numbers of the Metropolis-Hastings jumps of a step
   rd::Buf cnt;  // accept / reject counts per accept workgroup of a jump
   rd::Buf img_hi, img_lo;  // packed sphere images of the particles (mixed, tip)
+  // far image partners as an expansion (rd_tail.hip): the plan of the last
+  // (d, N_ic_max), work arrays, lateral boxes; force options (rd_set_force_options)
+  rd::Buf tail_plan, tail_work, tail_box;
+  double tail_d = -1.0;
+  int tail_nic = -1, tail_nnz = 0;
+  int force_opts = 0;
   // particle records of the step (rd_set_logs): velocity and reason at
   // marking (4 doubles a particle), emission and absorption records
   int logs = 0;
@@ -222,10 +231,29 @@ int build_geom(const rd_geometry* in, DevGeom* g);
 // Accelerations of the particles of every entry (n from the slot state).
 int launch_accel(rd_ctx* c, const std::vector<Ent>& ents, int precision, hipEvent_t k0,
                  hipEvent_t k1);
+// Device memory given to a launch instead of the context's work buffers:
+// points and results SoA (stride `mtot`), and [next, end) for everything else
+// the launch needs (launch table, partial sums, packed copies), handed out in
+// 256-byte aligned pieces.  The launch allocates nothing and fails with
+// RD_ERR_CAPACITY when [next, end) is too small.
+struct FixedBufs {
+  double* pts;
+  double* fld;
+  char* next;
+  char* end;
+  void* take(size_t bytes) {
+    const size_t a = (bytes + 255) & ~(size_t)255;
+    if ((size_t)(end - next) < a) return nullptr;
+    void* p = next;
+    next += a;
+    return p;
+  }
+};
 // Field at the points [pbase, pbase + m) of every entry from its particles,
-// points and results SoA in c->pts / c->fld (stride `mtot`).
+// points and results SoA in c->pts / c->fld (stride `mtot`), or in fixed's
+// buffers when `fixed` is given.
 int launch_field(rd_ctx* c, const std::vector<Ent>& ents, int mtot, int precision, hipEvent_t k0,
-                 hipEvent_t k1);
+                 hipEvent_t k1, FixedBufs* fixed = nullptr);
 int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab);
 // Packed mixed-precision copies (c->pack_hi / pack_lo, and c->pack_srf when
 // `surface`) of every entry's particles.
@@ -244,6 +272,15 @@ constexpr int kWantJump = 512;   // fields of the Metropolis-Hastings chains
 constexpr int kPoolJumpChunks = 4;  // least source chunks per jump partial in a pool
 int launch_grid_points(rd_ctx* c, double x0, double y0, double lx, double ly, int nx, int ny,
                        double zz);
+
+// Far image partners (n >= 2) of the mixed planar accelerations (rd_tail.hip).
+struct TailUse {
+  int ent;             // index into the entries of the launch
+  double cx, cy, cz;   // expansion centre (nm)
+};
+bool tail_wanted(const rd_ctx* c, const DevGeom& g, int precision);
+int tail_decide(rd_ctx* c, std::vector<Ent>& ents, int precision, std::vector<TailUse>& use);
+int tail_apply(rd_ctx* c, const std::vector<Ent>& ents, const std::vector<TailUse>& use);
 
 // Engine (rd_engine.hip).
 int engine_step(rd_ctx* c, int precision, rd_step_record* rec_per_slot);

@@ -338,6 +338,7 @@ __global__ void k_pack_points(const Ent* tab, const SimDesc* desc, const double*
 
 // Mixed accelerations (planar).  part layout per entry: [3][segs][n_ub] FP64
 // sums of q_j/q_0 * (offset / r^3 + image terms) in nm^-2.
+template <bool NEAR>
 __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int ne, const SimDesc* desc,
                                                            const SimState* st,
                                                            const float4* __restrict__ HI,
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
   const int c1 = min(nchunks, c0 + e.cps);
   const DevGeom& gg = desc[e.slot].g;
   if (gg.kind != RD_GEOM_PLANAR) return;
-  const int ic = gg.ic, nic = gg.nic;
+  const int ic = gg.ic, nic = e.nic_pair;
   const float two_d = (float)(2.0 * gg.d * 1e9);
   const float4* hi = HI + e.base;
   const float4* lo = LO + e.base;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
         if (ic) {
           float wxy, wz;
           float dzab = (j > i) ? dz : -dz;
-          ic_planar32(h1.z + h2.z, l1.w + l2.w, dzab, dxy2, nic, two_d, wxy, wz);
+          ic_planar32<NEAR>(h1.z + h2.z, l1.w + l2.w, dzab, dxy2, nic, two_d, wxy, wz);
           cxy += wxy;
           cz += wz;
         }
@@ -450,6 +451,7 @@ __device__ __forceinline__ double wave_rotate(double v) {
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+template <bool NEAR>
 __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int ne,
                                                            const SimDesc* desc, const SimState* st,
                                                            const float4* __restrict__ HI,
@@ -469,7 +471,7 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
   const int n = st[e.slot].n;
   if (Y * kSymBlock >= n) return;  // X <= Y: both blocks hold particles
   const DevGeom& gg = desc[e.slot].g;
-  const int ic = gg.ic, nic = gg.nic;
+  const int ic = gg.ic, nic = e.nic_pair;
   const float two_d = (float)(2.0 * gg.d * 1e9);
   const float dnm = (float)(gg.d * 1e9);
   const float4* hi = HI + e.base;
@@ -478,10 +480,21 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
   // padding (beyond n): charge 0 at the origin, so every term it enters is 0
   const float4 h0 = make_float4(0, 0, 0, 0), l0 = make_float4(0, 0, 0, dnm);
   const int i = X * kSymBlock + t;
-  const float4 h1 = i < n ? hi[i] : h0, l1 = i < n ? lo[i] : l0;
+  // (values, not a select of addresses: selecting between a global element
+  // and a local would put the local in scratch memory)
+  float4 h1 = h0, l1 = l0;
+  if (i < n) {
+    h1 = hi[i];
+    l1 = lo[i];
+  }
   const int js = Y * kSymBlock + t;
-  shi[t] = js < n ? hi[js] : h0;
-  slo[t] = js < n ? lo[js] : l0;
+  float4 hs = h0, ls = l0;
+  if (js < n) {
+    hs = hi[js];
+    ls = lo[js];
+  }
+  shi[t] = hs;
+  slo[t] = ls;
   jsum[0][t] = 0.0;
   jsum[1][t] = 0.0;
   jsum[2][t] = 0.0;
@@ -506,7 +519,7 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
       if (ic) {
         float wxy, wz;
         float dzab = (j > i) ? dz : -dz;
-        ic_planar32(h1.z + h2.z, l1.w + l2.w, dzab, dxy2, nic, two_d, wxy, wz);
+        ic_planar32<NEAR>(h1.z + h2.z, l1.w + l2.w, dzab, dxy2, nic, two_d, wxy, wz);
         cxy += wxy;
         cz += wz;
       }
@@ -539,7 +552,7 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
       float cxy = r3, czi = dz * r3, czj = -czi;
       if (ic) {
         float wxy, wz;
-        ic_planar32(h1.z + h2.z, l1.w + l2.w, dz, dxy2, nic, two_d, wxy, wz);
+        ic_planar32<NEAR>(h1.z + h2.z, l1.w + l2.w, dz, dxy2, nic, two_d, wxy, wz);
         cxy += wxy;
         czi += wz;
         czj += wz;
@@ -869,7 +882,8 @@ int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab) {
 // form, whose source segments spread one simulation over more workgroups,
 // finishes sooner.
 static bool sym_form(const rd_ctx* c, const Ent& e) {
-  return c->slots[e.slot].desc.g.kind == RD_GEOM_PLANAR && e.n_ub >= kSymMin;
+  return c->slots[e.slot].desc.g.kind == RD_GEOM_PLANAR && e.n_ub >= kSymMin &&
+         !(c->force_opts & RD_FORCE_NO_SYMMETRIC);
 }
 
 int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_t k0,
@@ -889,6 +903,10 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   if (ents.empty()) return RD_OK;
   if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
     return fail(RD_ERR_ARG, "unknown precision");
+  // far image partners by expansion where it converges (sets nic_pair)
+  std::vector<TailUse> tail;
+  int r = tail_decide(c, ents, precision, tail);
+  if (r) return r;
   long long off = 0;
   int nwg = 0;   // workgroups of the compact gather launch
   int nsym = 0;  // workgroups of the pair-symmetric launch
@@ -910,7 +928,7 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
     e.part_off = off;
     off += 3LL * (precision == RD_PREC_FP64 ? nch : e.segs) * e.n_ub;
   }
-  int r = ensure(c, c->part, sizeof(double) * (size_t)off);
+  r = ensure(c, c->part, sizeof(double) * (size_t)off);
   if (r) return r;
   Ent* dt = nullptr;
   r = upload_tab(c, ents, &dt);
@@ -952,15 +970,30 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
     k_pack_images<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->x, c->y, c->z,
                                                (float4*)c->img_hi.p, (float4*)c->img_lo.p);
   }
+  // every entry within partners n <= 1: the pair kernels without the series loop
+  bool near = true;
+  for (const Ent& e : ents) near &= e.nic_pair <= 1;
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-  if (nsym > 0)
-    k_accel_sym32<<<nsym, kSymBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
-                                                      (const float4*)c->pack_hi.p,
-                                                      (const float4*)c->pack_lo.p, part);
-  if (any_planar && grid > 0)
-    k_accel_planar32<<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state,
-                                                      (const float4*)c->pack_hi.p,
-                                                      (const float4*)c->pack_lo.p, part);
+  if (nsym > 0) {
+    if (near)
+      k_accel_sym32<true><<<nsym, kSymBlock, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
+          (const float4*)c->pack_lo.p, part);
+    else
+      k_accel_sym32<false><<<nsym, kSymBlock, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
+          (const float4*)c->pack_lo.p, part);
+  }
+  if (any_planar && grid > 0) {
+    if (near)
+      k_accel_planar32<true><<<grid, kBlock, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
+          (const float4*)c->pack_lo.p, part);
+    else
+      k_accel_planar32<false><<<grid, kBlock, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
+          (const float4*)c->pack_lo.p, part);
+  }
   if (any_tip)
     k_accel_tip32<<<grid, kBlock, 0, c->stream>>>(
         dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p,
@@ -969,11 +1002,12 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->x, c->y,
                                                 c->z, c->q, c->m, c->ax, c->ay, c->az);
   RD_HIP(hipGetLastError());
+  if (!tail.empty()) return tail_apply(c, ents, tail);
   return RD_OK;
 }
 
 int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision, hipEvent_t k0,
-                 hipEvent_t k1) {
+                 hipEvent_t k1, FixedBufs* fixed) {
   std::vector<Ent> ents;
   int maxm = 0;
   bool any_tip = false, any_planar = false;
@@ -1002,19 +1036,30 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
     e.wg0 = nwg;
     nwg += gx * e.segs;
   }
-  int r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
-  if (r) return r;
+  const size_t part_bytes = sizeof(double) * (size_t)std::max(off, 1LL);
+  double* part = nullptr;
   Ent* dt = nullptr;
-  r = upload_tab(c, ents, &dt);
-  if (r) return r;
+  int r;
+  if (fixed) {
+    part = (double*)fixed->take(part_bytes);
+    dt = (Ent*)fixed->take(sizeof(Ent) * ents.size());
+    if (!part || !dt) return fail(RD_ERR_CAPACITY, "field launch exceeds the fixed buffer");
+    RD_HIP(hipMemcpyAsync(dt, ents.data(), sizeof(Ent) * ents.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  } else {
+    r = ensure(c, c->part, part_bytes);
+    if (r) return r;
+    part = (double*)c->part.p;
+    r = upload_tab(c, ents, &dt);
+    if (r) return r;
+  }
   const int ne = (int)ents.size();
-  double* ptx = (double*)c->pts.p;
+  double* ptx = fixed ? fixed->pts : (double*)c->pts.p;
   double* pty = ptx + mtot;
   double* ptz = pty + mtot;
-  double* fx = (double*)c->fld.p;
+  double* fx = fixed ? fixed->fld : (double*)c->fld.p;
   double* fy = fx + mtot;
   double* fz = fy + mtot;
-  double* part = (double*)c->part.p;
   const int grid = nwg;
   dim3 fin((maxm + 255) / 256, 1, ne);
   if (precision == RD_PREC_FP64) {
@@ -1033,17 +1078,6 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
     RD_HIP(hipGetLastError());
     return RD_OK;
   }
-  r = ensure(c, c->pts_hi, sizeof(float4) * (size_t)mtot);
-  if (!r) r = ensure(c, c->pts_lo, sizeof(float4) * (size_t)mtot);
-  if (!r) r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)c->cap);
-  if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)c->cap);
-  if (r) return r;
-  k_pack_points<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, ptx, pty, ptz, (float4*)c->pts_hi.p,
-                                             (float4*)c->pts_lo.p);
-  if (maxn > 0)
-    k_pack_particles<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(
-        dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (float4*)c->pack_hi.p,
-        (float4*)c->pack_lo.p, nullptr);
   // far-partner series of the sources, for planar entries with image partners
   // beyond n = 0 (used where a wavefront's points lie on the cathode)
   bool any_far = false;
@@ -1051,22 +1085,51 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
     const DevGeom& g = c->slots[e.slot].desc.g;
     any_far |= g.kind == RD_GEOM_PLANAR && g.ic && g.nic >= 1;
   }
-  if (any_far && maxn > 0) {
-    r = ensure(c, c->pack_far, 2 * sizeof(float4) * (size_t)c->cap);
+  // Packed copies.  The kernels index the particle copies by store index
+  // (base + i); fixed buffers hold only the single entry's particles, so
+  // their pointers are shifted down by that entry's base.
+  float4 *phi = nullptr, *plo = nullptr, *ahi = nullptr, *alo = nullptr, *afar = nullptr;
+  if (fixed) {
+    if (ne != 1) return fail(RD_ERR_ARG, "fixed buffers serve one simulation");
+    const size_t np = (size_t)std::max(maxn, 1);
+    phi = (float4*)fixed->take(sizeof(float4) * (size_t)mtot);
+    plo = (float4*)fixed->take(sizeof(float4) * (size_t)mtot);
+    float4* h = (float4*)fixed->take(sizeof(float4) * np);
+    float4* l = (float4*)fixed->take(sizeof(float4) * np);
+    float4* f = any_far ? (float4*)fixed->take(2 * sizeof(float4) * np) : h;
+    if (!phi || !plo || !h || !l || !f)
+      return fail(RD_ERR_CAPACITY, "field launch exceeds the fixed buffer");
+    const uintptr_t shift = sizeof(float4) * (uintptr_t)ents[0].base;
+    ahi = (float4*)((uintptr_t)h - shift);
+    alo = (float4*)((uintptr_t)l - shift);
+    afar = (float4*)((uintptr_t)f - 2 * shift);
+  } else {
+    r = ensure(c, c->pts_hi, sizeof(float4) * (size_t)mtot);
+    if (!r) r = ensure(c, c->pts_lo, sizeof(float4) * (size_t)mtot);
+    if (!r) r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)c->cap);
+    if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)c->cap);
+    if (!r && any_far && maxn > 0) r = ensure(c, c->pack_far, 2 * sizeof(float4) * (size_t)c->cap);
     if (r) return r;
-    k_pack_far<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(
-        dt, c->d_desc, c->d_state, c->z, (float4*)c->pack_far.p);
+    phi = (float4*)c->pts_hi.p;
+    plo = (float4*)c->pts_lo.p;
+    ahi = (float4*)c->pack_hi.p;
+    alo = (float4*)c->pack_lo.p;
+    afar = (float4*)c->pack_far.p;
   }
+  k_pack_points<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, ptx, pty, ptz, phi, plo);
+  if (maxn > 0)
+    k_pack_particles<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(
+        dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, ahi, alo, nullptr);
+  if (any_far && maxn > 0)
+    k_pack_far<<<dim3((maxn + 255) / 256, 1, ne), 256, 0, c->stream>>>(dt, c->d_desc, c->d_state,
+                                                                         c->z, afar);
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
   if (any_planar)
-    k_field_planar32<<<grid, kBlock, 0, c->stream>>>(
-        dt, ne, c->d_desc, c->d_state, (const float4*)c->pts_hi.p, (const float4*)c->pts_lo.p,
-        (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p,
-        any_far ? (const float4*)c->pack_far.p : nullptr, part);
+    k_field_planar32<<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state, phi, plo,
+                                                      ahi, alo, any_far ? afar : nullptr, part);
   if (any_tip)
     k_field_tip32<<<grid, kBlock, 0, c->stream>>>(dt, ne, c->d_desc, c->d_state, ptx, pty, ptz,
-                                                   (const float4*)c->pack_hi.p,
-                                                   (const float4*)c->pack_lo.p, part);
+                                                   ahi, alo, part);
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
   k_field_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, ptx, pty, ptz,
                                                 fx, fy, fz);

@@ -111,6 +111,13 @@ class Diode:
         """Launch each step's Metropolis-Hastings jump loop as one HIP graph."""
         check(self._lib.rd_set_graphs(self._h, int(bool(on))))
 
+    def set_force_options(self, no_symmetric: bool = False, no_far_expansion: bool = False):
+        """Forms of the mixed planar accelerations (rd_set_force_options):
+        by default the pair-symmetric form from 16384 particles and the far
+        image partners (n >= 2) as one FP64 expansion."""
+        flags = (1 if no_symmetric else 0) | (2 if no_far_expansion else 0)
+        check(self._lib.rd_set_force_options(self._h, flags))
+
     def set_jump_pairing(self, min_workgroups: int):
         """Pool launches of at least min_workgroups jump workgroups pair their
         partial sums (results unchanged; 0: always)."""

@@ -1,19 +1,8 @@
-! ISO_C_BINDING interface of librumdeed.so (include/rumdeed.h) and drop-in
-! routines with the RUMDEED module interfaces on top of it.
-!
-!   module rumdeed_hip      bind(C) interfaces, derived types, constants
-!   module rumdeed_dropin   routines named and shaped like the reference's:
-!     Calculate_Acceleration_Particles   (src/mod_verlet.F90:597-620)
-!     Calc_Field_at / Calc_Field_at_Batch (src/mod_verlet.F90:1466-1529, :1635-1911)
-!     Update_Position                     (src/mod_verlet.F90:115-162)
-!     Remove_Particles                    (src/mod_pair.F90:352-562)
-!     Add_Particle                        (src/mod_pair.F90:29-159)
-!     Do_Emission                         (ptr_Do_Emission, src/mod_global.F90:445-506)
-!   The reference keeps its particle arrays in mod_global; the device keeps
-!   them in HBM.  The host-array routines upload, compute and download (the
-!   reference's OpenACC residency pattern); the step routines work on the
-!   device-resident state, which is what a main loop calling Update_Position /
-!   Remove_Particles / Do_Emission per step uses.
+! ISO_C_BINDING interface of librumdeed.so (include/rumdeed.h): the bind(C)
+! types, constants and entry points.  The drop-in modules of this directory
+! (mod_verlet, mod_pair, mod_field_emission_v2, mod_field_thermo_emission,
+! mod_emission_tip, over the context of rd_device) are built on it.
+! Buffers the C ABI accepts as NULL are passed as type(c_ptr).
 module rumdeed_hip
   use, intrinsic :: iso_c_binding
   implicit none
@@ -87,17 +76,21 @@ module rumdeed_hip
       integer(c_int) :: r
     end function
     function rd_put_particles(ctx, n, pos, vel, q, m, species) bind(C, name='rd_put_particles') result(r)
-      import :: c_int, c_ptr, c_double
+      import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: n
-      real(c_double), intent(in) :: pos(3, *), vel(3, *), q(*), m(*)
-      integer(c_int), intent(in) :: species(*)
+      type(c_ptr), value :: pos, vel, q, m, species
       integer(c_int) :: r
     end function
     function rd_particle_count(ctx, n) bind(C, name='rd_particle_count') result(r)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int) :: n
+      integer(c_int) :: r
+    end function
+    function rd_get_particles(ctx, pos, vel, q, m, species) bind(C, name='rd_get_particles') result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, pos, vel, q, m, species
       integer(c_int) :: r
     end function
     function rd_accel(ctx, precision) bind(C, name='rd_accel') result(r)
@@ -122,11 +115,10 @@ module rumdeed_hip
     end function
     function rd_add_particles(ctx, n, pos, vel, species, step, emit, sec) &
         bind(C, name='rd_add_particles') result(r)
-      import :: c_int, c_ptr, c_double
+      import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: n, step, emit
-      real(c_double), intent(in) :: pos(3, *), vel(3, *)
-      integer(c_int), intent(in) :: species(*), sec(*)
+      type(c_ptr), value :: pos, vel, species, sec
       integer(c_int) :: r
     end function
     function rd_update_position(ctx, step, precision, rec) bind(C, name='rd_update_position') result(r)
@@ -151,10 +143,14 @@ module rumdeed_hip
       integer(c_int) :: r
     end function
     function rd_get_state(ctx, pos, prev, vel, acc, a1, a2, ints) bind(C, name='rd_get_state') result(r)
-      import :: c_int, c_ptr, c_double
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, pos, prev, vel, acc, a1, a2, ints
+      integer(c_int) :: r
+    end function
+    function rd_get_life_time(ctx, hist) bind(C, name='rd_get_life_time') result(r)
+      import :: c_int, c_ptr
       type(c_ptr), value :: ctx
-      real(c_double) :: pos(3, *), prev(3, *), vel(3, *), acc(3, *), a1(3, *), a2(3, *)
-      integer(c_int) :: ints(5, *)
+      integer(c_int) :: hist(1000, 2)
       integer(c_int) :: r
     end function
   end interface
@@ -180,107 +176,3 @@ contains
     error stop 1
   end subroutine rd_check
 end module rumdeed_hip
-
-
-module rumdeed_dropin
-  use, intrinsic :: iso_c_binding
-  use rumdeed_hip
-  implicit none
-  type(c_ptr) :: rd_ctx = c_null_ptr
-  integer(c_int) :: rd_precision = RD_PREC_FP64
-
-contains
-
-  subroutine RD_Init(max_particles, geom, dt)
-    integer, intent(in) :: max_particles
-    type(rd_geometry), intent(in) :: geom
-    double precision, intent(in) :: dt
-    call rd_check(rd_ctx_create(0_c_int, int(max_particles, c_int64_t), rd_ctx), 'rd_ctx_create')
-    call rd_check(rd_set_geometry(rd_ctx, geom), 'rd_set_geometry')
-    call rd_check(rd_set_time_step(rd_ctx, dt), 'rd_set_time_step')
-  end subroutine RD_Init
-
-  subroutine RD_Finalize()
-    integer(c_int) :: r
-    r = rd_ctx_destroy(rd_ctx)
-    rd_ctx = c_null_ptr
-  end subroutine RD_Finalize
-
-  ! Calculate_Acceleration_Particles on host arrays: the reference adds the
-  ! result onto particles_cur_accel (zeroed by the position update).
-  subroutine Calculate_Acceleration_Particles(n, cur_pos, charge, mass, species, cur_accel)
-    integer, intent(in) :: n
-    double precision, intent(in) :: cur_pos(3, n), charge(n), mass(n)
-    integer, intent(in) :: species(n)
-    double precision, intent(inout) :: cur_accel(3, n)
-    double precision, allocatable :: vel(:, :), acc(:, :)
-    integer(c_int), allocatable :: sp(:)
-    allocate(vel(3, n), acc(3, n), sp(n))
-    vel = 0.0d0
-    sp = int(species, c_int)
-    call rd_check(rd_put_particles(rd_ctx, int(n, c_int), cur_pos, vel, charge, mass, sp), &
-                  'rd_put_particles')
-    call rd_check(rd_accel(rd_ctx, rd_precision), 'rd_accel')
-    call rd_check(rd_get_accel(rd_ctx, acc), 'rd_get_accel')
-    cur_accel = cur_accel + acc
-  end subroutine Calculate_Acceleration_Particles
-
-  ! Calc_Field_at_Batch on the particles last put / resident on the device.
-  subroutine Calc_Field_at_Batch(M, pos_in, field_out)
-    integer, intent(in) :: M
-    double precision, intent(in) :: pos_in(3, M)
-    double precision, intent(out) :: field_out(3, M)
-    call rd_check(rd_field_batch(rd_ctx, int(M, c_int), pos_in, field_out, rd_precision), &
-                  'rd_field_batch')
-  end subroutine Calc_Field_at_Batch
-
-  function Calc_Field_at(pos_xyz) result(field)
-    double precision, intent(in) :: pos_xyz(3)
-    double precision :: field(3)
-    double precision :: p(3, 1), f(3, 1)
-    p(:, 1) = pos_xyz
-    call Calc_Field_at_Batch(1, p, f)
-    field = f(:, 1)
-  end function Calc_Field_at
-
-  ! Add_Particle (one particle, appended to the device-resident state).
-  ! `life` (particles_life, src/mod_pair.F90:73) is read only by the ion
-  ! collision model (src/mod_collisions.F90:123), which this path does not
-  ! include; it is accepted for interface compatibility and not stored.
-  subroutine Add_Particle(par_pos, par_vel, par_species, step, emit, life, opt_sec)
-    double precision, intent(in) :: par_pos(3), par_vel(3)
-    integer, intent(in) :: par_species, step, emit, life
-    integer, intent(in), optional :: opt_sec
-    integer(c_int) :: sp(1), sec(1)
-    sp(1) = int(par_species, c_int)
-    sec(1) = 1
-    if (present(opt_sec)) sec(1) = int(opt_sec, c_int)
-    call rd_check(rd_add_particles(rd_ctx, 1_c_int, par_pos, par_vel, sp, int(step, c_int), &
-                  int(emit, c_int), sec), 'rd_add_particles')
-  end subroutine Add_Particle
-
-  subroutine Update_Position(step, rec)
-    integer, intent(in) :: step
-    type(rd_step_record), intent(out) :: rec
-    call rd_check(rd_update_position(rd_ctx, int(step, c_int), rd_precision, rec), &
-                  'rd_update_position')
-  end subroutine Update_Position
-
-  subroutine Remove_Particles(step, rec)
-    integer, intent(in) :: step
-    type(rd_step_record), intent(out) :: rec
-    call rd_check(rd_remove_particles(rd_ctx, int(step, c_int), rec), 'rd_remove_particles')
-  end subroutine Remove_Particles
-
-  subroutine Do_Emission(step, rec)
-    integer, intent(in) :: step
-    type(rd_step_record), intent(out) :: rec
-    call rd_check(rd_do_emission(rd_ctx, int(step, c_int), rd_precision, rec), 'rd_do_emission')
-  end subroutine Do_Emission
-
-  integer function RD_Count()
-    integer(c_int) :: n
-    call rd_check(rd_particle_count(rd_ctx, n), 'rd_particle_count')
-    RD_Count = n
-  end function RD_Count
-end module rumdeed_dropin

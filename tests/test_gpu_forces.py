@@ -268,3 +268,52 @@ def test_absorbed_particles_have_no_effect(Diode, oracle_mod):
     am = dd.calculate_acceleration_particles("mixed")
     ref = oracle_mod.accel(g, pos[keep], q[keep], m[keep], long_double=True)
     assert rel_err(am[keep], ref).max() < MIXED_TOL
+
+
+@pytest.mark.parametrize("n,form", [(4096, "gather"), (16384, "symmetric"), (20000, "symmetric")])
+def test_mixed_far_partner_expansion(Diode, oracle_mod, n, form):
+    """With N_ic_max >= 2 the mixed accelerations take the image partners
+    n >= 2 from one FP64 expansion of their field (the pair kernels keep
+    n <= 1).  Against the form that sums every partner pair by pair: within
+    1e-6 relative of the total (the far part is about 1e-3 of it); against
+    the long-double oracle on sampled targets: within 1e-5; absorbed
+    particles (charge 0) feel and exert nothing."""
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(n, seed=23)
+    q = q.copy()
+    q[11::101] = 0.0
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, 1, 10)
+    if form == "gather":
+        dd.set_force_options(no_symmetric=True)
+    a_exp = dd.calculate_acceleration_particles("mixed")
+    dd.set_force_options(no_symmetric=form == "gather", no_far_expansion=True)
+    a_dir = dd.calculate_acceleration_particles("mixed")
+    dd.close()
+    keep = q != 0.0
+    e = rel_err(a_exp[keep], a_dir[keep])
+    print(f"far expansion n={n} {form}: vs direct max {e.max():.2e} p99 {np.percentile(e, 99):.2e}")
+    assert e.max() < 1e-6
+    assert np.all(a_exp[~keep] == 0.0)
+    g = oracle_mod.geom_planar(W.SNAPSHOT_D, W.SNAPSHOT_V, True, 10)
+    t = np.flatnonzero(keep)[::max(1, n // 512)]
+    ref = oracle_mod.accel_subset(g, pos, q, m, t, long_double=True)
+    el = rel_err(a_exp[t], ref)
+    print(f"far expansion n={n}: vs long double max {el.max():.2e}")
+    assert el.max() < MIXED_TOL
+
+
+def test_far_partner_expansion_falls_back_when_it_would_not_converge(Diode):
+    """Particles spread laterally over several gaps: the expansion would not
+    converge, so the pair kernels keep every partner -- the accelerations
+    equal the pair-by-pair form bit for bit."""
+    from rumdeed_amd import workloads as W
+
+    pos, q, m = W.snapshot(3000, seed=4)
+    pos[:, :2] *= 20.0  # lateral +-2000 nm, gap 1000 nm
+    dd = _diode_planar(Diode, pos, q, m, W.SNAPSHOT_D, W.SNAPSHOT_V, 1, 10)
+    a = dd.calculate_acceleration_particles("mixed")
+    dd.set_force_options(no_far_expansion=True)
+    b = dd.calculate_acceleration_particles("mixed")
+    dd.close()
+    assert np.array_equal(a, b)
