@@ -160,6 +160,24 @@ int rd_get_accel(rd_ctx* ctx, double* acc);
  * each point as Calc_Field_at, :1466-1529).  pts, field: (3,m) host buffers. */
 int rd_field_batch(rd_ctx* ctx, int32_t m, const double* pts, double* field, int32_t precision);
 
+/* Allocation-free variant of rd_field_batch.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, errors and results as rd_field_batch, bit for
+ * bit.  Every device buffer it needs beyond the context's particle store --
+ * host staging, points, fields, partial sums, launch table, packed copies --
+ * comes from one file-scope static device array of RD_STATIC_SCRATCH_BYTES
+ * per device, held from the library's load on that device until the process
+ * ends; the call makes no device or pinned-host allocation.  It needs about
+ * 72 B per point plus 24 B per point per 256 particles (FP64 partial sums;
+ * mixed: 32 B per point and 32-64 B per particle more), and returns
+ * RD_ERR_CAPACITY beyond the array.  Every context on a device shares that
+ * device's array, which keeps the last call's contents: calls that overlap on
+ * one device race. */
+#define RD_STATIC_SCRATCH_BYTES (64ull << 20)
+int rd_field_batch_static(rd_ctx* ctx, int32_t m, const double* pts, double* field,
+                          int32_t precision);
+
 /* Field on the nx*ny cell centres of the rectangle [x0,x0+lx]x[y0,y0+ly] at
  * height z (points ordered x fastest), kept in HBM; copied to `field` (3,nx*ny)
  * when it is not NULL.  The surface grid of the force/field snapshot. */

@@ -106,6 +106,7 @@ _SIGS = {
     "rd_accel": (c_int32, [c_void_p, c_int32]),
     "rd_get_accel": (c_int32, [c_void_p, _dp]),
     "rd_field_batch": (c_int32, [c_void_p, c_int32, _dp, _dp, c_int32]),
+    "rd_field_batch_static": (c_int32, [c_void_p, c_int32, _dp, _dp, c_int32]),
     "rd_set_force_options": (c_int32, [c_void_p, c_int32]),
     "rd_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double, c_int32,
                                 c_int32, c_double, c_int32, _dp]),

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -750,6 +752,88 @@ int rd_field_batch(rd_ctx* c, int32_t m, const double* pts, double* field, int32
   if (r) return r;
   double* f = (double*)c->fld.p;
   return download_vec3(c, m, f, f + m, f + 2 * m, field);
+}
+
+}  // extern "C"
+
+// ---- allocation-free field at points -------------------------------------
+// Work memory of rd_field_batch_static: one array per device, reserved when
+// the library's code object is loaded on that device and held until the
+// process ends.  Host code never uses the array's name as an address: its
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[RD_STATIC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch;  // device -> address of `scratch` on it
+
+// The device address of `scratch` on the current device `dev`.
+int scratch_on(int dev, char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it != g_scratch.end()) {
+    *out = it->second;
+    return RD_OK;
+  }
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  if (!p) return fail(RD_ERR_HIP, "no device address for the static work array");
+  g_scratch[dev] = (char*)p;
+  *out = (char*)p;
+  return RD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rd_field_batch_static(rd_ctx* c, int32_t m, const double* pts, double* field,
+                          int32_t precision) {
+  RD_CUR(c);
+  if (!sl->geom_set) return fail(RD_ERR_STATE, "geometry not set");
+  if (m < 0) return fail(RD_ERR_ARG, "negative point count");
+  if (m == 0) return RD_OK;
+  if (!pts || !field) return fail(RD_ERR_ARG, "null point/field buffer");
+  if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
+    return fail(RD_ERR_ARG, "unknown precision");
+  char* base = nullptr;
+  int r = scratch_on(c->device, &base);
+  if (r) return r;
+  // staging (AoS), points and fields (SoA), then the launch's own pieces
+  FixedBufs fb;
+  fb.next = base;
+  fb.end = base + RD_STATIC_SCRATCH_BYTES;
+  const size_t v3 = sizeof(double) * 3 * (size_t)m;
+  double* stage = (double*)fb.take(v3);
+  fb.pts = (double*)fb.take(v3);
+  fb.fld = (double*)fb.take(v3);
+  if (!stage || !fb.pts || !fb.fld)
+    return fail(RD_ERR_CAPACITY, "rd_field_batch_static: " + std::to_string(m) +
+                                     " points exceed the static work array of " +
+                                     std::to_string(RD_STATIC_SCRATCH_BYTES >> 20) + " MiB");
+  RD_HIP(hipMemcpyAsync(stage, pts, v3, hipMemcpyHostToDevice, c->stream));
+  k_aos_to_soa<<<(m + 255) / 256, 256, 0, c->stream>>>(m, stage, fb.pts, fb.pts + m,
+                                                        fb.pts + 2 * m);
+  RD_HIP(hipGetLastError());
+  Ent e = slot_ent(c, c->cur);
+  e.pbase = 0;
+  e.m = m;
+  std::vector<Ent> ents{e};
+  r = launch_field(c, ents, m, precision, nullptr, nullptr, &fb);
+  if (r) {
+    if (r == RD_ERR_CAPACITY)
+      return fail(RD_ERR_CAPACITY,
+                  "rd_field_batch_static: " + std::to_string(m) + " points over " +
+                      std::to_string(sl->n) + " particles exceed the static work array of " +
+                      std::to_string(RD_STATIC_SCRATCH_BYTES >> 20) + " MiB");
+    return r;
+  }
+  k_soa_to_aos<<<(m + 255) / 256, 256, 0, c->stream>>>(m, fb.fld, fb.fld + m, fb.fld + 2 * m,
+                                                        stage);
+  RD_HIP(hipGetLastError());
+  RD_HIP(hipMemcpyAsync(field, stage, v3, hipMemcpyDeviceToHost, c->stream));
+  RD_HIP(hipStreamSynchronize(c->stream));
+  return RD_OK;
 }
 
 int rd_field_grid(rd_ctx* c, double x0, double y0, double lx, double ly, int32_t nx, int32_t ny,

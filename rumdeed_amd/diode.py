@@ -231,6 +231,15 @@ class Diode:
                                        _prec(precision)))
         return out
 
+    def calc_field_at_batch_static(self, pts, precision="fp64"):
+        """Calc_Field_at_Batch through rd_field_batch_static, which allocates
+        no device memory (its work memory is a static array per device)."""
+        pts = f64(pts).reshape(-1, 3)
+        out = np.zeros_like(pts)
+        check(self._lib.rd_field_batch_static(self._h, pts.shape[0], dptr(pts), dptr(out),
+                                              _prec(precision)))
+        return out
+
     def calc_field_at(self, pos, precision="fp64"):
         """Calc_Field_at: field (3,) at one point."""
         return self.calc_field_at_batch(np.asarray(pos, dtype=np.float64).reshape(1, 3),
