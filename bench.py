@@ -41,10 +41,10 @@ COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per launch
 # HBM bytes per dispatch of the dominant kernel from the FETCH_SIZE and
 # WRITE_SIZE counter passes over `bench.py --profile-pass` (one rocprofv3
 # --pmc run per counter), each scaled by what the counter reads for the 1 GiB
-# calibration copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot be
-# read from inside this process, so the figures are those of the committed
-# summary profiles/snapshot65536_nic10_hbm_counters.txt, for that workload only.
-MEASURED_TRAFFIC = {("k_accel_sym32", 65536, 10): 39.169e6 + 168.819e6}
+# calibration copy.  Counters cannot be read from inside this process, so the
+# figures are those of the committed summary profiles/
+# snapshot65536_nic10_hbm_counters.txt, for that workload and kernel form only.
+MEASURED_TRAFFIC = {}
 
 
 def cpu_baseline(n_sample: int, nic: int, threads: int):
@@ -73,18 +73,52 @@ def cpu_baseline(n_sample: int, nic: int, threads: int):
         kind = "port"
         what = "oracle FP64 gather restatement (gcc -O2 -fopenmp)"
     pairs = n_sample * (n_sample - 1)
-    cpu = "unknown CPU"
+    host = host_cpu()
+    return {"value": pairs / dt, "unit": "pair-interactions/s", "cores": threads, "kind": kind,
+            "host": host,
+            "sample": f"one force evaluation, N={n_sample}, N_ic_max={nic}, {what}, "
+                      f"{dt:.2f} s, {threads} OpenMP threads on {host['model']}"}
+
+
+def host_cpu():
+    """The host CPU: model, sockets x cores x threads (lscpu's figures from
+    /proc/cpuinfo) and the CPUs this process may run on."""
+    model, phys, cores, siblings = "unknown CPU", set(), 0, 0
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys.add(v)
+                elif k == "cpu cores":
+                    cores = int(v)
+                elif k == "siblings":
+                    siblings = int(v)
     except OSError:
         pass
-    return {"value": pairs / dt, "unit": "pair-interactions/s", "cores": threads, "kind": kind,
-            "sample": f"one force evaluation, N={n_sample}, N_ic_max={nic}, {what}, "
-                      f"{dt:.2f} s, {threads} OpenMP threads on {cpu}"}
+    sockets = max(1, len(phys))
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    return {"model": model, "sockets": sockets, "cores_per_socket": cores,
+            "threads_per_core": (siblings // cores) if cores else None,
+            "physical_cores": sockets * cores, "cpus_available_to_process": share}
+
+
+def default_cpu_threads():
+    """OpenMP threads of the CPU baseline: OMP_NUM_THREADS when set (the
+    host's share for this process), else every CPU the process may use."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def serve_bench(n, precision, realtime, steps, rate, graphs=False):
@@ -192,6 +226,53 @@ def e2e_bench(name, steps, precision, graphs=False):
     return out
 
 
+def ensemble_bench(name, seeds, steps):
+    """Ensemble-mean current of an end-to-end configuration (the statistical
+    parity criterion: current within 1%): `seeds` simulations of one pool
+    per precision, `steps` steps each; the mean Ramo current over the second
+    half of each run, its mean over seeds and the standard error."""
+    from rumdeed_amd.ensemble import DiodeSpec, run_ensemble, steady_current
+
+    cfg = dict(E2E[name])
+    if steps:
+        cfg["steps"] = steps
+    if cfg["kind"] == "planar":
+        wf = [[cfg.get("phi", 4.7)]]
+        if cfg["mode"] == 9:
+            wf = np.where((np.arange(10)[:, None] + np.arange(10)[None, :]) % 2 == 0,
+                          4.0, 4.5).tolist()
+        spec = DiodeSpec(kind="planar", mode=cfg["mode"], V=cfg["V"], d=cfg["d"],
+                         side=cfg["side"], wf=wf, T=cfg.get("T", 300.0), dt=cfg["dt"],
+                         nic=cfg["nic"])
+    else:
+        spec = DiodeSpec(kind="tip", mode=cfg["mode"], V=cfg["V"], d_tip=cfg["d_tip"],
+                         R_base=cfg["R_base"], h_tip=cfg["h_tip"], phi=cfg["phi"],
+                         box_top=cfg["box_top"], dt=cfg["dt"])
+    out = {"workload": name, "seeds": seeds, "steps": cfg["steps"]}
+    cur = {}
+    for prec in ("mixed", "fp64"):
+        t0 = time.perf_counter()
+
+        def progress(k, prec=prec):
+            print(f"[ensemble {name} {prec}] step {k}", file=sys.stderr, flush=True)
+
+        recs, _ = run_ensemble(spec, range(1, seeds + 1), cfg["steps"], prec,
+                               capacity=1 << 16, progress=progress)
+        wall = time.perf_counter() - t0
+        cur[prec] = np.array([steady_current(r) for r in recs])
+        out[prec] = {"wall_s": wall, "I_mean_A": float(cur[prec].mean()),
+                     "I_se_A": float(cur[prec].std(ddof=1) / np.sqrt(seeds)),
+                     "I_per_seed_A": cur[prec].tolist(),
+                     "electrons_final_mean": float(np.mean([r[-1]["n"] for r in recs])),
+                     "emitted_per_seed": [int(sum(x["emitted"] for x in r)) for r in recs]}
+    d = cur["mixed"] - cur["fp64"]
+    out["rel_diff_mixed_vs_fp64"] = float(d.mean() / cur["fp64"].mean())
+    out["rel_se_of_diff"] = float(d.std(ddof=1) / np.sqrt(seeds) / cur["fp64"].mean())
+    out["within_1pct"] = bool(abs(out["rel_diff_mixed_vs_fp64"]) <= 0.01)
+    out["mixed_over_fp64_wall"] = out["fp64"]["wall_s"] / out["mixed"]["wall_s"]
+    return out
+
+
 def serving_summary(n, steps):
     """The metric's batched half on a short cut of the serving stream: n
     requests of `steps` steps.  Mixed precision with Poisson arrivals at 4/s
@@ -247,6 +328,10 @@ def main():
     ap.add_argument("--e2e", default="", help="end-to-end workload: " + ", ".join(E2E))
     ap.add_argument("--e2e-steps", type=int, default=0)
     ap.add_argument("--e2e-precision", default="mixed")
+    ap.add_argument("--ensemble", default="",
+                    help="ensemble-mean current, mixed vs FP64: " + ", ".join(E2E))
+    ap.add_argument("--ensemble-seeds", type=int, default=8)
+    ap.add_argument("--ensemble-steps", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -254,7 +339,8 @@ def main():
     ap.add_argument("--nic", type=int, default=10)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--fp64-steps", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="particles of the CPU baseline's force evaluation (0: the workload's)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-summary", type=int, default=64,
@@ -278,6 +364,10 @@ def main():
     if args.e2e:
         print(json.dumps({"e2e": e2e_bench(args.e2e, args.e2e_steps, args.e2e_precision,
                                            args.graphs)}), flush=True)
+        return
+    if args.ensemble:
+        print(json.dumps({"ensemble": ensemble_bench(args.ensemble, args.ensemble_seeds,
+                                                     args.ensemble_steps)}), flush=True)
         return
 
     from rumdeed_amd import workloads as W
@@ -310,10 +400,22 @@ def main():
     ms_fp64 = timed(0, args.fp64_steps, 1) * 1e3
 
     # dominant kernel: mixed all-pairs accelerations; HIP events on its stream
+    far = W.far_expansion_used(pos, W.SNAPSHOT_D, nic)
     tot_ms, ker_ms = dd.time_accel("mixed", reps=5)
-    flops = W.force_flops(n, nic, "mixed")
+    flops = W.force_flops(n, nic, "mixed", far_expansion=far)
     kernel = "k_accel_sym32" if n >= W.SYM_MIN else "k_accel_planar32"
+    nic_pair = min(nic, 1) if far else nic
     achieved = flops / (ker_ms * 1e-3) / 1e12
+    # like for like: the same accelerations by the reference's algorithm in
+    # FP64 and in mixed precision, then the pair-symmetric form, then the far
+    # image partners as an expansion (the default)
+    forms = {}
+    for name, opts in (("mixed_gather_pairwise", dict(no_symmetric=True, no_far_expansion=True)),
+                       ("mixed_symmetric_pairwise", dict(no_far_expansion=True)),
+                       ("mixed_symmetric_far_expansion", {})):
+        dd.set_force_options(**opts)
+        forms[name] = dd.time_accel("mixed", reps=3)[0]
+    dd.set_force_options()
     ftot, fker = dd.time_field_grid(*box, G, G, 0.0, "mixed", reps=3)
     t64, k64 = dd.time_accel("fp64", reps=2)
     copy_ms = dd.time_copy(COPY_BYTES, reps=3)
@@ -341,6 +443,13 @@ def main():
                  "accel_kernel_ms": k64,
                  "achieved_tflops": W.force_flops(n, nic, "fp64") / (k64 * 1e-3) / 1e12},
         "mixed_over_fp64": ms_fp64 / ms_mixed,
+        "accel_forms_ms": dict(forms, fp64_gather_pairwise=t64),
+        "accel_speedup_factors": {
+            "precision (fp64 -> mixed, same algorithm)": t64 / forms["mixed_gather_pairwise"],
+            "pair-symmetric form": forms["mixed_gather_pairwise"] /
+                                   forms["mixed_symmetric_pairwise"],
+            "far partners as an FP64 expansion": forms["mixed_symmetric_pairwise"] /
+                                                 forms["mixed_symmetric_far_expansion"]},
         "kernels_ms": {"accel_mixed": ker_ms, "accel_mixed_total": tot_ms,
                        "field_grid_mixed": fker, "field_grid_mixed_total": ftot,
                        "accel_fp64": k64},
@@ -351,16 +460,18 @@ def main():
                      "traffic_unit": "bytes per launch (HBM read + write, counter passes)",
                      "kernel": kernel,
                      "flops_per_launch": flops,
-                     "flops_per_interaction": W.pair_flops(nic),
+                     "image_partners_in_kernel": 1 + 4 * nic_pair,
+                     "far_partners_as_expansion": bool(far),
+                     "flops_per_interaction": W.pair_flops(nic_pair),
                      "pairs_per_launch": n * (n - 1) // (2 if kernel == "k_accel_sym32" else 1)},
     }
     dd.close()
     if args.serve_summary > 0:
         out["serving"] = serving_summary(args.serve_summary, args.serve_summary_steps)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or default_cpu_threads()
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_sample, nic, threads)
+            out["cpu_baseline"] = cpu_baseline(args.cpu_sample or n, nic, threads)
         except Exception as e:  # reported, never substituted for the GPU number
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(out), flush=True)

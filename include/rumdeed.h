@@ -79,6 +79,11 @@ typedef struct {
   const double* wf_table; /* wf_ny rows of wf_nx values, first row = top, as in `work` */
   int32_t quad_n;         /* supply-integral points per side, planar (0 = 32) */
   int32_t ndim;           /* Metropolis-Hastings jumps per chain (0 = 200 FE, 25 GTF, 80 tip) */
+  int32_t quad_levels;    /* planar supply: doublings of quad_n until two levels agree within
+                             max(0.5 electrons, 1e-3 relative), the reference's Divonne
+                             tolerances; -1 none (the fixed rule), 0 up to 4, k in 1..8 up to
+                             k; never beyond 1024 points per side */
+  int32_t reserved;
 } rd_emitter;
 
 /* Per-step record of one simulation (the figures of emitted.dt, absorbed.dt,
@@ -86,7 +91,9 @@ typedef struct {
 typedef struct {
   int32_t step, n, emitted, n_round;
   int32_t absorbed, absorbed_top, absorbed_bot, dropped;
-  int32_t start_rounds, reserved0, reserved1, reserved2;
+  int32_t start_rounds;
+  int32_t supply_level;  /* planar supply: the level used (quad_n << level points per side) */
+  int32_t reserved1, reserved2;
   double n_sup, ramo_current, mh_std, a_rate, df_avg;
   double f_avg[3];
   double ramo[3];
@@ -269,6 +276,20 @@ int rd_set_jump_pairing(rd_ctx* ctx, int64_t min_workgroups);
 #define RD_FORCE_NO_SYMMETRIC 1
 #define RD_FORCE_NO_FAR_EXPANSION 2
 int rd_set_force_options(rd_ctx* ctx, int32_t flags);
+
+/* Bound on the partial sums of one pair launch (bytes; 0 = the default,
+ * 512 MiB).  The FP64 forces and fields keep one partial per target and
+ * source chunk of 256, the pair-symmetric mixed forces one per particle and
+ * block of 1024; a launch whose partials exceed the bound runs in bands whose
+ * partials are added, in the same order, to running sums, so the memory of
+ * the sums is O(N) and every result is the same for every bound.  Library
+ * control, no reference counterpart. */
+int rd_set_partial_budget(rd_ctx* ctx, int64_t bytes);
+
+/* Device bytes the context's work buffers hold (partial sums, packed copies,
+ * launch tables, chains; grown on demand, kept until rd_ctx_destroy).
+ * Library query, no reference counterpart. */
+int rd_work_bytes(rd_ctx* ctx, int64_t* bytes);
 
 /* Collect the particle records of each step (on != 0) or not (0, the
  * default).  Library control, no reference counterpart: the reference

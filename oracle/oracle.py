@@ -360,7 +360,7 @@ class Record(ctypes.Structure):
 
     _fields_ = [("step", c_int32), ("n", c_int32), ("emitted", c_int32), ("n_round", c_int32),
                 ("absorbed", c_int32), ("absorbed_top", c_int32), ("absorbed_bot", c_int32),
-                ("dropped", c_int32), ("start_rounds", c_int32), ("r0", c_int32),
+                ("dropped", c_int32), ("start_rounds", c_int32), ("supply_level", c_int32),
                 ("r1", c_int32), ("r2", c_int32),
                 ("n_sup", c_double), ("ramo_current", c_double), ("mh_std", c_double),
                 ("a_rate", c_double), ("df_avg", c_double), ("f_avg", c_double * 3),
@@ -402,6 +402,8 @@ class Sim:
         L.rdo_sim_set_emitter.argtypes = [ctypes.c_void_p, c_int, c_int, c_double, c_double,
                                           c_double, c_double, c_double, c_double, c_int, c_int,
                                           _dp, c_int, c_int]
+        L.rdo_sim_set_quad_levels.restype = None
+        L.rdo_sim_set_quad_levels.argtypes = [ctypes.c_void_p, c_int]
         L.rdo_sim_step.argtypes = [ctypes.c_void_p, c_int, c_int, ctypes.c_void_p]
         L.rdo_sim_count.argtypes = [ctypes.c_void_p]
         self.L = L
@@ -435,13 +437,16 @@ class Sim:
         self.L.rdo_sim_set_serial(ctypes.c_void_p(self.h), int(bool(on)))
 
     def set_emitter(self, mode, delay=0, x0=0.0, y0=0.0, lx=0.0, ly=0.0, T=300.0, w_tip=4.7,
-                    wf=None, quad=0, ndim=0):
+                    wf=None, quad=0, ndim=0, quad_levels=0):
+        """quad_levels: refinement doublings of the planar supply rule (-1
+        none, 0 up to four, k up to k; never beyond 1024 points per side)."""
         wf = np.array([[4.7]]) if wf is None else np.atleast_2d(np.asarray(wf, dtype=np.float64))
         ny, nx = wf.shape
         tab = np.ascontiguousarray(wf.ravel())
         self._wf = tab
         self.L.rdo_sim_set_emitter(ctypes.c_void_p(self.h), mode, delay, x0, y0, lx, ly, T, w_tip,
                                    nx, ny, _d(tab), quad, ndim)
+        self.L.rdo_sim_set_quad_levels(ctypes.c_void_p(self.h), c_int(quad_levels))
 
     def put(self, pos, q, m, vel=None, species=None):
         pos = _f64(pos).reshape(-1, 3)

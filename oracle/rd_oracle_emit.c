@@ -23,9 +23,12 @@
  * fixes the same choice as the device engine:
  *   - draws come from Philox4x64-10 with key (seed, stream) and counter
  *     (step, purpose<<40 | sub, chain, block);
- *   - the supply integral is the midpoint rule on quad x quad points of the
- *     emitter (100 x 100 (xi, phi) cells on the tip, as the reference),
- *     summed in tiles of 1024 points, pairwise within a tile, tiles in order;
+ *   - the planar supply integral (Divonne in the reference) is the midpoint
+ *     rule on quad x quad points of the emitter, refined by doubling the
+ *     points per side until two levels agree within the reference's Divonne
+ *     tolerances; the tip keeps the reference's 100 x 100 (xi, phi) cells;
+ *     each level is summed in tiles of 1024 points, pairwise within a tile,
+ *     tiles in order;
  *   - all chains of a step see the particles of the start of the step; the
  *     planar FE width is updated per jump from the batch acceptance rate, the
  *     thermal-field and tip widths per chain in chain order after the step.
@@ -351,6 +354,8 @@ typedef struct {
   int wf_nx, wf_ny;
   double wf[96 * 96];
   int quad, ndim;
+  int qgx, qgy;  /* level-0 points per side of the planar supply (x, y) */
+  int quad_lmax; /* refinement doublings of the planar supply integral */
   uint64_t seed, stream;
   int cap, n, nrID;
   double *pos, *prev, *vel, *acc, *a1, *a2, *q, *m;
@@ -358,6 +363,7 @@ typedef struct {
   double residual, MH_std, a_rate, N_sup, df_avg, F_avg[3], ramo[3], vsum[9];
   int emitted, n_round, rem_total, rem_top, rem_bot, rem_elec, rem_ion, rem_top_emit, dropped;
   int start_rounds, n_elec;
+  int supply_level;
   int life[2][NLIFE];
   int nthreads;
   int serial; /* 1: the reference's serial samplers (emit_serial) */
@@ -393,12 +399,15 @@ typedef struct rdo_plane_rec {
 typedef struct {
   int32_t step, n, emitted, n_round;
   int32_t absorbed, absorbed_top, absorbed_bot, dropped;
-  int32_t start_rounds, r0, r1, r2;
+  int32_t start_rounds, supply_level, r1, r2;
   double n_sup, ramo_current, mh_std, a_rate, df_avg;
   double f_avg[3];
   double ramo[3];
   double avg_elec_vel[3];
 } rdo_record;
+
+int rdo_quad_lmax(int g0, int levels);
+int rdo_quad_side(int quad, int sections);
 
 rdo_sim* rdo_sim_create(int cap, int nthreads) {
   rdo_sim* s = (rdo_sim*)calloc(1, sizeof(rdo_sim));
@@ -425,6 +434,8 @@ rdo_sim* rdo_sim_create(int cap, int nthreads) {
   s->MH_std = 0.0125;
   s->a_rate = 1.0;
   s->quad = 32;
+  s->qgx = s->qgy = 32;
+  s->quad_lmax = rdo_quad_lmax(32, 0);
   s->ndim = 200;
   s->w_tip = 4.7;
   return s;
@@ -466,12 +477,34 @@ void rdo_sim_set_emitter(rdo_sim* s, int mode, int delay, double x0, double y0, 
   s->wf_ny = ny;
   if (table && nx > 0 && ny > 0) memcpy(s->wf, table, sizeof(double) * nx * ny);
   s->quad = quad > 0 ? quad : 32;
+  s->qgx = rdo_quad_side(s->quad, nx);
+  s->qgy = rdo_quad_side(s->quad, ny);
+  s->quad_lmax = rdo_quad_lmax(s->qgx > s->qgy ? s->qgx : s->qgy, 0);
   int ndef = mode == EM_GTF ? 25 : (mode == EM_TIP ? 80 : 200);
   s->ndim = ndim > 0 ? ndim : ndef;
   if (mode == EM_GTF) { s->MH_std = 0.125; s->a_rate = 1.0; }
   else if (mode == EM_TIP) { s->MH_std = 1.0; s->a_rate = 0.5; }
   else { s->MH_std = 0.0125; s->a_rate = 1.0; }
   s->residual = 0.0;
+}
+
+/* Level-0 points per side of the planar supply along an axis the work
+ * function splits into `sections` equal parts: the least multiple of
+ * `sections` not below quad, so that every cell of the grid (and of each
+ * refinement) lies in one section and the integrand is smooth on it. */
+int rdo_quad_side(int quad, int sections) {
+  if (sections < 1) sections = 1;
+  return sections * ((quad + sections - 1) / sections);
+}
+/* Doublings of the planar supply rule from g0 points on the longer side:
+ * levels < 0 none, 0 four, else `levels`; never beyond 1024 points per side. */
+int rdo_quad_lmax(int g0, int levels) {
+  int l = levels < 0 ? 0 : (levels == 0 ? 4 : levels);
+  while (l > 0 && (g0 << l) > 1024) --l;
+  return l;
+}
+void rdo_sim_set_quad_levels(rdo_sim* s, int levels) {
+  s->quad_lmax = rdo_quad_lmax(s->qgx > s->qgy ? s->qgx : s->qgy, levels);
 }
 
 static void vacuum(const rdo_sim* s, const double* p, double* e) { rdo_vacuum_field(&s->g, p, e); }
@@ -569,12 +602,16 @@ static void add_particle(rdo_sim* s, const double* p, const double* v, int spc, 
                          int emit, int sec);
 static void emit_serial(rdo_sim* s, int step, int nr);
 
-static void do_emission(rdo_sim* s, int step) {
-  if (s->mode == EM_NONE || !(s->delay < step)) return;
-  const int tip = s->mode == EM_TIP;
-  /* supply integral */
-  int M = tip ? 100 * 100 : s->quad * s->quad;
-  double* pts = malloc(sizeof(double) * 3 * M);
+/* One level of the supply integral: the integrand at the Gx x Gy cell
+ * centres of the planar emitter, x fastest (Gx = 0: the tip's 100 x 100
+ * (xi, phi) cells) and its sums -- value, f_x, f_y, f_z (tip: the normal
+ * field) -- in the device's order: tiles of 1024 points, a pairwise tree
+ * within a tile (zeros beyond the last point), tile sums added in tile
+ * order. */
+static void supply_level(rdo_sim* s, int Gx, int Gy, double tot[4]) {
+  const int tip = Gx == 0;
+  int M = tip ? 100 * 100 : Gx * Gy;
+  double* pts = calloc((size_t)3 * M, sizeof(double));
   double* fld = malloc(sizeof(double) * 3 * M);
   for (int k = 0; k < M; ++k) {
     if (tip) {
@@ -584,9 +621,8 @@ static void do_emission(rdo_sim* s, int step) {
       tip_xyz(&s->g, 1.0 + (i - 0.5) * len_xi, s->g.eta_1, (j - 0.5) * len_phi, &pts[3 * k],
               &pts[3 * k + 1], &pts[3 * k + 2]);
     } else {
-      const int G = s->quad;
-      int ix = k % G, iy = k / G;
-      double u = (ix + 0.5) / G, v = (iy + 0.5) / G;
+      int ix = k % Gx, iy = k / Gx;
+      double u = (ix + 0.5) / Gx, v = (iy + 0.5) / Gy;
       pts[3 * k] = s->ex0 + u * s->elx;
       pts[3 * k + 1] = s->ey0 + v * s->ely;
       pts[3 * k + 2] = 0.0;
@@ -630,7 +666,7 @@ static void do_emission(rdo_sim* s, int step) {
   }
   /* the device's order: tiles of 1024 points, a pairwise tree within a tile
      (zeros beyond the last point), tile sums added in tile order */
-  double tot[4] = {0.0, 0.0, 0.0, 0.0};
+  tot[0] = tot[1] = tot[2] = tot[3] = 0.0;
   for (int b = 0; b < M; b += 1024) {
     double t[4][1024];
     for (int q = 0; q < 4; ++q)
@@ -641,7 +677,55 @@ static void do_emission(rdo_sim* s, int step) {
     for (int q = 0; q < 4; ++q) tot[q] = tot[q] + t[q][0];
   }
   free(rv);
+  free(pts);
+  free(fld);
+}
+
+/* The planar supply: the midpoint rule on qgx x qgy points (cells within the
+ * work function's sections), then on twice as many per side, and so on up to
+ * quad_lmax doublings, until two
+ * successive levels agree within the reference's Divonne tolerances
+ * (src/mod_global.F90:89-90, the test of src/mod_cuba_integration.F90:87:
+ * |difference| <= max(0.5 electrons, 1e-3 |integral|)).  The last
+ * level is the result; *M receives its number of points. */
+static void supply_planar(rdo_sim* s, double tot[4], int* Mout) {
+  s->supply_level = 0;
+  int Gx = s->qgx, Gy = s->qgy;
+  supply_level(s, Gx, Gy, tot);
+  double prev = tot[0] / ((double)Gx * (double)Gy);
+  for (int L = 1; L <= s->quad_lmax; ++L) {
+    const int G2x = s->qgx << L, G2y = s->qgy << L;
+    double t2[4];
+    supply_level(s, G2x, G2y, t2);
+    const double cur = t2[0] / ((double)G2x * (double)G2y);
+    double tol = 1.0e-3 * fabs(cur);
+    if (tol < 0.5) tol = 0.5;
+    const int conv = fabs(cur - prev) <= tol;
+    for (int k = 0; k < 4; ++k) tot[k] = t2[k];
+    Gx = G2x;
+    Gy = G2y;
+    prev = cur;
+    s->supply_level = L;
+    if (conv) break;
+  }
+  *Mout = Gx * Gy;
+}
+
+static void do_emission(rdo_sim* s, int step) {
+  if (s->mode == EM_NONE || !(s->delay < step)) return;
+  const int tip = s->mode == EM_TIP;
+  /* supply integral: the tip's 100 x 100 cells; the planar emitter's
+     midpoint rule refined (supply_planar) */
+  double tot[4];
+  int M;
+  if (tip) {
+    M = 100 * 100;
+    supply_level(s, 0, 0, tot);
+  } else {
+    supply_planar(s, tot, &M);
+  }
   const double sum = tot[0], f0 = tot[1], f1 = tot[2], f2 = tot[3];
+  const int ic = s->g.image_charge;
   int nr;
   if (tip) {
     s->N_sup = sum;
@@ -649,7 +733,7 @@ static void do_emission(rdo_sim* s, int step) {
     if (nr < 0) nr = 0;
     s->F_avg[0] = 0.0; s->F_avg[1] = 0.0; s->F_avg[2] = f2 / M;
   } else {
-    double nsup = sum / ((double)s->quad * (double)s->quad);
+    double nsup = sum / (double)M;
     s->N_sup = nsup;
     if (s->mode == EM_FE) {
       nr = fnint(nsup + s->residual);
@@ -660,8 +744,6 @@ static void do_emission(rdo_sim* s, int step) {
     }
     s->F_avg[0] = f0 / M; s->F_avg[1] = f1 / M; s->F_avg[2] = f2 / M;
   }
-  free(pts);
-  free(fld);
   s->n_round = nr;
   if (nr <= 0) return;
   if (s->serial) {
@@ -1257,6 +1339,7 @@ static void reset_counters(rdo_sim* s) {
   s->n_round = 0;
   s->rem_total = s->rem_top = s->rem_bot = s->rem_elec = s->rem_ion = s->rem_top_emit = 0;
   s->start_rounds = 0;
+  s->supply_level = 0;
   s->N_sup = 0.0;
   s->df_avg = 0.0;
   s->F_avg[0] = s->F_avg[1] = s->F_avg[2] = 0.0;
@@ -1288,6 +1371,7 @@ void rdo_sim_step(rdo_sim* s, int step, int phases, rdo_record* rec) {
     rec->absorbed_bot = s->rem_bot;
     rec->dropped = s->dropped;
     rec->start_rounds = s->start_rounds;
+    rec->supply_level = s->supply_level;
     rec->n_sup = s->N_sup;
     rec->ramo_current = s->ramo[0] + s->ramo[1] + s->ramo[2];
     rec->mh_std = s->MH_std;

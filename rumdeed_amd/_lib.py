@@ -55,7 +55,8 @@ class Emitter(ctypes.Structure):
     _fields_ = [("mode", c_int32), ("delay", c_int32), ("x0", c_double), ("y0", c_double),
                 ("lx", c_double), ("ly", c_double), ("T", c_double), ("w_tip", c_double),
                 ("wf_nx", c_int32), ("wf_ny", c_int32), ("wf_table", _dp),
-                ("quad_n", c_int32), ("ndim", c_int32)]
+                ("quad_n", c_int32), ("ndim", c_int32), ("quad_levels", c_int32),
+                ("reserved", c_int32)]
 
 
 class StepRecord(ctypes.Structure):
@@ -63,7 +64,7 @@ class StepRecord(ctypes.Structure):
 
     _fields_ = [("step", c_int32), ("n", c_int32), ("emitted", c_int32), ("n_round", c_int32),
                 ("absorbed", c_int32), ("absorbed_top", c_int32), ("absorbed_bot", c_int32),
-                ("dropped", c_int32), ("start_rounds", c_int32), ("reserved0", c_int32),
+                ("dropped", c_int32), ("start_rounds", c_int32), ("supply_level", c_int32),
                 ("reserved1", c_int32), ("reserved2", c_int32),
                 ("n_sup", c_double), ("ramo_current", c_double), ("mh_std", c_double),
                 ("a_rate", c_double), ("df_avg", c_double), ("f_avg", c_double * 3),
@@ -108,6 +109,8 @@ _SIGS = {
     "rd_field_batch": (c_int32, [c_void_p, c_int32, _dp, _dp, c_int32]),
     "rd_field_batch_static": (c_int32, [c_void_p, c_int32, _dp, _dp, c_int32]),
     "rd_set_force_options": (c_int32, [c_void_p, c_int32]),
+    "rd_set_partial_budget": (c_int32, [c_void_p, c_int64]),
+    "rd_work_bytes": (c_int32, [c_void_p, POINTER(c_int64)]),
     "rd_field_grid": (c_int32, [c_void_p, c_double, c_double, c_double, c_double, c_int32,
                                 c_int32, c_double, c_int32, _dp]),
     "rd_set_emitter": (c_int32, [c_void_p, POINTER(Emitter)]),

@@ -40,6 +40,7 @@ int ensure(rd_ctx* c, Buf& b, size_t bytes) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     hipFree(b.p);
+    c->work_bytes -= b.bytes;
     b.p = nullptr;
     b.bytes = 0;
   }
@@ -47,6 +48,7 @@ int ensure(rd_ctx* c, Buf& b, size_t bytes) {
   hipError_t e = hipMalloc(&b.p, want);
   if (e != hipSuccess) return hip_fail(e, "hipMalloc(work buffer)");
   b.bytes = want;
+  c->work_bytes += want;
   return RD_OK;
 }
 
@@ -219,6 +221,8 @@ static int open_slot(rd_ctx* c, int64_t cap, int32_t* out) {
   sl.desc.base = base;
   sl.desc.cap = (int)cap;
   sl.desc.quad = 32;
+  sl.desc.qgx = sl.desc.qgy = 32;
+  sl.desc.quad_lmax = quad_lmax(32, 0);
   sl.desc.ndim = 200;
   sl.desc.w_tip = 4.7;
   sl.desc.wf_nx = 0;
@@ -496,6 +500,8 @@ int rd_set_emitter(rd_ctx* c, const rd_emitter* em) {
     return fail(RD_ERR_ARG, "tip work function must be > 0");
   if (em->quad_n < 0 || em->quad_n > 1024) return fail(RD_ERR_ARG, "quad_n outside 0..1024");
   if (em->ndim < 0 || em->ndim > 100000) return fail(RD_ERR_ARG, "ndim outside 0..100000");
+  if (em->quad_levels < -1 || em->quad_levels > 8)
+    return fail(RD_ERR_ARG, "quad_levels outside -1..8");
   d.mode = mode;
   d.delay = em->delay;
   d.ex0 = em->x0;
@@ -505,6 +511,11 @@ int rd_set_emitter(rd_ctx* c, const rd_emitter* em) {
   d.T = em->T;
   d.w_tip = em->w_tip;
   d.quad = em->quad_n > 0 ? em->quad_n : 32;
+  d.qgx = quad_side(d.quad, d.wf_nx);
+  d.qgy = quad_side(d.quad, d.wf_ny);
+  if (d.qgx > 2048 || d.qgy > 2048)
+    return fail(RD_ERR_ARG, "supply grid beyond 2048 points per side (quad_n and the work-function sections)");
+  d.quad_lmax = quad_lmax(std::max(d.qgx, d.qgy), em->quad_levels);
   int ndef = mode == RD_EMIT_PLANAR_GTF ? 25 : (mode == RD_EMIT_TIP_FE ? 80 : 200);
   d.ndim = em->ndim > 0 ? em->ndim : ndef;
   int r = sync_state(c);
@@ -1048,6 +1059,19 @@ int rd_set_force_options(rd_ctx* c, int32_t flags) {
   if (flags & ~(RD_FORCE_NO_SYMMETRIC | RD_FORCE_NO_FAR_EXPANSION))
     return fail(RD_ERR_ARG, "unknown force option");
   c->force_opts = flags;
+  return RD_OK;
+}
+
+int rd_work_bytes(rd_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return fail(RD_ERR_ARG, "null pointer");
+  *bytes = (int64_t)c->work_bytes;
+  return RD_OK;
+}
+
+int rd_set_partial_budget(rd_ctx* c, int64_t bytes) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  if (bytes < 0) return fail(RD_ERR_ARG, "negative partial-sum budget");
+  c->part_budget = bytes == 0 ? kPartBudget : (size_t)bytes;
   return RD_OK;
 }
 

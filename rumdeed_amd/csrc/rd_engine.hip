@@ -137,12 +137,15 @@ __global__ void k_reset_counters(const Ent* tab, SimState* st) {
   s.rem_total = s.rem_top = s.rem_bot = s.rem_elec = s.rem_ion = s.rem_top_emit = 0;
   s.pending = 0;
   s.start_rounds = 0;
+  s.q_level = 0;
   s.N_sup = 0.0;
   s.df_avg = 0.0;
   for (int k = 0; k < 3; ++k) s.F_avg[k] = 0.0;
 }
 
-// Quadrature points of the supply integral.
+// Quadrature points of the supply integral: the tip's 100 x 100 (xi, phi)
+// cells; the planar emitter's cell centres at level e.lev (qgx << lev by
+// qgy << lev, x fastest).
 __global__ void k_quad_points(const Ent* tab, const SimDesc* desc, double* __restrict__ ptx,
                               double* __restrict__ pty, double* __restrict__ ptz) {
   const Ent e = tab[blockIdx.z];
@@ -159,9 +162,9 @@ __global__ void k_quad_points(const Ent* tab, const SimDesc* desc, double* __res
     double phi_c = (j - 0.5) * len_phi;
     tip_xyz(tip_surf(s.g), xi_c, s.g.eta_1, phi_c, x, y, z);
   } else {
-    const int G = s.quad;
-    int ix = k % G, iy = k / G;
-    double u = (ix + 0.5) / G, v = (iy + 0.5) / G;
+    const int Gx = s.qgx << e.lev, Gy = s.qgy << e.lev;
+    int ix = k % Gx, iy = k / Gx;
+    double u = (ix + 0.5) / Gx, v = (iy + 0.5) / Gy;
     x = s.ex0 + u * s.elx;
     y = s.ey0 + v * s.ely;
     z = 0.0;
@@ -171,18 +174,20 @@ __global__ void k_quad_points(const Ent* tab, const SimDesc* desc, double* __res
   ptz[e.pbase + k] = z;
 }
 
-// Integrand per point, then the point-ordered sum and the number of chains.
-__global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimDesc* desc,
-                                                     SimState* st, const double* wf,
-                                                     const double* __restrict__ ptx,
-                                                     const double* __restrict__ pty,
-                                                     const double* __restrict__ ptz,
-                                                     const double* __restrict__ fx,
-                                                     const double* __restrict__ fy,
-                                                     const double* __restrict__ fz,
-                                                     double* __restrict__ qv,
-                                                     double* __restrict__ qn) {
-  // LDS: four 1024-entry reduction rows, reused as the Poisson uniforms
+// Integrand per point and the sums of one level (value, f_x, f_y, f_z; the
+// tip: value, 0, 0, normal field) into S.qsum[lev & 1]: tiles of 1024
+// points, a fixed pairwise tree within a tile (zeros beyond the last point),
+// tile sums added in tile order.
+__global__ __launch_bounds__(1024) void k_quad_sum(const Ent* tab, const SimDesc* desc,
+                                                  SimState* st, const double* wf,
+                                                  const double* __restrict__ ptx,
+                                                  const double* __restrict__ pty,
+                                                  const double* __restrict__ ptz,
+                                                  const double* __restrict__ fx,
+                                                  const double* __restrict__ fy,
+                                                  const double* __restrict__ fz,
+                                                  double* __restrict__ qv,
+                                                  double* __restrict__ qn) {
   constexpr int kTile = 1024;
   __shared__ double lds[4 * kTile];
   double *t0 = lds, *t1 = lds + kTile, *t2 = lds + 2 * kTile, *t3 = lds + 3 * kTile;
@@ -224,8 +229,6 @@ __global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimD
     }
     qv[p] = val;
   }
-  // The sums over the points: tiles of 1024 points, a fixed pairwise tree
-  // within a tile (zeros beyond the last point), tile sums added in order.
   double sum = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
   for (int b = 0; b < e.m; b += kTile) {
     const int k = threadIdx.x;  // blockDim.x == kTile
@@ -262,17 +265,57 @@ __global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimD
       f2 = f2 + t3[0];
     }
   }
+  if (threadIdx.x == 0) {
+    double* q = st[e.slot].qsum[e.lev & 1];
+    q[0] = sum;
+    q[1] = f0;
+    q[2] = f1;
+    q[3] = f2;
+  }
+}
+
+// The supply's level e.lev is final when it is the tip's rule, the last level
+// allowed, or within max(0.5, 1e-3 |I|) of the level before (the reference's
+// Divonne tolerances, src/mod_global.F90:89-90 with the test of
+// src/mod_cuba_integration.F90:87); I = sum / (Gx Gy) electrons.  A final level
+// gives N_sup, the number of chains and F_avg; otherwise S.q_final = 0 and the
+// host computes the next level.
+__global__ __launch_bounds__(1024) void k_quad_close(const Ent* tab, const SimDesc* desc,
+                                                    SimState* st) {
+  constexpr int kTile = 1024;
+  __shared__ double lds[4 * kTile];
+  __shared__ int s_final;
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
   SimState& S = st[e.slot];
-  __shared__ double s_lambda;
-  if (threadIdx.x == 0) s_lambda = sum / ((double)s.quad * (double)s.quad);
+  const bool tip = s.mode == RD_EMIT_TIP_FE;
+  const int Gx = s.qgx << e.lev, Gy = s.qgy << e.lev;
+  const double* cur = S.qsum[e.lev & 1];
+  const double sum = cur[0];
+  const double lambda = tip ? sum : sum / ((double)Gx * (double)Gy);
+  if (threadIdx.x == 0) {
+    int fin = 1;
+    if (!tip && e.lev > 0 && e.lev < s.quad_lmax) {
+      const int Gpx = s.qgx << (e.lev - 1), Gpy = s.qgy << (e.lev - 1);
+      const double prev = S.qsum[(e.lev - 1) & 1][0] / ((double)Gpx * (double)Gpy);
+      double tol = 1.0e-3 * fabs(lambda);
+      if (tol < 0.5) tol = 0.5;
+      fin = fabs(lambda - prev) <= tol;
+    }
+    S.q_final = fin;
+    S.q_level = e.lev;
+    s_final = fin;
+  }
+  __syncthreads();
+  if (!s_final) return;
+  const double f0 = cur[1], f1 = cur[2], f2 = cur[3];
+  const double M = tip ? 100.0 * 100.0 : (double)Gx * (double)Gy;
   if (s.mode == RD_EMIT_PLANAR_GTF) {
     // Rand_Poisson: the product of uniforms runs in one lane; the uniforms,
     // in stream order, are drawn kPois at a time by the whole block.
     constexpr int kPois = 4 * kTile;
     double* ut = lds;
     __shared__ int s_more;
-    __syncthreads();
-    const double lambda = s_lambda;
     double lambda_left = lambda, pp = 1.0;
     int kk = 0;
     for (uint64_t b0 = 0;; b0 += kPois) {
@@ -306,35 +349,29 @@ __global__ __launch_bounds__(1024) void k_quad_reduce(const Ent* tab, const SimD
     if (threadIdx.x == 0) {
       S.N_sup = lambda;
       S.n_round = kk - 1;
-      S.F_avg[0] = f0 / e.m;
-      S.F_avg[1] = f1 / e.m;
-      S.F_avg[2] = f2 / e.m;
+      S.F_avg[0] = f0 / M;
+      S.F_avg[1] = f1 / M;
+      S.F_avg[2] = f2 / M;
     }
     return;
   }
   if (threadIdx.x != 0) return;
   int nr;
-  if (s.mode == RD_EMIT_TIP_FE) {
+  if (tip) {
     S.N_sup = sum;
     nr = nint(sum);
     if (nr < 0) nr = 0;
     S.F_avg[0] = 0.0;
     S.F_avg[1] = 0.0;
-    S.F_avg[2] = f2 / e.m;
+    S.F_avg[2] = f2 / M;
   } else {
-    double G2 = (double)s.quad * (double)s.quad;
-    double nsup = sum / G2;
-    S.N_sup = nsup;
-    if (s.mode == RD_EMIT_PLANAR_FE) {
-      nr = nint(nsup + S.residual);
-      S.residual = nsup - nr;
-      if (nr < 0) nr = 0;
-    } else {
-      nr = 0;  // thermal-field: drawn above
-    }
-    S.F_avg[0] = f0 / e.m;
-    S.F_avg[1] = f1 / e.m;
-    S.F_avg[2] = f2 / e.m;
+    S.N_sup = lambda;
+    nr = nint(lambda + S.residual);
+    S.residual = lambda - nr;
+    if (nr < 0) nr = 0;
+    S.F_avg[0] = f0 / M;
+    S.F_avg[1] = f1 / M;
+    S.F_avg[2] = f2 / M;
   }
   S.n_round = nr;
 }
@@ -2249,48 +2286,75 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
 static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision,
                            std::vector<int>& n_round) {
   n_round.assign(all.size(), 0);
-  std::vector<Ent> q;
+  // Supply integral.  The first batch holds level 0 of every emitting entry
+  // and, where refinement is allowed, level 1; each later batch holds the next
+  // level of the entries whose last two levels disagree.
+  std::vector<Ent> q, top;
   std::vector<int> qidx;
-  int mtot = 0;
   for (size_t k = 0; k < all.size(); ++k) {
     const Ent& e0 = all[k];
     const SimDesc& d = c->slots[e0.slot].desc;
     if (d.mode == RD_EMIT_NONE || !(d.delay < e0.step)) continue;
     Ent e = e0;
-    e.pbase = mtot;
-    e.m = (d.mode == RD_EMIT_TIP_FE) ? 100 * 100 : d.quad * d.quad;
-    mtot += e.m;
+    e.lev = 0;
     q.push_back(e);
+    if (d.mode != RD_EMIT_TIP_FE && d.quad_lmax >= 1) {
+      e.lev = 1;
+      q.push_back(e);
+    }
+    top.push_back(e);
     qidx.push_back((int)k);
   }
   if (q.empty()) return RD_OK;
-  int r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)mtot);
-  if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)mtot);
-  if (!r) r = ensure(c, c->stage, sizeof(double) * 2 * (size_t)mtot);
-  if (r) return r;
-  double* ptx = (double*)c->pts.p;
-  Ent* dq;
-  r = upload_tab(c, q, &dq);
-  if (r) return r;
-  int maxm = 0;
-  for (const Ent& e : q) maxm = std::max(maxm, e.m);
-  k_quad_points<<<dim3((maxm + 255) / 256, 1, q.size()), 256, 0, c->stream>>>(
-      dq, c->d_desc, ptx, ptx + mtot, ptx + 2 * mtot);
-  r = launch_field(c, q, mtot, precision, nullptr, nullptr);
-  if (r) return r;
-  double* f = (double*)c->fld.p;
-  k_quad_reduce<<<q.size(), 1024, 0, c->stream>>>(dq, c->d_desc, c->d_state, c->d_wf, ptx,
-                                                  ptx + mtot, ptx + 2 * mtot, f, f + mtot,
-                                                  f + 2 * mtot, (double*)c->stage.p,
-                                                  (double*)c->stage.p + mtot);
-  RD_HIP(hipGetLastError());
-  r = sync_state(c);
-  if (r) return r;
+  int r;
+  while (!q.empty()) {
+    int mtot = 0, maxm = 0;
+    for (Ent& e : q) {
+      const SimDesc& d = c->slots[e.slot].desc;
+      e.pbase = mtot;
+      e.m = (d.mode == RD_EMIT_TIP_FE) ? 100 * 100 : (d.qgx << e.lev) * (d.qgy << e.lev);
+      mtot += e.m;
+      maxm = std::max(maxm, e.m);
+    }
+    r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)mtot);
+    if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)mtot);
+    if (!r) r = ensure(c, c->stage, sizeof(double) * 2 * (size_t)mtot);
+    if (r) return r;
+    double* ptx = (double*)c->pts.p;
+    Ent* dq;
+    r = upload_tab(c, q, &dq);
+    if (r) return r;
+    k_quad_points<<<dim3((maxm + 255) / 256, 1, q.size()), 256, 0, c->stream>>>(
+        dq, c->d_desc, ptx, ptx + mtot, ptx + 2 * mtot);
+    r = launch_field(c, q, mtot, precision, nullptr, nullptr);
+    if (r) return r;
+    double* f = (double*)c->fld.p;
+    k_quad_sum<<<q.size(), 1024, 0, c->stream>>>(dq, c->d_desc, c->d_state, c->d_wf, ptx,
+                                                 ptx + mtot, ptx + 2 * mtot, f, f + mtot,
+                                                 f + 2 * mtot, (double*)c->stage.p,
+                                                 (double*)c->stage.p + mtot);
+    Ent* dtop;
+    r = upload_tab(c, top, &dtop);
+    if (r) return r;
+    k_quad_close<<<top.size(), 1024, 0, c->stream>>>(dtop, c->d_desc, c->d_state);
+    RD_HIP(hipGetLastError());
+    r = sync_state(c);
+    if (r) return r;
+    q.clear();
+    std::vector<Ent> next;
+    for (Ent e : top)
+      if (!c->h_state[e.slot].q_final) {
+        e.lev += 1;
+        q.push_back(e);
+        next.push_back(e);
+      }
+    top.swap(next);
+  }
   // chains
   std::vector<Ent> ch;
   int C = 0, maxc = 0, maxndim = 0;
-  for (size_t k = 0; k < q.size(); ++k) {
-    Ent e = q[k];
+  for (size_t k = 0; k < qidx.size(); ++k) {
+    Ent e = all[qidx[k]];
     int nr = c->h_state[e.slot].n_round;
     n_round[qidx[k]] = nr;
     if (nr <= 0) continue;
@@ -2308,8 +2372,8 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
   if (!r) r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)C);
   if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)C);
   if (r) return r;
-  ptx = (double*)c->pts.p;
-  f = (double*)c->fld.p;
+  double* ptx = (double*)c->pts.p;
+  double* f = (double*)c->fld.p;
   double *pty = ptx + C, *ptz = ptx + 2 * (size_t)C;
   double *fx = f, *fy = f + C, *fz = f + 2 * (size_t)C;
   Chains V = chain_view(c, C);
@@ -2356,6 +2420,7 @@ static void fill_record(rd_ctx* c, int slot, int step, rd_step_record* rec) {
   rec->absorbed_bot = S.rem_bot;
   rec->dropped = S.dropped;
   rec->start_rounds = S.start_rounds;
+  rec->supply_level = S.q_level;
   rec->n_sup = S.N_sup;
   rec->ramo_current = S.ramo[0] + S.ramo[1] + S.ramo[2];
   rec->mh_std = S.MH_std;

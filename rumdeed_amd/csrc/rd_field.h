@@ -403,7 +403,7 @@ __device__ __forceinline__ void field_chunks64(const DevGeom& g, bool act, doubl
                                                const double* __restrict__ q, int n, int c0,
                                                int c1, double* sx, double* sy, double* sz,
                                                double* sq, double* __restrict__ P, size_t S,
-                                               int m, int p) {
+                                               int m, int p, int cfirst = 0) {
   const int t = threadIdx.x;
   double da = 0, xim = 0, yim = 0, zim = 0;
   if (KIND == RD_GEOM_TIP && act && g.ic) tip_image_point64(x1, y1, z1, g.z0, g.R, da, xim, yim, zim);
@@ -457,9 +457,9 @@ __device__ __forceinline__ void field_chunks64(const DevGeom& g, bool act, doubl
         fys = fys + ay;
         fzs = fzs + az;
       }
-      P[0 * S + (size_t)c * m + p] = fxs;
-      P[1 * S + (size_t)c * m + p] = fys;
-      P[2 * S + (size_t)c * m + p] = fzs;
+      P[0 * S + (size_t)(c - cfirst) * m + p] = fxs;
+      P[1 * S + (size_t)(c - cfirst) * m + p] = fys;
+      P[2 * S + (size_t)(c - cfirst) * m + p] = fzs;
     }
   }
 }

@@ -57,7 +57,10 @@ struct SimDesc {
   double T;                   // cathode temperature (GTF)
   double w_tip;               // tip work function
   int wf_nx, wf_ny, wf_off;   // checkerboard table in ctx->wf
-  int quad;                   // quadrature points per side (planar)
+  int quad;                   // requested supply points per side (planar)
+  int qgx, qgy;               // level-0 supply points per side (x, y): multiples
+                              // of the work function's sections (quad_side)
+  int quad_lmax;              // refinement doublings allowed (planar)
   int base, cap;
   int ndim;                   // Metropolis-Hastings jumps per chain
   int nplanes;                // recording planes (Check_Planes)
@@ -84,6 +87,9 @@ struct SimState {
   double ramo[3];   // Ramo current per species (A)
   double vsum[9];   // velocity sums: all, electrons, ions
   int plane_n[RD_MAX_PLANES];  // plane crossings of the last step
+  double qsum[2][4];  // supply sums (value, f_x, f_y, f_z) of levels of parity 0 / 1
+  int q_final;        // the supply's last computed level is its result
+  int q_level;        // that level
 };
 
 // One launch-table entry (per active slot and launch).
@@ -97,6 +103,18 @@ struct Ent {
   int wg0, wg1;   // first workgroup of the entry in a compact launch (two layouts)
   int nic_pair;   // image partners n <= nic_pair in the mixed pair kernels (the
                   // rest, when fewer than N_ic_max, come from rd_tail.hip)
+  int lev;        // refinement level of the supply points (quad << lev per side)
+  // Banded launches (partial sums bounded by rd_ctx::part_budget).  FP64: the
+  // launch takes source chunks [band0, band1), its partial slot k holding
+  // chunk band0 + k; pair-symmetric: the launch takes block rows
+  // [band0, band1), the reaction partials of row X in slot X - band0 at
+  // part_off, the direct partials of the band's particles at d_off.
+  int band0, band1;
+  int pslots;        // partial slots per component at part_off
+  int banded;        // the sums come from acc (k_fold64 / k_fold_sym), not one partial set
+  int d_ld;          // leading dimension of the direct partials (pair-symmetric)
+  long long acc_off; // running sums [3][targets] at rd_ctx::acc + acc_off
+  long long d_off;   // direct partials [3][segs][d_ld] (pair-symmetric, banded)
 };
 
 // Entry of a compact launch whose workgroups [wg0, next wg0) belong to one
@@ -153,6 +171,11 @@ struct Slot {
 // hold about two workgroups per CU each.
 constexpr int kJumpPairWgs = 2048;
 
+// Partial sums of one pair launch are bounded by this many bytes (512 MiB);
+// larger launches run in bands whose partials are folded, in order, into
+// running sums, with the results of the unbanded order.
+constexpr size_t kPartBudget = size_t(512) << 20;
+
 struct rd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -188,6 +211,11 @@ struct rd_ctx {
   // far image partners as an expansion (rd_tail.hip): the plan of the last
   // (d, N_ic_max), work arrays, lateral boxes; force options (rd_set_force_options)
   rd::Buf tail_plan, tail_work, tail_box;
+  // running sums of banded launches and the partial-sum bound of one band
+  // (rd_set_partial_budget)
+  rd::Buf acc;
+  size_t part_budget = kPartBudget;
+  size_t work_bytes = 0;  // device bytes held by the work buffers (ensure)
   double tail_d = -1.0;
   int tail_nic = -1, tail_nnz = 0;
   int force_opts = 0;
@@ -255,6 +283,22 @@ struct FixedBufs {
 int launch_field(rd_ctx* c, const std::vector<Ent>& ents, int mtot, int precision, hipEvent_t k0,
                  hipEvent_t k1, FixedBufs* fixed = nullptr);
 int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab);
+
+// Level-0 points per side of the planar supply along an axis the work
+// function splits into `sections` equal parts: the least multiple of
+// `sections` not below quad, so that every cell of the grid (and of each
+// refinement) lies in one section and the integrand is smooth on it.
+inline int quad_side(int quad, int sections) {
+  if (sections < 1) sections = 1;
+  return sections * ((quad + sections - 1) / sections);
+}
+// Doublings of the planar supply rule from g0 points on the longer side:
+// levels < 0 none, 0 four, else `levels`; never beyond 1024 points per side.
+inline int quad_lmax(int g0, int levels) {
+  int l = levels < 0 ? 0 : (levels == 0 ? 4 : levels);
+  while (l > 0 && (g0 << l) > 1024) --l;
+  return l;
+}
 // Packed mixed-precision copies (c->pack_hi / pack_lo, and c->pack_srf when
 // `surface`) of every entry's particles.
 int launch_pack(rd_ctx* c, const std::vector<Ent>& ents, bool surface = false);

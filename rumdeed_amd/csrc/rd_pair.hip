@@ -91,8 +91,8 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
   const Tile tl = tile_of(tab, ne, false);
   const int n = st[tl.e.slot].n;
   if (tl.bx * kBlock >= n) return;
-  const int nchunks = (n + kChunk - 1) / kChunk;
-  const int c0 = tl.by * tl.e.cps;
+  const int nchunks = min((n + kChunk - 1) / kChunk, tl.e.band1);
+  const int c0 = tl.e.band0 + tl.by * tl.e.cps;
   if (c0 >= nchunks) return;
   const int c1 = min(nchunks, c0 + tl.e.cps);
   const DevGeom g = desc[tl.e.slot].g;
@@ -188,17 +188,49 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
         py = py + ayy;
         pz = pz + azz;
       }
-      const size_t S = (size_t)tl.e.nchunks_ub * tl.e.n_ub;
-      P[0 * S + (size_t)c * tl.e.n_ub + i] = px;
-      P[1 * S + (size_t)c * tl.e.n_ub + i] = py;
-      P[2 * S + (size_t)c * tl.e.n_ub + i] = pz;
+      const size_t S = (size_t)tl.e.pslots * tl.e.n_ub;
+      const size_t o = (size_t)(c - tl.e.band0) * tl.e.n_ub + i;
+      P[0 * S + o] = px;
+      P[1 * S + o] = py;
+      P[2 * S + o] = pz;
     }
   }
 }
 
-// Sum the chunk partials in order and form a = (F + q E_vac)/m.
+// Running sums of a banded FP64 launch: acc = acc + the band's chunk
+// partials, in chunk order (the first band starts from zero), so that the
+// bands together add every chunk in the order of one unbanded launch.
+// points: the targets are the entry's m points, else its particles.
+__global__ void k_fold64(const Ent* tab, const SimState* st, const double* __restrict__ part,
+                         double* __restrict__ acc, int points) {
+  const Ent e = tab[blockIdx.z];
+  if (!e.banded) return;
+  const int n = st[e.slot].n;
+  const int ld = points ? e.m : e.n_ub;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (points ? e.m : n)) return;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  const int cl = min(nchunks, e.band1);
+  double* A = acc + e.acc_off;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  if (e.band0 > 0) {
+    if (cl <= e.band0) return;
+    sx = A[p];
+    sy = A[(size_t)ld + p];
+    sz = A[2 * (size_t)ld + p];
+  }
+  if (cl > e.band0)
+    sum3_in_order(part + e.part_off + p, (size_t)e.pslots * ld, ld, cl - e.band0, sx, sy, sz);
+  A[p] = sx;
+  A[(size_t)ld + p] = sy;
+  A[2 * (size_t)ld + p] = sz;
+}
+
+// Sum the chunk partials in order (banded: the running sums) and form
+// a = (F + q E_vac)/m.
 __global__ void k_accel_finish64(const Ent* tab, const SimDesc* desc, const SimState* st,
-                                 const double* __restrict__ part, const double* __restrict__ X,
+                                 const double* __restrict__ part, const double* __restrict__ acc,
+                                 const double* __restrict__ X,
                                  const double* __restrict__ Y, const double* __restrict__ Z,
                                  const double* __restrict__ Q, const double* __restrict__ M,
                                  double* __restrict__ AX, double* __restrict__ AY,
@@ -209,10 +241,15 @@ __global__ void k_accel_finish64(const Ent* tab, const SimDesc* desc, const SimS
   if (i >= n) return;
   const DevGeom& g = desc[e.slot].g;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  const double* P = part + e.part_off;
-  const size_t S = (size_t)e.nchunks_ub * e.n_ub;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  sum3_in_order(P + i, S, e.n_ub, nchunks, sx, sy, sz);
+  if (e.banded) {
+    const double* A = acc + e.acc_off;
+    sx = A[i];
+    sy = A[(size_t)e.n_ub + i];
+    sz = A[2 * (size_t)e.n_ub + i];
+  } else {
+    sum3_in_order(part + e.part_off + i, (size_t)e.pslots * e.n_ub, e.n_ub, nchunks, sx, sy, sz);
+  }
   const int gi = e.base + i;
   double q1 = Q[gi];
   double im = 1.0 / M[gi];
@@ -249,8 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_field64(const Ent* tab, int ne, cons
   const int m = e.m;
   if (tl.bx * kBlock >= m) return;
   const int n = st[e.slot].n;
-  const int nchunks = (n + kChunk - 1) / kChunk;
-  const int c0 = tl.by * e.cps;
+  const int nchunks = min((n + kChunk - 1) / kChunk, e.band1);
+  const int c0 = e.band0 + tl.by * e.cps;
   if (c0 >= nchunks) return;
   const int c1 = min(nchunks, c0 + e.cps);
   const DevGeom g = desc[e.slot].g;
@@ -264,12 +301,14 @@ __global__ __launch_bounds__(kBlock) void k_field64(const Ent* tab, int ne, cons
     z1 = ptz[e.pbase + p];
   }
   field_chunks64<KIND>(g, act, x1, y1, z1, X + e.base, Y + e.base, Z + e.base, Q + e.base, n, c0,
-                       c1, sx, sy, sz, sq, part + e.part_off, (size_t)e.nchunks_ub * m, m, p);
+                       c1, sx, sy, sz, sq, part + e.part_off, (size_t)e.pslots * m, m, p,
+                       e.band0);
 }
 
 // field = E_vac + sum of chunk partials (in order).
 __global__ void k_field_finish64(const Ent* tab, const SimDesc* desc, const SimState* st,
-                                 const double* __restrict__ part, const double* __restrict__ ptx,
+                                 const double* __restrict__ part, const double* __restrict__ acc,
+                                 const double* __restrict__ ptx,
                                  const double* __restrict__ pty, const double* __restrict__ ptz,
                                  double* __restrict__ fx, double* __restrict__ fy,
                                  double* __restrict__ fz) {
@@ -279,7 +318,16 @@ __global__ void k_field_finish64(const Ent* tab, const SimDesc* desc, const SimS
   const int n = st[e.slot].n;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const int gp = e.pbase + p;
-  field_finish64(desc[e.slot].g, part + e.part_off, (size_t)e.nchunks_ub * e.m, e.m, p, nchunks,
+  if (e.banded) {
+    const double* A = acc + e.acc_off;
+    double ex, ey, ez;
+    vacuum64(desc[e.slot].g, ptx[gp], pty[gp], ptz[gp], ex, ey, ez);
+    fx[gp] = ex + A[p];
+    fy[gp] = ey + A[(size_t)e.m + p];
+    fz[gp] = ez + A[2 * (size_t)e.m + p];
+    return;
+  }
+  field_finish64(desc[e.slot].g, part + e.part_off, (size_t)e.pslots * e.m, e.m, p, nchunks,
                  ptx[gp], pty[gp], ptz[gp], fx[gp], fy[gp], fz[gp]);
 }
 // ------------------------------------------------------------------------
@@ -435,7 +483,11 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
 // Partials: for a particle of block X, slot Y holds its sum from the block
 // pair (X, Y) for Y > X, from (Y, X) for Y < X, and the diagonal for Y = X:
 // the [3][segs][n_ub] layout of the gather kernels with segs = blocks, so
-// k_accel_finish32 adds them in block order.
+// k_accel_finish32 adds them in block order.  A banded launch takes the rows
+// X in [band0, band1): slot X of the particles of later blocks (the
+// reaction sums) at part_off, [3][pslots][n_ub] from slot band0; the slots
+// Y >= X of the rows' own particles at d_off, [3][segs][d_ld] from particle
+// band0 * kSymBlock; k_fold_sym adds them to the running sums in slot order.
 constexpr int kSymWaves = 16;
 constexpr int kSymBlock = 64 * kSymWaves;
 constexpr int kSymMin = 16384;
@@ -462,7 +514,7 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
   const Ent e = tab[find_entry(tab, ne, blockIdx.x, true)];
   const int nb = e.segs;
   int local = blockIdx.x - e.wg1;
-  int X = 0;
+  int X = e.band0;
   while (local >= nb - X) {
     local -= nb - X;
     ++X;
@@ -499,8 +551,13 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
   jsum[1][t] = 0.0;
   jsum[2][t] = 0.0;
   __syncthreads();
-  double* P = part + e.part_off;
-  const size_t S = (size_t)e.segs * e.n_ub;
+  // direct sums of the row's particles: D[c][slot][i - i0]; reaction sums of
+  // the column's particles: R[c][X - band0][js]
+  double* D = part + (e.banded ? e.d_off : e.part_off);
+  const size_t DS = (size_t)e.segs * e.d_ld;
+  const int i0 = e.banded ? e.band0 * kSymBlock : 0;
+  double* R = part + e.part_off;
+  const size_t RS = (size_t)e.pslots * e.n_ub;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (X == Y) {
     const int cnt = min(kSymBlock, n - Y * kSymBlock);
@@ -530,9 +587,10 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
       fz += (double)cz;
     }
     if (i < n) {
-      P[0 * S + (size_t)X * e.n_ub + i] = fx;
-      P[1 * S + (size_t)X * e.n_ub + i] = fy;
-      P[2 * S + (size_t)X * e.n_ub + i] = fz;
+      const size_t o = (size_t)X * e.d_ld + (i - i0);
+      D[0 * DS + o] = fx;
+      D[1 * DS + o] = fy;
+      D[2 * DS + o] = fz;
     }
     return;
   }
@@ -576,19 +634,72 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
     __syncthreads();
   }
   if (i < n) {
-    P[0 * S + (size_t)Y * e.n_ub + i] = fx;
-    P[1 * S + (size_t)Y * e.n_ub + i] = fy;
-    P[2 * S + (size_t)Y * e.n_ub + i] = fz;
+    const size_t o = (size_t)Y * e.d_ld + (i - i0);
+    D[0 * DS + o] = fx;
+    D[1 * DS + o] = fy;
+    D[2 * DS + o] = fz;
   }
   if (js < n) {
-    P[0 * S + (size_t)X * e.n_ub + js] = jsum[0][t];
-    P[1 * S + (size_t)X * e.n_ub + js] = jsum[1][t];
-    P[2 * S + (size_t)X * e.n_ub + js] = jsum[2][t];
+    const size_t o = (size_t)(X - e.band0) * e.n_ub + js;
+    R[0 * RS + o] = jsum[0][t];
+    R[1 * RS + o] = jsum[1][t];
+    R[2 * RS + o] = jsum[2][t];
   }
 }
 
+// Running sums of a banded pair-symmetric launch over rows [band0, band1):
+// for a particle of a later block the slots band0..band1-1 (reaction sums),
+// for a particle of the band's blocks every slot from band0 on (reaction
+// sums below its own block, then its direct sums), added in slot order to
+// the sums of the earlier bands (the first band starts from zero).
+__global__ void k_fold_sym(const Ent* tab, const SimState* st, const double* __restrict__ part,
+                           double* __restrict__ acc) {
+  const Ent e = tab[blockIdx.z];
+  if (!e.banded) return;
+  const int n = st[e.slot].n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int B = i / kSymBlock;
+  const int x0 = e.band0, x1 = e.band1;
+  if (B < x0) return;
+  const int nbk = min(e.segs, (n + kSymBlock - 1) / kSymBlock);
+  const int last = B >= x1 ? x1 : nbk;
+  const double* R = part + e.part_off + i;
+  const size_t RS = (size_t)e.pslots * e.n_ub;
+  const double* D = part + e.d_off + (i - x0 * kSymBlock);
+  const size_t DS = (size_t)e.segs * e.d_ld;
+  double* A = acc + e.acc_off;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  if (x0 > 0) {
+    sx = A[i];
+    sy = A[(size_t)e.n_ub + i];
+    sz = A[2 * (size_t)e.n_ub + i];
+  }
+  for (int s = x0; s < last; ++s) {
+    double vx, vy, vz;
+    if (s < B) {
+      const size_t o = (size_t)(s - x0) * e.n_ub;
+      vx = R[o];
+      vy = R[RS + o];
+      vz = R[2 * RS + o];
+    } else {
+      const size_t o = (size_t)s * e.d_ld;
+      vx = D[o];
+      vy = D[DS + o];
+      vz = D[2 * DS + o];
+    }
+    sx = sx + vx;
+    sy = sy + vy;
+    sz = sz + vz;
+  }
+  A[i] = sx;
+  A[(size_t)e.n_ub + i] = sy;
+  A[2 * (size_t)e.n_ub + i] = sz;
+}
+
 __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimState* st,
-                                 const double* __restrict__ part, const double* __restrict__ X,
+                                 const double* __restrict__ part, const double* __restrict__ acc,
+                                 const double* __restrict__ X,
                                  const double* __restrict__ Y, const double* __restrict__ Z,
                                  const double* __restrict__ Q, const double* __restrict__ M,
                                  double* __restrict__ AX, double* __restrict__ AY,
@@ -600,10 +711,15 @@ __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimS
   const DevGeom& g = desc[e.slot].g;
   const int nchunks = (n + kChunk - 1) / kChunk;
   const int segs = min(e.segs, (nchunks + e.cps - 1) / e.cps);
-  const double* P = part + e.part_off;
-  const size_t S = (size_t)e.segs * e.n_ub;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  sum3_in_order(P + i, S, e.n_ub, segs, sx, sy, sz);
+  if (e.banded) {
+    const double* A = acc + e.acc_off;
+    sx = A[i];
+    sy = A[(size_t)e.n_ub + i];
+    sz = A[2 * (size_t)e.n_ub + i];
+  } else {
+    sum3_in_order(part + e.part_off + i, (size_t)e.segs * e.n_ub, e.n_ub, segs, sx, sy, sz);
+  }
   const int gi = e.base + i;
   double q1 = Q[gi];
   double im = 1.0 / M[gi];
@@ -846,7 +962,7 @@ void segment(int nchunks, int max_segs, int& segs, int& cps) {
 // launches the same table for every jump, uploads it once.
 int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab) {
   const int kRing = 256;
-  const int per = c->nslots_max;
+  const int per = 2 * c->nslots_max;  // the supply's first two levels share a table
   if (ents.size() > (size_t)per) return fail(RD_ERR_ARG, "launch table larger than the pool");
   int r = ensure(c, c->tab, sizeof(Ent) * (size_t)per * kRing);
   if (r) return r;
@@ -886,6 +1002,92 @@ static bool sym_form(const rd_ctx* c, const Ent& e) {
          !(c->force_opts & RD_FORCE_NO_SYMMETRIC);
 }
 
+// FP64 pair launches in bands: the fewest bands B for which one band's
+// partials, ceil(nchunks / B) source chunks of every entry's targets, fit
+// rd_ctx::part_budget.  Band b takes chunks [b K, (b + 1) K) of an entry
+// (K = ceil(nchunks / B)), split into segments for the grid; with B > 1,
+// k_fold64 adds each band's partials to the running sums in chunk order.
+// points: targets are the entries' field points (k_field64), else their
+// particles (k_accel64).  *dlast receives the last band's table, which the
+// finish kernel reads (banded, acc_off, part_off, pslots).
+static int launch_fp64_bands(rd_ctx* c, std::vector<Ent>& ents, bool points, hipEvent_t k0,
+                             bool any_planar, bool any_tip, Ent** dlast, int mtot = 0) {
+  auto targets = [&](const Ent& e) { return points ? e.m : e.n_ub; };
+  int maxc = 1, maxt = 0;
+  long long total = 0;
+  for (const Ent& e : ents) {
+    maxc = std::max(maxc, e.nchunks_ub);
+    maxt = std::max(maxt, targets(e));
+    total += 24LL * std::max(e.nchunks_ub, 1) * targets(e);
+  }
+  int B = (int)std::max(1LL, std::min<long long>(maxc, total / (long long)c->part_budget));
+  for (;; ++B) {
+    long long bytes = 0;
+    for (const Ent& e : ents)
+      bytes += 24LL * ((std::max(e.nchunks_ub, 1) + B - 1) / B) * targets(e);
+    if ((size_t)bytes <= c->part_budget || B >= maxc) break;
+  }
+  long long off = 0, acc_n = 0;
+  for (Ent& e : ents) {
+    const int K = (std::max(e.nchunks_ub, 1) + B - 1) / B;
+    e.pslots = K;
+    e.banded = B > 1;
+    e.part_off = off;
+    off += 3LL * K * targets(e);
+    e.acc_off = acc_n;
+    if (B > 1) acc_n += 3LL * targets(e);
+  }
+  int r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
+  if (!r && acc_n > 0) r = ensure(c, c->acc, sizeof(double) * (size_t)acc_n);
+  if (r) return r;
+  double* part = (double*)c->part.p;
+  const int ne = (int)ents.size();
+  dim3 fin((maxt + 255) / 256, 1, ne);
+  double* ptx = (double*)c->pts.p;
+  if (k0) RD_HIP(hipEventRecord(k0, c->stream));
+  for (int b = 0; b < B; ++b) {
+    int nwg = 0;
+    for (Ent& e : ents) {
+      const int nch = std::max(e.nchunks_ub, 1);
+      e.band0 = std::min(nch, b * e.pslots);
+      e.band1 = std::min(nch, (b + 1) * e.pslots);
+      e.wg0 = nwg;
+      e.segs = 0;
+      if (e.band1 > e.band0 && targets(e) > 0) {
+        const int gx = std::max(1, (targets(e) + kBlock - 1) / kBlock);
+        segment(e.band1 - e.band0, seg_limit(gx, kWantPair), e.segs, e.cps);
+        nwg += gx * e.segs;
+      }
+    }
+    r = upload_tab(c, ents, dlast);
+    if (r) return r;
+    if (nwg > 0) {
+      if (points) {
+        if (any_planar)
+          k_field64<RD_GEOM_PLANAR><<<nwg, kBlock, 0, c->stream>>>(
+              *dlast, ne, c->d_desc, c->d_state, ptx, ptx + mtot, ptx + 2 * (size_t)mtot, c->x,
+              c->y, c->z, c->q, part);
+        if (any_tip)
+          k_field64<RD_GEOM_TIP><<<nwg, kBlock, 0, c->stream>>>(
+              *dlast, ne, c->d_desc, c->d_state, ptx, ptx + mtot, ptx + 2 * (size_t)mtot, c->x,
+              c->y, c->z, c->q, part);
+      } else {
+        if (any_planar)
+          k_accel64<RD_GEOM_PLANAR><<<nwg, kBlock, 0, c->stream>>>(
+              *dlast, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q,
+              (const double*)c->img64.p, part);
+        if (any_tip)
+          k_accel64<RD_GEOM_TIP><<<nwg, kBlock, 0, c->stream>>>(
+              *dlast, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q,
+              (const double*)c->img64.p, part);
+      }
+    }
+    if (B > 1) k_fold64<<<fin, 256, 0, c->stream>>>(*dlast, c->d_state, part, (double*)c->acc.p, points);
+  }
+  RD_HIP(hipGetLastError());
+  return RD_OK;
+}
+
 int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_t k0,
                  hipEvent_t k1) {
   std::vector<Ent> ents;
@@ -907,56 +1109,118 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   std::vector<TailUse> tail;
   int r = tail_decide(c, ents, precision, tail);
   if (r) return r;
-  long long off = 0;
-  int nwg = 0;   // workgroups of the compact gather launch
-  int nsym = 0;  // workgroups of the pair-symmetric launch
   for (Ent& e : ents) {
-    int nch = (e.n_ub + kChunk - 1) / kChunk;
-    const int gx = std::max(1, (e.n_ub + kBlock - 1) / kBlock);
-    e.nchunks_ub = nch;
-    e.wg0 = nwg;
-    e.wg1 = nsym;
-    if (precision == RD_PREC_MIXED && sym_form(c, e)) {
-      const int nb = (e.n_ub + kSymBlock - 1) / kSymBlock;
-      e.segs = nb;
-      e.cps = kSymBlock / kChunk;
-      nsym += nb * (nb + 1) / 2;
-    } else {
-      segment(nch, seg_limit(gx, kWantPair), e.segs, e.cps);
-      nwg += gx * e.segs;
-    }
-    e.part_off = off;
-    off += 3LL * (precision == RD_PREC_FP64 ? nch : e.segs) * e.n_ub;
+    e.nchunks_ub = (e.n_ub + kChunk - 1) / kChunk;
+    e.banded = 0;
+    e.acc_off = 0;
+    e.d_off = 0;
   }
-  r = ensure(c, c->part, sizeof(double) * (size_t)off);
-  if (r) return r;
-  Ent* dt = nullptr;
-  r = upload_tab(c, ents, &dt);
-  if (r) return r;
-  const int ne = (int)ents.size();
-  const int grid = nwg;
-  dim3 fin((maxn + 255) / 256, 1, ne);
-  double* part = (double*)c->part.p;
+  dim3 fin((maxn + 255) / 256, 1, ents.size());
   if (precision == RD_PREC_FP64) {
     if (any_tip) {
       r = ensure(c, c->img64, sizeof(double) * 4 * (size_t)c->cap);
       if (r) return r;
-      k_tip_images64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, c->x, c->y, c->z,
+      Ent* dt0 = nullptr;
+      r = upload_tab(c, ents, &dt0);
+      if (r) return r;
+      k_tip_images64<<<fin, 256, 0, c->stream>>>(dt0, c->d_desc, c->d_state, c->x, c->y, c->z,
                                                   (double*)c->img64.p);
     }
-    if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-    if (any_planar)
-      k_accel64<RD_GEOM_PLANAR><<<grid, kBlock, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
-    if (any_tip)
-      k_accel64<RD_GEOM_TIP><<<grid, kBlock, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, (const double*)c->img64.p, part);
+    Ent* dt = nullptr;
+    r = launch_fp64_bands(c, ents, false, k0, any_planar, any_tip, &dt);
+    if (r) return r;
     if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-    k_accel_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->x, c->y,
-                                                  c->z, c->q, c->m, c->ax, c->ay, c->az);
+    k_accel_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, (double*)c->part.p,
+                                                  (const double*)c->acc.p, c->x, c->y, c->z,
+                                                  c->q, c->m, c->ax, c->ay, c->az);
     RD_HIP(hipGetLastError());
     return RD_OK;
   }
+  // Mixed.  Gather-form entries: one partial set each, kept to the finish;
+  // pair-symmetric entries: nb = ceil(n_ub / kSymBlock) slots, in B bands of
+  // ceil(nb / B) block rows whose partials (reaction and direct) fit the
+  // budget, after the gather partials.
+  long long off = 0;
+  int nwg = 0;  // workgroups of the compact gather launch
+  std::vector<int> sym;
+  for (size_t k = 0; k < ents.size(); ++k) {
+    Ent& e = ents[k];
+    const int gx = std::max(1, (e.n_ub + kBlock - 1) / kBlock);
+    e.wg0 = nwg;
+    e.band0 = 0;
+    if (sym_form(c, e)) {
+      e.segs = (e.n_ub + kSymBlock - 1) / kSymBlock;
+      e.cps = kSymBlock / kChunk;
+      e.band1 = e.segs;
+      sym.push_back((int)k);
+    } else {
+      segment(e.nchunks_ub, seg_limit(gx, kWantPair), e.segs, e.cps);
+      nwg += gx * e.segs;
+      e.band1 = e.segs;
+      e.pslots = e.segs;
+      e.d_ld = e.n_ub;
+      e.part_off = off;
+      off += 3LL * e.segs * e.n_ub;
+    }
+  }
+  int B = 1;
+  if (!sym.empty()) {
+    auto band_bytes = [&](int b) {
+      long long bytes = 0;
+      for (int k : sym) {
+        const Ent& e = ents[k];
+        const long long K = (e.segs + b - 1) / b;
+        bytes += 24LL * K * (e.n_ub + (b > 1 ? (long long)e.segs * kSymBlock : 0));
+      }
+      return bytes;
+    };
+    int maxnb = 1;
+    for (int k : sym) maxnb = std::max(maxnb, ents[k].segs);
+    while (B < maxnb && (size_t)band_bytes(B) > c->part_budget) ++B;
+  }
+  long long acc_n = 0;
+  for (int k : sym) {
+    Ent& e = ents[k];
+    const int K = (e.segs + B - 1) / B;
+    e.pslots = K;
+    e.part_off = off;
+    off += 3LL * K * e.n_ub;
+    if (B > 1) {
+      e.banded = 1;
+      e.d_ld = K * kSymBlock;
+      e.d_off = off;
+      off += 3LL * e.segs * e.d_ld;
+      e.acc_off = acc_n;
+      acc_n += 3LL * e.n_ub;
+    } else {
+      e.d_ld = e.n_ub;
+    }
+  }
+  r = ensure(c, c->part, sizeof(double) * (size_t)std::max(off, 1LL));
+  if (!r && acc_n > 0) r = ensure(c, c->acc, sizeof(double) * (size_t)acc_n);
+  if (r) return r;
+  double* part = (double*)c->part.p;
+  // the table of band b: the symmetric entries' rows and workgroup offsets
+  auto band_table = [&](int b, int& nsym) {
+    nsym = 0;
+    std::vector<Ent> t = ents;
+    for (Ent& e : t) {
+      e.wg1 = nsym;
+      if (!sym_form(c, e)) continue;
+      const int K = e.pslots;
+      e.band0 = std::min(e.segs, b * K);
+      e.band1 = std::min(e.segs, (b + 1) * K);
+      for (int X = e.band0; X < e.band1; ++X) nsym += e.segs - X;
+    }
+    return t;
+  };
+  int nsym = 0;
+  std::vector<Ent> t0 = band_table(0, nsym);
+  Ent* dt = nullptr;
+  r = upload_tab(c, t0, &dt);
+  if (r) return r;
+  const int ne = (int)ents.size();
+  const int grid = nwg;
   r = ensure(c, c->pack_hi, sizeof(float4) * (size_t)c->cap);
   if (!r) r = ensure(c, c->pack_lo, sizeof(float4) * (size_t)c->cap);
   if (r) return r;
@@ -974,16 +1238,6 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   bool near = true;
   for (const Ent& e : ents) near &= e.nic_pair <= 1;
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
-  if (nsym > 0) {
-    if (near)
-      k_accel_sym32<true><<<nsym, kSymBlock, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
-          (const float4*)c->pack_lo.p, part);
-    else
-      k_accel_sym32<false><<<nsym, kSymBlock, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
-          (const float4*)c->pack_lo.p, part);
-  }
   if (any_planar && grid > 0) {
     if (near)
       k_accel_planar32<true><<<grid, kBlock, 0, c->stream>>>(
@@ -998,9 +1252,29 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
     k_accel_tip32<<<grid, kBlock, 0, c->stream>>>(
         dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p, (const float4*)c->pack_lo.p,
         (const float4*)c->img_hi.p, (const float4*)c->img_lo.p, part);
+  for (int b = 0; b < B; ++b) {
+    if (b > 0) {
+      std::vector<Ent> tb = band_table(b, nsym);
+      r = upload_tab(c, tb, &dt);
+      if (r) return r;
+    }
+    if (nsym > 0) {
+      if (near)
+        k_accel_sym32<true><<<nsym, kSymBlock, 0, c->stream>>>(
+            dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
+            (const float4*)c->pack_lo.p, part);
+      else
+        k_accel_sym32<false><<<nsym, kSymBlock, 0, c->stream>>>(
+            dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
+            (const float4*)c->pack_lo.p, part);
+    }
+    if (B > 1)
+      k_fold_sym<<<fin, 256, 0, c->stream>>>(dt, c->d_state, part, (double*)c->acc.p);
+  }
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-  k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, c->x, c->y,
-                                                c->z, c->q, c->m, c->ax, c->ay, c->az);
+  k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part,
+                                                (const double*)c->acc.p, c->x, c->y, c->z, c->q,
+                                                c->m, c->ax, c->ay, c->az);
   RD_HIP(hipGetLastError());
   if (!tail.empty()) return tail_apply(c, ents, tail);
   return RD_OK;
@@ -1032,9 +1306,30 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
     segment(nch, seg_limit(gx, kWantPair), e.segs, e.cps);
     maxn = std::max(maxn, e.n_ub);
     e.part_off = off;
-    off += 3LL * (precision == RD_PREC_FP64 ? std::max(nch, 1) : e.segs) * e.m;
+    e.band0 = 0;
+    e.band1 = std::max(nch, 1);
+    e.pslots = precision == RD_PREC_FP64 ? std::max(nch, 1) : e.segs;
+    e.banded = 0;
+    e.acc_off = 0;
+    off += 3LL * e.pslots * e.m;
     e.wg0 = nwg;
     nwg += gx * e.segs;
+  }
+  dim3 fin((maxm + 255) / 256, 1, ents.size());
+  if (precision == RD_PREC_FP64 && !fixed) {
+    // FP64 in bands of bounded partials (the fixed buffers take one band)
+    double* ptx = (double*)c->pts.p;
+    double* fx = (double*)c->fld.p;
+    Ent* dt = nullptr;
+    int r = launch_fp64_bands(c, ents, true, k0, any_planar, any_tip, &dt, mtot);
+    if (r) return r;
+    if (k1) RD_HIP(hipEventRecord(k1, c->stream));
+    k_field_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, (double*)c->part.p,
+                                                  (const double*)c->acc.p, ptx, ptx + mtot,
+                                                  ptx + 2 * (size_t)mtot, fx, fx + mtot,
+                                                  fx + 2 * (size_t)mtot);
+    RD_HIP(hipGetLastError());
+    return RD_OK;
   }
   const size_t part_bytes = sizeof(double) * (size_t)std::max(off, 1LL);
   double* part = nullptr;
@@ -1061,7 +1356,6 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
   double* fy = fx + mtot;
   double* fz = fy + mtot;
   const int grid = nwg;
-  dim3 fin((maxm + 255) / 256, 1, ne);
   if (precision == RD_PREC_FP64) {
     if (k0) RD_HIP(hipEventRecord(k0, c->stream));
     if (maxn > 0) {
@@ -1073,8 +1367,8 @@ int launch_field(rd_ctx* c, const std::vector<Ent>& in, int mtot, int precision,
             dt, ne, c->d_desc, c->d_state, ptx, pty, ptz, c->x, c->y, c->z, c->q, part);
     }
     if (k1) RD_HIP(hipEventRecord(k1, c->stream));
-    k_field_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, ptx, pty, ptz,
-                                                  fx, fy, fz);
+    k_field_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part, nullptr, ptx,
+                                                  pty, ptz, fx, fy, fz);
     RD_HIP(hipGetLastError());
     return RD_OK;
   }

@@ -218,10 +218,19 @@ __global__ __launch_bounds__(256) void k_tail_scan(const double* __restrict__ Mb
                                                    double* __restrict__ Mtot) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= kTailTerms) return;
+  // 16 blocks' loads in flight at a time; the sum itself stays in block order
+  constexpr int kB = 16;
   double run = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    Mpre[(size_t)b * kTailTerms + t] = run;
-    run += Mblk[(size_t)b * kTailTerms + t];
+  for (int b0 = 0; b0 < nblk; b0 += kB) {
+    double v[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k)
+      v[k] = b0 + k < nblk ? Mblk[(size_t)(b0 + k) * kTailTerms + t] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kB; ++k) {
+      if (b0 + k < nblk) Mpre[(size_t)(b0 + k) * kTailTerms + t] = run;
+      run += v[k];
+    }
   }
   Mtot[t] = run;
 }

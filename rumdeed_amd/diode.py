@@ -118,6 +118,17 @@ class Diode:
         flags = (1 if no_symmetric else 0) | (2 if no_far_expansion else 0)
         check(self._lib.rd_set_force_options(self._h, flags))
 
+    def set_partial_budget(self, nbytes: int = 0):
+        """Bound on one pair launch's partial sums (rd_set_partial_budget;
+        0 = 512 MiB); larger launches run in bands with identical results."""
+        check(self._lib.rd_set_partial_budget(self._h, int(nbytes)))
+
+    def work_bytes(self) -> int:
+        """Device bytes held by the context's work buffers (rd_work_bytes)."""
+        v = ctypes.c_int64()
+        check(self._lib.rd_work_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
     def set_jump_pairing(self, min_workgroups: int):
         """Pool launches of at least min_workgroups jump workgroups pair their
         partial sums (results unchanged; 0: always)."""
@@ -128,14 +139,17 @@ class Diode:
 
     def set_emitter(self, mode: int, delay: int = 0, x0: float = 0.0, y0: float = 0.0,
                     lx: float = 0.0, ly: float = 0.0, T: float = 300.0, w_tip: float = 4.7,
-                    wf=None, quad: int = 0, ndim: int = 0):
-        """Emitter + work function (`work` file: rows top first)."""
+                    wf=None, quad: int = 0, ndim: int = 0, quad_levels: int = 0):
+        """Emitter + work function (`work` file: rows top first).  quad:
+        supply points per side (0 = 32); quad_levels: doublings of the planar
+        supply rule until two levels agree within the reference's Divonne
+        tolerances (-1 none, 0 up to four, k up to k)."""
         wf = np.array([[4.7]]) if wf is None else np.atleast_2d(np.asarray(wf, dtype=np.float64))
         ny, nx = wf.shape
         self._wf = np.ascontiguousarray(wf.ravel())
         em = L.Emitter(mode=int(mode), delay=int(delay), x0=x0, y0=y0, lx=lx, ly=ly, T=T,
                        w_tip=w_tip, wf_nx=nx, wf_ny=ny, wf_table=dptr(self._wf),
-                       quad_n=int(quad), ndim=int(ndim))
+                       quad_n=int(quad), ndim=int(ndim), quad_levels=int(quad_levels))
         check(self._lib.rd_set_emitter(self._h, ctypes.byref(em)))
 
     # ---- the time step (main.F90:175-219)

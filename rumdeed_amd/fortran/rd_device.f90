@@ -89,11 +89,12 @@ contains
 
   ! The emitter of the simulation (emitters_pos/dim/delay(1) of mod_global):
   ! mode RD_EMIT_*, work-function table (rows top first, as in `work`).
-  subroutine RD_Use_Emitter(mode, w_tip, wf, quad_n)
+  subroutine RD_Use_Emitter(mode, w_tip, wf, quad_n, quad_levels)
     integer(c_int), intent(in) :: mode
     double precision, intent(in) :: w_tip
     double precision, intent(in), target :: wf(:, :)  ! (nx, ny): wf(:, row)
     integer, intent(in), optional :: quad_n
+    integer, intent(in), optional :: quad_levels  ! supply refinement (-1 none, 0 default)
     type(rd_emitter) :: em
     em%mode = mode
     em%delay = 0
@@ -116,6 +117,9 @@ contains
     em%quad_n = 0
     if (present(quad_n)) em%quad_n = int(quad_n, c_int)
     em%ndim = 0
+    em%quad_levels = 0
+    if (present(quad_levels)) em%quad_levels = int(quad_levels, c_int)
+    em%reserved = 0
     call rd_check(rd_set_emitter(rd_ctx, em), 'rd_set_emitter')
   end subroutine RD_Use_Emitter
 

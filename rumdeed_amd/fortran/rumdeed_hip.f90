@@ -23,13 +23,13 @@ module rumdeed_hip
     real(c_double) :: x0, y0, lx, ly, T, w_tip
     integer(c_int) :: wf_nx, wf_ny
     type(c_ptr)    :: wf_table
-    integer(c_int) :: quad_n, ndim
+    integer(c_int) :: quad_n, ndim, quad_levels, reserved
   end type rd_emitter
 
   type, bind(C) :: rd_step_record
     integer(c_int) :: step, n, emitted, n_round
     integer(c_int) :: absorbed, absorbed_top, absorbed_bot, dropped
-    integer(c_int) :: start_rounds, reserved0, reserved1, reserved2
+    integer(c_int) :: start_rounds, supply_level, reserved1, reserved2
     real(c_double) :: n_sup, ramo_current, mh_std, a_rate, df_avg
     real(c_double) :: f_avg(3), ramo(3), avg_elec_vel(3)
   end type rd_step_record

@@ -45,16 +45,33 @@ def grid_points(nx=SNAPSHOT_GRID, ny=SNAPSHOT_GRID, box=SNAPSHOT_BOX, z=0.0):
 
 SYM_MIN = 16384  # particles from which the mixed planar force is pair-symmetric
 SYM_REACTION_FLOPS = 9  # reaction on the source: z terms 2, charge products 4, sums 3
+FAR_RHO_MAX = 0.42  # convergence ratio up to which the far partners are an expansion
 
 
-def force_flops(n: int, n_ic_max: int, precision: str = "mixed") -> int:
-    """Algorithmic flops of one planar acceleration evaluation of n
-    particles: the gather form evaluates n (n - 1) ordered pairs; the
-    pair-symmetric form (mixed, n >= SYM_MIN) evaluates n (n - 1) / 2 pairs,
-    each with the reaction on the source added."""
+def far_expansion_used(pos, d: float, n_ic_max: int) -> bool:
+    """Whether the mixed planar accelerations take the image partners n >= 2
+    from the FP64 expansion (the library's rule): N_ic_max >= 2 and
+    2 R / (3 d) <= FAR_RHO_MAX, R the half-diagonal of the particles' lateral
+    bounding box extended by d / 2 vertically."""
+    if n_ic_max < 2 or len(pos) == 0:
+        return False
+    hx = 0.5 * (pos[:, 0].max() - pos[:, 0].min())
+    hy = 0.5 * (pos[:, 1].max() - pos[:, 1].min())
+    R = (hx * hx + hy * hy + 0.25 * d * d) ** 0.5
+    return 2.0 * R / (3.0 * d) <= FAR_RHO_MAX
+
+
+def force_flops(n: int, n_ic_max: int, precision: str = "mixed", far_expansion=False) -> int:
+    """Algorithmic flops of the pair kernel of one planar acceleration
+    evaluation of n particles: the gather form evaluates n (n - 1) ordered
+    pairs; the pair-symmetric form (mixed, n >= SYM_MIN) evaluates
+    n (n - 1) / 2 pairs, each with the reaction on the source added.  With
+    the far expansion the pair kernel takes the partners n <= 1 only (the
+    far ones are O(n) work in FP64, outside this count)."""
+    nic = min(n_ic_max, 1) if (precision == "mixed" and far_expansion) else n_ic_max
     if precision == "mixed" and n >= SYM_MIN:
-        return n * (n - 1) // 2 * (pair_flops(n_ic_max) + SYM_REACTION_FLOPS)
-    return n * (n - 1) * pair_flops(n_ic_max)
+        return n * (n - 1) // 2 * (pair_flops(nic) + SYM_REACTION_FLOPS)
+    return n * (n - 1) * pair_flops(nic)
 
 
 def pair_flops(n_ic_max: int) -> int:

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 NM = 1e-9
 
 REC_INT = ("n", "emitted", "n_round", "absorbed", "absorbed_top", "absorbed_bot", "dropped",
-           "start_rounds")
+           "start_rounds", "supply_level")
 REC_DBL = ("n_sup", "ramo_current", "mh_std", "a_rate", "df_avg")
 
 
@@ -329,6 +329,82 @@ def test_tip_fe_run_bit_exact(Diode, oracle_mod):
     for k in range(1, 7):
         assert_records_equal(recs[k - 1], so.step(k), f"step {k}")
     assert_state_equal(dd.get_state(), so.state(), "end")
+
+
+def _supply_case(target, oracle_mod, quad, levels, wf, pos=None, seed=11):
+    """Thermal-field emission at 6 kV over 1000 nm, 200 x 200 nm emitter."""
+    if hasattr(target, "set_planar"):
+        target.set_planar(6000.0, 1000 * NM, True, 0)
+    else:
+        target.set_geometry(oracle_mod.geom_planar(1000 * NM, 6000.0, True, 0))
+    target.set_time_step(0.25e-15)
+    target.set_seed(seed)
+    target.set_emitter(9, 0, -100 * NM, -100 * NM, 200 * NM, 200 * NM, T=1500.0,
+                       wf=np.array(wf), quad=quad, quad_levels=levels)
+    if pos is not None:
+        n = len(pos)
+        q, m = np.full(n, -1.602176634e-19), np.full(n, 9.1093837015e-31)
+        if hasattr(target, "set_planar"):
+            target.put_particles(pos, q, m)
+        else:
+            target.put(pos, q, m)
+
+
+def _space_charge(n=30, seed=1):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(-100, 100, n), rng.uniform(-100, 100, n),
+                     rng.uniform(2.0, 20.0, n)], 1) * NM
+
+
+def test_supply_refinement_bit_exact(Diode, oracle_mod):
+    """The refined supply rule (from 4 x 4 points, under 30 electrons 2-20 nm
+    above the cathode, then under its own emission): the level reached, N_sup,
+    the chains and every particle equal the oracle's on the FP64 path."""
+    dd = Diode(4096)
+    so = oracle_mod.Sim(4096, nthreads=8)
+    pos = _space_charge()
+    _supply_case(dd, oracle_mod, 4, 0, [[4.0]], pos)
+    _supply_case(so, oracle_mod, 4, 0, [[4.0]], pos)
+    recs = dd.run(8, "fp64")
+    levels = []
+    for k in range(1, 9):
+        ro = so.step(k)
+        assert_records_equal(recs[k - 1], ro, f"step {k}")
+        levels.append(ro.supply_level)
+    assert max(levels) >= 2 and max(levels) <= 4, levels
+    assert_state_equal(dd.get_state(), so.state(), "end")
+
+
+def test_supply_levels_in_a_pool(oracle_mod):
+    """Three simulations whose supply stops at different levels (refined from
+    4 x 4, the fixed 32 x 32 rule, a 7 x 7 checkerboard on 35 x 35 cells)
+    advanced together in one pool equal the oracle running each alone."""
+    from rumdeed_amd.diode import Pool
+
+    wf7 = [[4.0 if (i + j) % 2 == 0 else 4.5 for i in range(7)] for j in range(7)]
+    cases = [dict(quad=4, levels=0, wf=[[4.0]], pos=_space_charge(), seed=11),
+             dict(quad=32, levels=-1, wf=[[4.2]], pos=_space_charge(20, 2), seed=12),
+             dict(quad=32, levels=0, wf=wf7, pos=None, seed=13)]
+    pool = Pool(3 * 2048, 8)
+    slots, oracles = [], []
+    for cs in cases:
+        s = pool.open(2048)
+        _supply_case(pool, oracle_mod, cs["quad"], cs["levels"], cs["wf"], cs["pos"], cs["seed"])
+        slots.append(s)
+        so = oracle_mod.Sim(2048, nthreads=8)
+        _supply_case(so, oracle_mod, cs["quad"], cs["levels"], cs["wf"], cs["pos"], cs["seed"])
+        oracles.append(so)
+    seen = set()
+    for k in range(1, 7):
+        recs = pool.step("fp64")
+        for s, so in zip(slots, oracles):
+            ro = so.step(k)
+            assert_records_equal(recs[s], ro, f"slot {s} step {k}")
+            seen.add((s, ro.supply_level))
+    assert (slots[1], 0) in seen and any(s == slots[0] and l >= 2 for s, l in seen), seen
+    for s, so in zip(slots, oracles):
+        pool.select(s)
+        assert_state_equal(pool.get_state(), so.state(), f"slot {s}")
 
 
 def test_pool_alone_equals_in_the_mix(oracle_mod):
