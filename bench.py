@@ -33,7 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # MI355X published peaks
-PEAK_FP32_VALU_TFLOPS = 157.3
+PEAK_FP32_VALU_TFLOPS = 157.3     # packed (dual-issue) FP32 vector peak
+PEAK_FP32_UNPACKED_TFLOPS = 78.6  # one FP32 FMA per lane per cycle (this build: no packed FP32)
 PEAK_FP64_VALU_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per launch
@@ -41,10 +42,17 @@ COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per launch
 # HBM bytes per dispatch of the dominant kernel from the FETCH_SIZE and
 # WRITE_SIZE counter passes over `bench.py --profile-pass` (one rocprofv3
 # --pmc run per counter), each scaled by what the counter reads for the 1 GiB
-# calibration copy.  Counters cannot be read from inside this process, so the
-# figures are those of the committed summary profiles/
-# snapshot65536_nic10_hbm_counters.txt, for that workload and kernel form only.
-MEASURED_TRAFFIC = {}
+# calibration copy (FETCH_SIZE x 2, WRITE_SIZE x 1), warm dispatch.  Counters
+# cannot be read from inside this process, so the figures are those of the
+# committed summary profiles/snapshot65536_nic10_hbm_counters.txt, for that
+# workload and kernel form only.  The writes are the kernel's FP64 partial
+# sums, 3 x 64 blocks x N doubles = 100.66 MB; the reads are the packed
+# particles (32 B each) as the block pairs fetch them, mostly from L2.
+MEASURED_TRAFFIC = {("k_accel_sym32", 65536, 10): {"read": 27.096e6, "write": 100.663e6}}
+# VALU issue of the same dispatch (profiles/snapshot65536_nic10_valu_counters.txt):
+# SQ_INSTS_VALU over 1024 SIMDs x 2.4 GHz x duration / 4 cycles per wave64
+# instruction.
+MEASURED_VALU_ISSUE = {("k_accel_sym32", 65536, 10): 1.010}
 
 
 def cpu_baseline(n_sample: int, nic: int, threads: int):
@@ -421,6 +429,7 @@ def main():
     copy_ms = dd.time_copy(COPY_BYTES, reps=3)
 
     value = interactions / (ms_mixed * 1e-3)
+    traffic = MEASURED_TRAFFIC.get((kernel, n, nic))
     out = {
         "metric": "pair-interactions/s (FP32-mixed vs FP64)",
         "value": value,
@@ -456,8 +465,11 @@ def main():
         "hbm_copy_gbs": 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
-                     "traffic": MEASURED_TRAFFIC.get((kernel, n, nic)),
+                     "frac_of_unpacked_fp32_peak": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+                     "traffic": traffic and traffic["read"] + traffic["write"],
+                     "traffic_read_write": traffic,
                      "traffic_unit": "bytes per launch (HBM read + write, counter passes)",
+                     "valu_issue_frac": MEASURED_VALU_ISSUE.get((kernel, n, nic)),
                      "kernel": kernel,
                      "flops_per_launch": flops,
                      "image_partners_in_kernel": 1 + 4 * nic_pair,

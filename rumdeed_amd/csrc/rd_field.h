@@ -267,12 +267,15 @@ __device__ __forceinline__ float surface_sum32(float zb, float wb, float dxy2, i
 // and each partner's series in s / u_k^2 <= s / d^2 alternates with
 // decreasing terms.  With kFarTerms terms and s <= kFarS d^2 the truncation
 // stays below |binom(-3/2, 8)| 0.1^8 = 3.4e-8 of each partner's term.  The
-// coefficients are FP64 sums over the partners rounded to FP32 once
-// (surface_far_coeffs); pairs beyond kFarS d^2 take surface_far32.
+// series is evaluated in t = s / d^2 <= kFarS with the coefficients
+// a_m = c_m d^(2m): FP64 sums over the partners, scaled in FP64 and rounded
+// to FP32 once (surface_far_coeffs).  c_m itself falls as d^-(2m+2) and
+// would leave FP32's range from d ~ 1 um; a_m ~ d^-2 stays in it for every
+// gap.  Pairs beyond kFarS d^2 take surface_far32.
 constexpr int kFarTerms = 8;
 constexpr float kFarS = 0.1f;
 
-// c_0 .. c_7 of one source: zb = z_b, wb = d - z_b, dnm = d (nm).
+// a_0 .. a_7 of one source: zb = z_b, wb = d - z_b, dnm = d (nm).
 __device__ __forceinline__ void surface_far_coeffs(double zb, double wb, double dnm, int nic,
                                                    float4& c0, float4& c1) {
   double S[kFarTerms];
@@ -290,22 +293,25 @@ __device__ __forceinline__ void surface_far_coeffs(double zb, double wb, double 
   }
   float c[kFarTerms];
   double b = 1.0;  // binom(-3/2, m)
+  double d2m = 1.0;  // d^(2m)
   for (int m = 0; m < kFarTerms; ++m) {
-    c[m] = (float)(b * S[m]);
+    c[m] = (float)(b * S[m] * d2m);
     b *= (-1.5 - m) / (m + 1.0);
+    d2m *= dnm * dnm;
   }
   c0 = make_float4(c[0], c[1], c[2], c[3]);
   c1 = make_float4(c[4], c[5], c[6], c[7]);
 }
 
-__device__ __forceinline__ float surface_far_eval(float s, float4 c0, float4 c1) {
-  float v = __builtin_fmaf(c1.w, s, c1.z);
-  v = __builtin_fmaf(v, s, c1.y);
-  v = __builtin_fmaf(v, s, c1.x);
-  v = __builtin_fmaf(v, s, c0.w);
-  v = __builtin_fmaf(v, s, c0.z);
-  v = __builtin_fmaf(v, s, c0.y);
-  return __builtin_fmaf(v, s, c0.x);
+// sum_m a_m t^m, t = s / d^2.
+__device__ __forceinline__ float surface_far_eval(float t, float4 c0, float4 c1) {
+  float v = __builtin_fmaf(c1.w, t, c1.z);
+  v = __builtin_fmaf(v, t, c1.y);
+  v = __builtin_fmaf(v, t, c1.x);
+  v = __builtin_fmaf(v, t, c0.w);
+  v = __builtin_fmaf(v, t, c0.z);
+  v = __builtin_fmaf(v, t, c0.y);
+  return __builtin_fmaf(v, t, c0.x);
 }
 
 // q_j/q_0 times surface_sum32 for the packed target (h1, l1) on the cathode
@@ -490,6 +496,7 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
   const float two_d = (float)(2.0 * g.d * 1e9);
   const float dnm = (float)(g.d * 1e9);
   const float slim = kFarS * dnm * dnm;
+  const float inv_d2 = (float)(1.0 / ((g.d * 1e9) * (g.d * 1e9)));
   const bool surf = wave_on_surface(ic, act, h1, l1);
   const int t = threadIdx.x;
   fx = 0.0;
@@ -518,7 +525,7 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
           const float ri = rsq(fmaxf(__builtin_fmaf(h2.z, h2.z, s), kR2Min));
           float w = h2.z * (ri * ri * ri);
           if (s <= slim)
-            w += surface_far_eval(s, sfar[2 * jj], sfar[2 * jj + 1]);
+            w += surface_far_eval(s * inv_d2, sfar[2 * jj], sfar[2 * jj + 1]);
           else
             w = surface_far32(h2.z, l2.w, s, nic, dnm, w);
           fz += (double)(h2.w * w);

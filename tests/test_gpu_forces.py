@@ -179,31 +179,40 @@ def test_mixed_field_on_and_off_the_cathode(Diode, oracle_mod, nic):
     assert np.all(f[:, :2] == 0.0)
 
 
+@pytest.mark.parametrize("d_nm", [200, 1000, 5000])
 @pytest.mark.parametrize("nic", [1, 10])
-def test_mixed_cathode_field_far_series_and_fallback(Diode, oracle_mod, nic):
+def test_mixed_cathode_field_far_series_and_fallback(Diode, oracle_mod, nic, d_nm):
     """On the cathode the image partners k != 0 of a source are summed as a
-    power series in the squared lateral distance s while s <= 0.1 d^2, and
-    one by one beyond it.  Sources spread five gaps wide put the pairs of
-    every wavefront on both sides of that bound; the fields stay within 1e-5
-    of the long-double oracle."""
+    power series in t = s / d^2 (s the squared lateral distance) while
+    t <= 0.1, and one by one beyond it.  Sources spread five gaps wide put
+    the pairs of every wavefront on both sides of that bound.  Gaps of 200 nm
+    to 5 um (the series' coefficients scale as d^-2 in t and stay in FP32's
+    range; in s they would fall as d^-(2m+2)), at a vacuum field of 2 GV/m:
+    per-point error <= 1e-5 against the long-double oracle where the field is
+    not a cancellation (kappa <= 50), scaled error <= 1e-6 everywhere."""
     from rumdeed_amd import workloads as W
+    from test_gpu_config2 import _assert_bars, _stats
 
-    d = 200e-9
+    d = d_nm * 1e-9
+    V = W.SNAPSHOT_V * d / W.SNAPSHOT_D
     pos, q, m = W.snapshot(1500, seed=13)
-    pos[:, :2] *= 5.0  # lateral +-500 nm
+    pos[:, :2] *= 5.0 * d / 200e-9  # lateral +- 2.5 d
     pos[:, 2] *= d / W.SNAPSHOT_D  # z in (0, d]
-    dd = _diode_planar(Diode, pos, q, m, d, W.SNAPSHOT_V, 1, nic)
-    g = oracle_mod.geom_planar(d, W.SNAPSHOT_V, True, nic)
-    on = W.grid_points(16, 16, box=(-500e-9, -500e-9, 1000e-9, 1000e-9))
+    dd = _diode_planar(Diode, pos, q, m, d, V, 1, nic)
+    g = oracle_mod.geom_planar(d, V, True, nic)
+    on = W.grid_points(16, 16, box=(-2.5 * d, -2.5 * d, 5 * d, 5 * d))
     rho2 = ((on[:, None, :2] - pos[None, :, :2]) ** 2).sum(-1)
     near = (rho2 <= 0.1 * d * d).mean()
     assert 0.001 < near < 0.5, near
     ref = oracle_mod.field(g, pos, q, on, long_double=True)
+    scale = oracle_mod.field_scale(g, pos, q, on)
     f = dd.calc_field_at_batch(on, "mixed")
-    e = rel_err(f, ref)
-    print(f"cathode field nic={nic}: {near:.3f} of the pairs in the series range, "
-          f"max {e.max():.2e}")
-    assert e.max() < MIXED_TOL
+    st = _stats(f, ref, scale)
+    print(f"cathode field d={d_nm} nm nic={nic}: {near:.3f} of the pairs in the series range, "
+          f"{st}", flush=True)
+    _assert_bars(st)
+    if d_nm == 200:
+        assert st["max"] < MIXED_TOL
     assert np.all(f[:, :2] == 0.0)
 
 
