@@ -146,6 +146,18 @@ int rd_ctx_destroy(rd_ctx* ctx);
 int rd_set_geometry(rd_ctx* ctx, const rd_geometry* geom);
 int rd_set_time_step(rd_ctx* ctx, double dt);
 
+/* Two-time-step mode: two_time_step with atom_time_interval K
+ * (src/main.F90:391-395; Update_Particle_Position / _Acceleration /
+ * _Velocity, src/mod_verlet.F90:182-193, 396-419, 886-897; the species
+ * routines :234-290, 511-580, 899-1151).  Electrons move every step; ions
+ * move, and their velocities change, only at steps that are multiples of K,
+ * over K time steps; ions receive the electrons' forces every step and the
+ * other ions' forces and the vacuum field at their steps, added to the
+ * acceleration they hold.  K = 0 (the default) is the standard integrator.
+ * FP64 on both geometries, mixed precision on the planar one (the tip
+ * returns RD_ERR_ARG there).  rd_accel keeps the standard accelerations. */
+int rd_set_two_time_step(rd_ctx* ctx, int32_t atom_time_interval);
+
 /* ---- particle store ------------------------------------------------------ */
 /* Replace the whole particle set (snapshot put; acceleration histories are
  * zeroed).  pos, vel: (3,n); q, m: (n); species: (n) or NULL for electrons. */

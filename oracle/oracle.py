@@ -119,6 +119,21 @@ def accel(g: Geom, pos, q, m, nthreads=None, long_double=False):
     return out
 
 
+def accel_tts(g: Geom, pos, q, m, species, K, step, acc=None, nthreads=None):
+    """Two-time-step accelerations (rdo_accel_tts) of step `step` with
+    atom_time_interval K; acc: the accelerations the particles hold (an ion
+    adds onto its own), zeros by default."""
+    pos = _f64(pos)
+    n = pos.shape[0]
+    out = np.zeros((n, 3)) if acc is None else np.array(acc, dtype=np.float64).reshape(n, 3)
+    sp = np.ascontiguousarray(species, np.int32)
+    L = lib()
+    L.rdo_accel_tts.restype = None
+    L.rdo_accel_tts(ctypes.byref(g), c_int(n), _d(pos), _d(_f64(q)), _d(_f64(m)), _i(sp),
+                    c_int(K), c_int(step), _d(out), c_int(nthreads or default_threads()))
+    return out
+
+
 def accel_subset(g: Geom, pos, q, m, targets, nthreads=None, long_double=False):
     """Accelerations (len(targets),3) of the listed particles (0-based) over
     all sources; row k equals row targets[k] of accel()."""
@@ -268,6 +283,9 @@ class Ref:
     def update_position(self, step):
         self.L.ref_update_position(c_int(step))
 
+    def set_two_time_step(self, atom_time_interval):
+        self.L.ref_set_two_time_step(c_int(atom_time_interval))
+
     def remove_particles(self, step):
         self.L.ref_remove_particles(c_int(step))
 
@@ -404,6 +422,8 @@ class Sim:
                                           _dp, c_int, c_int]
         L.rdo_sim_set_quad_levels.restype = None
         L.rdo_sim_set_quad_levels.argtypes = [ctypes.c_void_p, c_int]
+        L.rdo_sim_set_two_time_step.restype = None
+        L.rdo_sim_set_two_time_step.argtypes = [ctypes.c_void_p, c_int]
         L.rdo_sim_step.argtypes = [ctypes.c_void_p, c_int, c_int, ctypes.c_void_p]
         L.rdo_sim_count.argtypes = [ctypes.c_void_p]
         self.L = L
@@ -447,6 +467,10 @@ class Sim:
         self.L.rdo_sim_set_emitter(ctypes.c_void_p(self.h), mode, delay, x0, y0, lx, ly, T, w_tip,
                                    nx, ny, _d(tab), quad, ndim)
         self.L.rdo_sim_set_quad_levels(ctypes.c_void_p(self.h), c_int(quad_levels))
+
+    def set_two_time_step(self, atom_time_interval):
+        """two_time_step with atom_time_interval K (0: off)."""
+        self.L.rdo_sim_set_two_time_step(ctypes.c_void_p(self.h), c_int(atom_time_interval))
 
     def put(self, pos, q, m, vel=None, species=None):
         pos = _f64(pos).reshape(-1, 3)

@@ -243,7 +243,7 @@ void rdo_vacuum_field(const rdo_geom* g, const double* p, double* out) {
 /* ------------------------------------------------------------------------ */
 /* Pair term of the gather kernel for target i and source j != i.           */
 /* Planar: src/mod_verlet.F90:1281-1330.  Tip: :1355-1413.                   */
-static inline void pair_term(const rdo_geom* g, int i, int j, double x1, double y1,
+static inline void pair_term(const rdo_geom* g, int first, int j, double x1, double y1,
                              double z1, double qd1, const double* pos, const double* q,
                              double es, double kt, double z0, double R, double* fx,
                              double* fy, double* fz) {
@@ -260,7 +260,7 @@ static inline void pair_term(const rdo_geom* g, int i, int j, double x1, double 
     az = pre * inv_r3 * dz;
     if (g->image_charge) {
       double z_a, z_b, icx, icy, icz;
-      if (j > i) {
+      if (first) {
         z_a = z1;
         z_b = z2;
       } else {
@@ -280,7 +280,7 @@ static inline void pair_term(const rdo_geom* g, int i, int j, double x1, double 
     az = pre * inv_r3 * dz;
     if (g->image_charge) {
       double xa, ya, za, xb, yb, zb, sgn;
-      if (j > i) {
+      if (first) {
         xa = x1; ya = y1; za = z1; xb = x2; yb = y2; zb = z2; sgn = 1.0;
       } else {
         xa = x2; ya = y2; za = z2; xb = x1; yb = y1; zb = z1; sgn = -1.0;
@@ -299,8 +299,18 @@ static inline void pair_term(const rdo_geom* g, int i, int j, double x1, double 
 }
 
 /* Acceleration of target i over all n sources in the canonical order. */
-static void accel_target(const rdo_geom* g, int n, const double* pos, const double* q,
-                         const double* m, int i, double* out) {
+/* Acceleration of target i.  The target takes the first role of a pair
+ * (the image series evaluated at it) when it has the lower index
+ * (Calculate_Acceleration_Particles_ACC, src/mod_verlet.F90:1217-1429).
+ * With sp != NULL and K > 0, the two-time-step accelerations of step `step`
+ * (Update_Elec_Acceleration / Update_Ion_Acceleration,
+ * src/mod_verlet.F90:899-1151): between an electron and an ion the electron
+ * takes the first role; an ion between its update steps (step % K != 0)
+ * gets the electrons' reactions only, at its update steps also the other
+ * ions and the vacuum field; an ion adds onto the acceleration it holds. */
+static void accel_target_k(const rdo_geom* g, int n, const double* pos, const double* q,
+                           const double* m, const int* sp, int K, int step, int i,
+                           double* out) {
   const double es = eps_soft(), dfc = div_fac_c(), kt = k_tip();
   const double z0 = g->h_tip - g->r_tip, R = g->r_tip;
   {
@@ -308,14 +318,22 @@ static void accel_target(const rdo_geom* g, int n, const double* pos, const doub
     double x1 = pos[3 * i], y1 = pos[3 * i + 1], z1 = pos[3 * i + 2];
     double q1 = q[i];
     double qd1 = q1 * dfc;
+    const int tts = sp != NULL && K > 0;
+    const int sp1 = tts ? sp[i] : 0;
+    const int ion_only_e = tts && sp1 == 2 && step % K != 0;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int c0 = 0; c0 < n; c0 += RDO_CHUNK) {
       int c1 = c0 + RDO_CHUNK < n ? c0 + RDO_CHUNK : n;
       double px = 0.0, py = 0.0, pz = 0.0;
       for (int j = c0; j < c1; ++j) {
         if (j == i) continue;
+        int first = j > i;
+        if (tts) {
+          if (ion_only_e && sp[j] == 2) continue;
+          if (sp1 != sp[j]) first = sp1 == 1;
+        }
         double fx, fy, fz;
-        pair_term(g, i, j, x1, y1, z1, qd1, pos, q, es, kt, z0, R, &fx, &fy, &fz);
+        pair_term(g, first, j, x1, y1, z1, qd1, pos, q, es, kt, z0, R, &fx, &fy, &fz);
         px = px + fx;
         py = py + fy;
         pz = pz + fz;
@@ -325,18 +343,40 @@ static void accel_target(const rdo_geom* g, int n, const double* pos, const doub
       sz = sz + pz;
     }
     double im = 1.0 / m[i];
+    const int add = tts && sp1 == 2;
+    const int vac = !add || step % K == 0;
+    double a[3];
     if (g->kind == RDO_GEOM_PLANAR) {
-      acc[3 * i] = sx * im;
-      acc[3 * i + 1] = sy * im;
-      acc[3 * i + 2] = (sz + q1 * g->E_z) * im;
-    } else {
+      a[0] = sx * im;
+      a[1] = sy * im;
+      a[2] = (vac ? sz + q1 * g->E_z : sz) * im;
+    } else if (vac) {
       double fx, fy, fz;
       tip_vacuum(g, x1, y1, z1, &fx, &fy, &fz);
-      acc[3 * i] = (sx + q1 * fx) * im;
-      acc[3 * i + 1] = (sy + q1 * fy) * im;
-      acc[3 * i + 2] = (sz + q1 * fz) * im;
+      a[0] = (sx + q1 * fx) * im;
+      a[1] = (sy + q1 * fy) * im;
+      a[2] = (sz + q1 * fz) * im;
+    } else {
+      a[0] = sx * im;
+      a[1] = sy * im;
+      a[2] = sz * im;
     }
+    for (int k = 0; k < 3; ++k) acc[3 * i + k] = add ? acc[3 * i + k] + a[k] : a[k];
   }
+}
+static void accel_target(const rdo_geom* g, int n, const double* pos, const double* q,
+                         const double* m, int i, double* out) {
+  accel_target_k(g, n, pos, q, m, NULL, 0, 0, i, out);
+}
+
+void rdo_accel_tts(const rdo_geom* g, int n, const double* pos, const double* q,
+                   const double* m, const int* species, int K, int step, double* acc,
+                   int nthreads) {
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+  for (int i = 0; i < n; ++i) accel_target_k(g, n, pos, q, m, species, K, step, i, &acc[3 * i]);
 }
 
 void rdo_accel(const rdo_geom* g, int n, const double* pos, const double* q, const double* m,
@@ -481,7 +521,7 @@ void rdo_accel_scale(const rdo_geom* g, int n, const double* pos, const double* 
     for (int j = 0; j < n; ++j) {
       if (j == i) continue;
       double fx, fy, fz;
-      pair_term(g, i, j, x1, y1, z1, qd1, pos, q, es, kt, z0, R, &fx, &fy, &fz);
+      pair_term(g, j > i, j, x1, y1, z1, qd1, pos, q, es, kt, z0, R, &fx, &fy, &fz);
       sc += sqrt(fx * fx + fy * fy + fz * fz);
     }
     double e[3];

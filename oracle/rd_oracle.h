@@ -54,6 +54,15 @@ int rdo_sum_chunk(void);
  * canonical FP64 summation order.  pos/acc are (3,n) column-major. */
 void rdo_accel(const rdo_geom* g, int n, const double* pos, const double* q,
                const double* m, double* acc, int nthreads);
+/* The two-time-step accelerations of step `step` with atom_time_interval
+ * K (Update_Elec_Acceleration / Update_Ion_Acceleration,
+ * src/mod_verlet.F90:899-1151), species 1 electron, 2 ion: electrons as
+ * rdo_accel but with the electron in the first role of an electron-ion
+ * pair; an ion adds onto acc the electrons' reactions, and at steps that are
+ * multiples of K also the other ions' forces and the vacuum field. */
+void rdo_accel_tts(const rdo_geom* g, int n, const double* pos, const double* q,
+                   const double* m, const int* species, int K, int step, double* acc,
+                   int nthreads);
 /* Same physics, long double, natural j order (brute-force error reference). */
 void rdo_accel_ld(const rdo_geom* g, int n, const double* pos, const double* q,
                   const double* m, double* acc, int nthreads);

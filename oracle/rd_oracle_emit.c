@@ -11,7 +11,9 @@
  *   FN / GTF ........ src/mod_field_emission_v2.F90:515-625, src/mod_emission_tip.f90:1657-1758,
  *                     src/mod_kevin_rjgtf_v2.f90:56-175
  *   work function ... w_theta_checkerboard, src/mod_work_function.F90:389-487
- *   Beeman .......... Update_ElecIon_Position / _Velocity, src/mod_verlet.F90:197-232, :449-509
+ *   Beeman .......... Update_ElecIon_Position / _Velocity, src/mod_verlet.F90:197-232, :449-509;
+ *                     two-time-step Update_Species_Position / _Velocity and the species
+ *                     accelerations, src/mod_verlet.F90:182-193, :234-290, :511-580, :899-1151
  *   boundaries ...... Check_Boundary_Planar src/mod_verlet.F90:325-338,
  *                     Check_Boundary_Tip src/mod_emission_tip.f90:1627-1647
  *   removal ......... Mark_Particles_Remove, Remove_Particles, record_lifetime,
@@ -24,7 +26,8 @@
  *   - draws come from Philox4x64-10 with key (seed, stream) and counter
  *     (step, purpose<<40 | sub, chain, block);
  *   - the planar supply integral (Divonne in the reference) is the midpoint
- *     rule on quad x quad points of the emitter, refined by doubling the
+ *     rule on a grid of the emitter whose cells follow the work function's
+ *     sections (at least quad points per side), refined by doubling the
  *     points per side until two levels agree within the reference's Divonne
  *     tolerances; the tip keeps the reference's 100 x 100 (xi, phi) cells;
  *     each level is summed in tiles of 1024 points, pairwise within a tile,
@@ -356,6 +359,7 @@ typedef struct {
   int quad, ndim;
   int qgx, qgy;  /* level-0 points per side of the planar supply (x, y) */
   int quad_lmax; /* refinement doublings of the planar supply integral */
+  int tts;       /* two-time-step atom_time_interval; 0: the standard integrator */
   uint64_t seed, stream;
   int cap, n, nrID;
   double *pos, *prev, *vel, *acc, *a1, *a2, *q, *m;
@@ -503,6 +507,7 @@ int rdo_quad_lmax(int g0, int levels) {
   while (l > 0 && (g0 << l) > 1024) --l;
   return l;
 }
+void rdo_sim_set_two_time_step(rdo_sim* s, int K) { s->tts = K > 0 ? K : 0; }
 void rdo_sim_set_quad_levels(rdo_sim* s, int levels) {
   s->quad_lmax = rdo_quad_lmax(s->qgx > s->qgy ? s->qgx : s->qgy, levels);
 }
@@ -1235,15 +1240,26 @@ static void check_planes(rdo_sim* s, int i) {
   }
 }
 
-static void update_position(rdo_sim* s, int precision_unused) {
-  (void)precision_unused;
+/* Velocity_Verlet (src/mod_verlet.F90:123-162): positions, accelerations,
+ * velocities.  Two-time-step mode (s->tts = atom_time_interval K > 0,
+ * Update_Species_Position / _Velocity, :234-290, :511-580): ions move only
+ * at steps that are multiples of K, over K time steps; between them nothing
+ * of an ion changes, no boundary is checked and it adds nothing to the Ramo
+ * current or the velocity sums. */
+static int ion_rests(const rdo_sim* s, int i, int step) {
+  return s->tts > 0 && s->species[i] == 2 && step % s->tts != 0;
+}
+static void update_position(rdo_sim* s, int step) {
   const double c16 = 1.0 / 6.0;
   for (int i = 0; i < s->n; ++i) {
+    if (ion_rests(s, i, step)) continue;
+    const double ks = (s->tts > 0 && s->species[i] == 2) ? (double)s->tts : 1.0;
     for (int k = 0; k < 3; ++k) {
       double x = s->pos[3 * i + k];
       s->prev[3 * i + k] = x;
       double a = s->acc[3 * i + k], a1 = s->a1[3 * i + k];
-      s->pos[3 * i + k] = x + s->vel[3 * i + k] * s->dt + c16 * (4.0 * a - a1) * s->dt2;
+      s->pos[3 * i + k] =
+          x + s->vel[3 * i + k] * s->dt * ks + c16 * (4.0 * a - a1) * s->dt2 * (ks * ks);
       s->a2[3 * i + k] = a1;
       s->a1[3 * i + k] = a;
       s->acc[3 * i + k] = 0.0;
@@ -1257,7 +1273,10 @@ static void update_position(rdo_sim* s, int precision_unused) {
     }
     check_planes(s, i);
   }
-  rdo_accel(&s->g, s->n, s->pos, s->q, s->m, s->acc, s->nthreads);
+  if (s->tts > 0)
+    rdo_accel_tts(&s->g, s->n, s->pos, s->q, s->m, s->species, s->tts, step, s->acc, s->nthreads);
+  else
+    rdo_accel(&s->g, s->n, s->pos, s->q, s->m, s->acc, s->nthreads);
   /* velocity + Ramo; the device sums 256-particle blocks with a fixed tree
    * and then the block sums in order: reproduced here */
   const int nb = s->n > 0 ? (s->n + 255) / 256 : 1;
@@ -1268,11 +1287,13 @@ static void update_position(rdo_sim* s, int precision_unused) {
     for (int t = 0; t < 256; ++t) {
       for (int k = 0; k < 13; ++k) lane[k][t] = 0.0;
       int i = b * 256 + t;
-      if (i >= s->n) continue;
+      if (i >= s->n || ion_rests(s, i, step)) continue;
+      const double ks = (s->tts > 0 && s->species[i] == 2) ? (double)s->tts : 1.0;
       double v[3];
       for (int k = 0; k < 3; ++k)
         v[k] = s->vel[3 * i + k] +
-               c16 * (2.0 * s->acc[3 * i + k] + 5.0 * s->a1[3 * i + k] - s->a2[3 * i + k]) * s->dt;
+               c16 * (2.0 * s->acc[3 * i + k] + 5.0 * s->a1[3 * i + k] - s->a2[3 * i + k]) *
+                   s->dt * ks;
       for (int k = 0; k < 3; ++k) s->vel[3 * i + k] = v[k];
       double u[3];
       if (s->g.kind == RDO_GEOM_TIP) {
@@ -1357,7 +1378,7 @@ void rdo_sim_step(rdo_sim* s, int step, int phases, rdo_record* rec) {
     /* a step's counters start at zero with its first phase: the emission
        phase of the same step when it ran on its own, else this update */
     if (!(phases & 1) && s->emit_step != step) reset_counters(s);
-    update_position(s, 0);
+    update_position(s, step);
   }
   if (phases & 4) remove_particles(s, step);
   if (rec) {

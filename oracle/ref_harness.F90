@@ -222,6 +222,14 @@ contains
     out(8) = theta_tip
   end subroutine ref_tip_params
 
+  ! two_time_step with atom_time_interval k (0: off), as Read_Input_Variables
+  ! sets them (src/main.F90:391-395).
+  subroutine ref_set_two_time_step(k) bind(C, name='ref_set_two_time_step')
+    integer(c_int), value :: k
+    atom_time_interval = k
+    two_time_step = (k /= 0)
+  end subroutine ref_set_two_time_step
+
   subroutine ref_add_particle(pos, vel, spc, step, emit, sec) bind(C, name='ref_add_particle')
     real(c_double), intent(in) :: pos(3), vel(3)
     integer(c_int), value :: spc, step, emit, sec

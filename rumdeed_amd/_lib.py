@@ -101,6 +101,7 @@ _SIGS = {
     "rd_ctx_destroy": (c_int32, [c_void_p]),
     "rd_set_geometry": (c_int32, [c_void_p, POINTER(Geometry)]),
     "rd_set_time_step": (c_int32, [c_void_p, c_double]),
+    "rd_set_two_time_step": (c_int32, [c_void_p, c_int32]),
     "rd_put_particles": (c_int32, [c_void_p, c_int32, _dp, _dp, _dp, _dp, _ip]),
     "rd_particle_count": (c_int32, [c_void_p, _ip]),
     "rd_get_particles": (c_int32, [c_void_p, _dp, _dp, _dp, _dp, _ip]),

@@ -451,6 +451,14 @@ int rd_set_time_step(rd_ctx* c, double dt) {
   return put_desc(c, c->cur);
 }
 
+int rd_set_two_time_step(rd_ctx* c, int32_t atom_time_interval) {
+  RD_CUR(c);
+  if (atom_time_interval < 0 || atom_time_interval > (1 << 20))
+    return fail(RD_ERR_ARG, "atom_time_interval outside 0..2^20");
+  sl->desc.tts = atom_time_interval;
+  return put_desc(c, c->cur);
+}
+
 int rd_set_seed(rd_ctx* c, uint64_t seed, uint64_t stream) {
   RD_CUR(c);
   sl->desc.seed = seed;

@@ -1562,6 +1562,12 @@ __global__ void k_position(const Ent* tab, const SimDesc* desc, SimState* st, do
   if (i >= n) return;
   const SimDesc& s = desc[e.slot];
   const int gi = e.base + i;
+  // two-time-step mode (Update_Species_Position, src/mod_verlet.F90:234-290):
+  // an ion moves only at steps that are multiples of K, over K time steps;
+  // between them nothing of it changes and no boundary is checked
+  const bool ion_k = s.tts > 0 && SP[gi] == RD_SPECIES_ION;
+  if (ion_k && (e.step % s.tts) != 0) return;
+  const double ks = ion_k ? (double)s.tts : 1.0;
   const double dt = s.dt, dt2 = s.dt2;
   const double c16 = 1.0 / 6.0;
   double x = X[gi], y = Y[gi], z = Z[gi];
@@ -1571,9 +1577,9 @@ __global__ void k_position(const Ent* tab, const SimDesc* desc, SimState* st, do
   PZ[gi] = z;
   double ax = AX[gi], ay = AY[gi], az = AZ[gi];
   double a1x = A1X[gi], a1y = A1Y[gi], a1z = A1Z[gi];
-  x = x + VX[gi] * dt + c16 * (4.0 * ax - a1x) * dt2;
-  y = y + VY[gi] * dt + c16 * (4.0 * ay - a1y) * dt2;
-  z = z + VZ[gi] * dt + c16 * (4.0 * az - a1z) * dt2;
+  x = x + VX[gi] * dt * ks + c16 * (4.0 * ax - a1x) * dt2 * (ks * ks);
+  y = y + VY[gi] * dt * ks + c16 * (4.0 * ay - a1y) * dt2 * (ks * ks);
+  z = z + VZ[gi] * dt * ks + c16 * (4.0 * az - a1z) * dt2 * (ks * ks);
   X[gi] = x;
   Y[gi] = y;
   Z[gi] = z;
@@ -1631,13 +1637,17 @@ __global__ __launch_bounds__(256) void k_velocity(const Ent* tab, const SimDesc*
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v[13];
   for (int k = 0; k < 13; ++k) v[k] = 0.0;
-  if (i < n) {
+  // two-time-step mode (Update_Species_Velocity, src/mod_verlet.F90:511-580):
+  // ions move only at steps that are multiples of K, over K time steps
+  const bool ion_k = s.tts > 0 && i < n && SP[e.base + i] == RD_SPECIES_ION;
+  if (i < n && !(ion_k && (e.step % s.tts) != 0)) {
     const int gi = e.base + i;
     const double dt = s.dt;
     const double c16 = 1.0 / 6.0;
-    double vx = VX[gi] + c16 * (2.0 * AX[gi] + 5.0 * A1X[gi] - A2X[gi]) * dt;
-    double vy = VY[gi] + c16 * (2.0 * AY[gi] + 5.0 * A1Y[gi] - A2Y[gi]) * dt;
-    double vz = VZ[gi] + c16 * (2.0 * AZ[gi] + 5.0 * A1Z[gi] - A2Z[gi]) * dt;
+    const double ks = ion_k ? (double)s.tts : 1.0;
+    double vx = VX[gi] + c16 * (2.0 * AX[gi] + 5.0 * A1X[gi] - A2X[gi]) * dt * ks;
+    double vy = VY[gi] + c16 * (2.0 * AY[gi] + 5.0 * A1Y[gi] - A2Y[gi]) * dt * ks;
+    double vz = VZ[gi] + c16 * (2.0 * AZ[gi] + 5.0 * A1Z[gi] - A2Z[gi]) * dt * ks;
     VX[gi] = vx;
     VY[gi] = vy;
     VZ[gi] = vz;
@@ -2447,6 +2457,12 @@ int engine_phases(rd_ctx* c, const std::vector<int>& slots, int precision, int p
     ents.push_back(e);
   }
   if (ents.empty()) return RD_OK;
+  if ((phases & 2) && precision == RD_PREC_MIXED)
+    for (const Ent& e : ents) {
+      const SimDesc& d = c->slots[e.slot].desc;
+      if (d.tts > 0 && d.g.kind == RD_GEOM_TIP)
+        return fail(RD_ERR_ARG, "two-time-step mode on the tip runs in FP64 only");
+    }
   int r;
   if (phases & 1) {
     r = reset_counters(c, ents);
@@ -2476,7 +2492,13 @@ int engine_phases(rd_ctx* c, const std::vector<int>& slots, int precision, int p
       }
     }
     r = engine_position(c, ents);
-    if (!r) r = launch_accel(c, ents, precision, nullptr, nullptr);
+    if (!r) {
+      // Update_Particle_Acceleration: the two-time-step accelerations where
+      // the simulation runs that mode
+      std::vector<Ent> acc = ents;
+      for (Ent& e : acc) e.tts = c->slots[e.slot].desc.tts;
+      r = launch_accel(c, acc, precision, nullptr, nullptr);
+    }
     if (!r) r = engine_velocity(c, ents);
     if (r) return r;
   }

@@ -61,6 +61,7 @@ struct SimDesc {
   int qgx, qgy;               // level-0 supply points per side (x, y): multiples
                               // of the work function's sections (quad_side)
   int quad_lmax;              // refinement doublings allowed (planar)
+  int tts;                    // atom_time_interval of the two-time-step mode; 0: off
   int base, cap;
   int ndim;                   // Metropolis-Hastings jumps per chain
   int nplanes;                // recording planes (Check_Planes)
@@ -104,6 +105,8 @@ struct Ent {
   int nic_pair;   // image partners n <= nic_pair in the mixed pair kernels (the
                   // rest, when fewer than N_ic_max, come from rd_tail.hip)
   int lev;        // refinement level of the supply points (quad << lev per side)
+  int tts;        // accelerations of the two-time-step update: atom_time_interval;
+                  // 0: the standard accelerations (Calculate_Acceleration_Particles)
   // Banded launches (partial sums bounded by rd_ctx::part_budget).  FP64: the
   // launch takes source chunks [band0, band1), its partial slot k holding
   // chunk band0 + k; pair-symmetric: the launch takes block rows

@@ -75,6 +75,36 @@ __global__ void k_tip_images64(const Ent* tab, const SimDesc* desc, const SimSta
 }
 
 // ------------------------------------------------------------------------
+// Two-time-step accelerations (Update_Elec_Acceleration and
+// Update_Ion_Acceleration, src/mod_verlet.F90:899-1151, with atom_time_interval
+// K = Ent::tts > 0).  Every step an electron gets the force of every particle
+// and the vacuum field; an ion gets the reaction of every electron, and at
+// steps that are multiples of K also the force of the other ions and the
+// vacuum field, all added to the acceleration it holds (an ion's is cleared
+// only by its own position update, every K steps).  Between an electron and
+// an ion the electron takes the first role of the pair (its image series is
+// evaluated at the electron; the ion gets the mirrored reaction); within a
+// species the lower index does, as in the standard loops.
+struct TtsRole {
+  int sp1 = 0;            // the target's species
+  bool ion_only_e = false;  // ion target between its updates: electron sources only
+};
+__device__ __forceinline__ TtsRole tts_role(const Ent& e, const int32_t* sp, int gi, bool act) {
+  TtsRole r;
+  if (e.tts > 0 && act) {
+    r.sp1 = sp[gi];
+    r.ion_only_e = r.sp1 == RD_SPECIES_ION && (e.step % e.tts) != 0;
+  }
+  return r;
+}
+// Whether the target takes the first role of the pair with source j (species
+// sp2), and whether the pair contributes at all.
+__device__ __forceinline__ bool tts_pair(const TtsRole& r, int i, int j, int sp2, bool& first) {
+  first = j > i;
+  if (r.sp1 != sp2) first = r.sp1 == RD_SPECIES_ELEC;
+  return !(r.ion_only_e && sp2 == RD_SPECIES_ION);
+}
+
 // FP64 accelerations.  part layout per entry: [3][nchunks_ub][n_ub].
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, const SimDesc* desc,
@@ -83,9 +113,11 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
                                                     const double* __restrict__ Y,
                                                     const double* __restrict__ Z,
                                                     const double* __restrict__ Q,
+                                                    const int32_t* __restrict__ SP,
                                                     const double* __restrict__ img,
                                                     double* __restrict__ part) {
   __shared__ double sx[kChunk], sy[kChunk], sz[kChunk], sq[kChunk];
+  __shared__ int ssp[kChunk];
   __shared__ double sd[KIND == RD_GEOM_TIP ? kChunk : 1], sxi[KIND == RD_GEOM_TIP ? kChunk : 1],
       syi[KIND == RD_GEOM_TIP ? kChunk : 1], szi[KIND == RD_GEOM_TIP ? kChunk : 1];
   const Tile tl = tile_of(tab, ne, false);
@@ -106,6 +138,8 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
   const int t = threadIdx.x;
   const int i = tl.bx * kBlock + t;
   const bool act = i < n;
+  const bool tts = tl.e.tts > 0;
+  const TtsRole role = tts_role(tl.e, SP, base + i, act);
   double x1 = 0, y1 = 0, z1 = 0, qd1 = 0, d1 = 0, xi1 = 0, yi1 = 0, zi1 = 0;
   if (act) {
     x1 = x[i];
@@ -127,6 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
       sy[t] = y[jb + t];
       sz[t] = z[jb + t];
       sq[t] = q[jb + t];
+      if (tts) ssp[t] = SP[base + jb + t];
       if (KIND == RD_GEOM_TIP && g.ic) {
         const size_t gj = 4 * (size_t)(base + jb + t);
         sd[t] = img[gj];
@@ -142,6 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
       for (int jj = 0; jj < cnt; ++jj) {
         const int j = jb + jj;
         if (j == i) continue;
+        // the target takes the first role of the pair (the lower index; see
+        // TtsRole for the two-time-step update)
+        bool first = j > i;
+        if (tts && !tts_pair(role, i, j, ssp[jj], first)) continue;
         double x2 = sx[jj], y2 = sy[jj], z2 = sz[jj];
         double pre = qd1 * sq[jj];
         double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
@@ -154,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
           ayy = pre * inv_r3 * dy;
           azz = pre * inv_r3 * dz;
           if (g.ic) {
-            double za = j > i ? z1 : z2;
-            double zb = j > i ? z2 : z1;
+            double za = first ? z1 : z2;
+            double zb = first ? z2 : z1;
             double icx, icy, icz;
             ic_planar64(za, zb, dx, dy, dxy2, g.nic, g.d, g.es, icx, icy, icz);
             axx = axx + pre * icx;
@@ -170,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_accel64(const Ent* tab, int ne, cons
           azz = pre * inv_r3 * dz;
           if (g.ic) {
             double icx, icy, icz;
-            if (j > i) {
+            if (first) {
               tip_ic_force64(x1, y1, z1, x2, y2, z2, d1, xi1, yi1, zi1, g.R, g.ktip, icx, icy,
                              icz);
               axx = axx + pre * 1.0 * icx;
@@ -233,6 +272,7 @@ __global__ void k_accel_finish64(const Ent* tab, const SimDesc* desc, const SimS
                                  const double* __restrict__ X,
                                  const double* __restrict__ Y, const double* __restrict__ Z,
                                  const double* __restrict__ Q, const double* __restrict__ M,
+                                 const int32_t* __restrict__ SP,
                                  double* __restrict__ AX, double* __restrict__ AY,
                                  double* __restrict__ AZ) {
   const Ent e = tab[blockIdx.z];
@@ -253,17 +293,34 @@ __global__ void k_accel_finish64(const Ent* tab, const SimDesc* desc, const SimS
   const int gi = e.base + i;
   double q1 = Q[gi];
   double im = 1.0 / M[gi];
+  // two-time-step ions add onto the acceleration they hold, with the vacuum
+  // field only at their update steps
+  const bool add = e.tts > 0 && SP[gi] == RD_SPECIES_ION;
+  const bool vac = !add || (e.step % e.tts) == 0;
+  double ax, ay, az;
   if (g.kind == RD_GEOM_PLANAR) {
-    AX[gi] = sx * im;
-    AY[gi] = sy * im;
-    AZ[gi] = (sz + q1 * g.Ez) * im;
-  } else {
+    ax = sx * im;
+    ay = sy * im;
+    az = (vac ? sz + q1 * g.Ez : sz) * im;
+  } else if (vac) {
     double fx, fy, fz;
     tip_vacuum64(g, X[gi], Y[gi], Z[gi], fx, fy, fz);
-    AX[gi] = (sx + q1 * fx) * im;
-    AY[gi] = (sy + q1 * fy) * im;
-    AZ[gi] = (sz + q1 * fz) * im;
+    ax = (sx + q1 * fx) * im;
+    ay = (sy + q1 * fy) * im;
+    az = (sz + q1 * fz) * im;
+  } else {
+    ax = sx * im;
+    ay = sy * im;
+    az = sz * im;
   }
+  if (add) {
+    ax = AX[gi] + ax;
+    ay = AY[gi] + ay;
+    az = AZ[gi] + az;
+  }
+  AX[gi] = ax;
+  AY[gi] = ay;
+  AZ[gi] = az;
 }
 
 // ------------------------------------------------------------------------
@@ -386,13 +443,15 @@ __global__ void k_pack_points(const Ent* tab, const SimDesc* desc, const double*
 
 // Mixed accelerations (planar).  part layout per entry: [3][segs][n_ub] FP64
 // sums of q_j/q_0 * (offset / r^3 + image terms) in nm^-2.
-template <bool NEAR>
+template <bool NEAR, bool TTS>
 __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int ne, const SimDesc* desc,
                                                            const SimState* st,
                                                            const float4* __restrict__ HI,
                                                            const float4* __restrict__ LO,
+                                                           const int32_t* __restrict__ SP,
                                                            double* __restrict__ part) {
   __shared__ float4 shi[kChunk], slo[kChunk];
+  __shared__ int ssp[TTS ? kChunk : 1];
   const Tile tl = tile_of(tab, ne, false);
   const Ent& e = tl.e;
   const int n = st[e.slot].n;
@@ -415,6 +474,8 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
     h1 = hi[i];
     l1 = lo[i];
   }
+  TtsRole role;
+  if (TTS) role = tts_role(e, SP, e.base + i, act);
   double fx = 0.0, fy = 0.0, fz = 0.0;
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
@@ -422,6 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
     if (jb + t < n) {
       shi[t] = hi[jb + t];
       slo[t] = lo[jb + t];
+      if (TTS) ssp[t] = e.tts > 0 ? SP[e.base + jb + t] : 0;
     }
     __syncthreads();
     const int cnt = min(kChunk, n - jb);
@@ -433,6 +495,8 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
         float dy = (h1.y - h2.y) + (l1.y - l2.y);
         float dz = (h1.z - h2.z) + (l1.z - l2.z);
         float qj = (j == i) ? 0.0f : h2.w;
+        bool first = j > i;  // the target takes the first role of the pair
+        if (TTS && e.tts > 0 && !tts_pair(role, i, j, ssp[jj], first)) qj = 0.0f;
         float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
         float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
         float ri = rsq(r2);
@@ -440,7 +504,7 @@ __global__ __launch_bounds__(kBlock) void k_accel_planar32(const Ent* tab, int n
         float cxy = r3, cz = dz * r3;
         if (ic) {
           float wxy, wz;
-          float dzab = (j > i) ? dz : -dz;
+          float dzab = first ? dz : -dz;
           ic_planar32<NEAR>(h1.z + h2.z, l1.w + l2.w, dzab, dxy2, nic, two_d, wxy, wz);
           cxy += wxy;
           cz += wz;
@@ -699,6 +763,7 @@ __global__ void k_fold_sym(const Ent* tab, const SimState* st, const double* __r
 
 __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimState* st,
                                  const double* __restrict__ part, const double* __restrict__ acc,
+                                 const int32_t* __restrict__ SP,
                                  const double* __restrict__ X,
                                  const double* __restrict__ Y, const double* __restrict__ Z,
                                  const double* __restrict__ Q, const double* __restrict__ M,
@@ -732,9 +797,14 @@ __global__ void k_accel_finish32(const Ent* tab, const SimDesc* desc, const SimS
     AZ[gi] = pre * sz + q1 * ez * im;
     return;
   }
-  AX[gi] = pre * sx;
-  AY[gi] = pre * sy;
-  AZ[gi] = pre * sz + q1 * g.Ez * im;
+  // two-time-step ions (planar only) add onto the acceleration they hold,
+  // with the vacuum field only at their update steps
+  const bool add = e.tts > 0 && SP[gi] == RD_SPECIES_ION;
+  const double ax = pre * sx, ay = pre * sy;
+  const double az = (!add || (e.step % e.tts) == 0) ? pre * sz + q1 * g.Ez * im : pre * sz;
+  AX[gi] = add ? AX[gi] + ax : ax;
+  AY[gi] = add ? AY[gi] + ay : ay;
+  AZ[gi] = add ? AZ[gi] + az : az;
 }
 
 // Packed sphere images of every particle of the tip entries (mixed).
@@ -998,7 +1068,7 @@ int upload_tab(rd_ctx* c, const std::vector<Ent>& ents, Ent** dtab) {
 // form, whose source segments spread one simulation over more workgroups,
 // finishes sooner.
 static bool sym_form(const rd_ctx* c, const Ent& e) {
-  return c->slots[e.slot].desc.g.kind == RD_GEOM_PLANAR && e.n_ub >= kSymMin &&
+  return c->slots[e.slot].desc.g.kind == RD_GEOM_PLANAR && e.n_ub >= kSymMin && e.tts == 0 &&
          !(c->force_opts & RD_FORCE_NO_SYMMETRIC);
 }
 
@@ -1074,11 +1144,11 @@ static int launch_fp64_bands(rd_ctx* c, std::vector<Ent>& ents, bool points, hip
       } else {
         if (any_planar)
           k_accel64<RD_GEOM_PLANAR><<<nwg, kBlock, 0, c->stream>>>(
-              *dlast, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q,
+              *dlast, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, c->species,
               (const double*)c->img64.p, part);
         if (any_tip)
           k_accel64<RD_GEOM_TIP><<<nwg, kBlock, 0, c->stream>>>(
-              *dlast, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q,
+              *dlast, ne, c->d_desc, c->d_state, c->x, c->y, c->z, c->q, c->species,
               (const double*)c->img64.p, part);
       }
     }
@@ -1105,6 +1175,10 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   if (ents.empty()) return RD_OK;
   if (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)
     return fail(RD_ERR_ARG, "unknown precision");
+  if (precision == RD_PREC_MIXED)
+    for (const Ent& e : ents)
+      if (e.tts > 0 && c->slots[e.slot].desc.g.kind == RD_GEOM_TIP)
+        return fail(RD_ERR_ARG, "two-time-step mode on the tip runs in FP64 only");
   // far image partners by expansion where it converges (sets nic_pair)
   std::vector<TailUse> tail;
   int r = tail_decide(c, ents, precision, tail);
@@ -1132,7 +1206,7 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
     if (k1) RD_HIP(hipEventRecord(k1, c->stream));
     k_accel_finish64<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, (double*)c->part.p,
                                                   (const double*)c->acc.p, c->x, c->y, c->z,
-                                                  c->q, c->m, c->ax, c->ay, c->az);
+                                                  c->q, c->m, c->species, c->ax, c->ay, c->az);
     RD_HIP(hipGetLastError());
     return RD_OK;
   }
@@ -1235,18 +1309,29 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
                                                (float4*)c->img_hi.p, (float4*)c->img_lo.p);
   }
   // every entry within partners n <= 1: the pair kernels without the series loop
-  bool near = true;
-  for (const Ent& e : ents) near &= e.nic_pair <= 1;
+  bool near = true, tts = false;
+  for (const Ent& e : ents) {
+    near &= e.nic_pair <= 1;
+    tts |= e.tts > 0;
+  }
   if (k0) RD_HIP(hipEventRecord(k0, c->stream));
   if (any_planar && grid > 0) {
-    if (near)
-      k_accel_planar32<true><<<grid, kBlock, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
-          (const float4*)c->pack_lo.p, part);
-    else
-      k_accel_planar32<false><<<grid, kBlock, 0, c->stream>>>(
-          dt, ne, c->d_desc, c->d_state, (const float4*)c->pack_hi.p,
-          (const float4*)c->pack_lo.p, part);
+    const float4* ph = (const float4*)c->pack_hi.p;
+    const float4* pl = (const float4*)c->pack_lo.p;
+    if (tts) {
+      if (near)
+        k_accel_planar32<true, true><<<grid, kBlock, 0, c->stream>>>(
+            dt, ne, c->d_desc, c->d_state, ph, pl, c->species, part);
+      else
+        k_accel_planar32<false, true><<<grid, kBlock, 0, c->stream>>>(
+            dt, ne, c->d_desc, c->d_state, ph, pl, c->species, part);
+    } else if (near) {
+      k_accel_planar32<true, false><<<grid, kBlock, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, ph, pl, nullptr, part);
+    } else {
+      k_accel_planar32<false, false><<<grid, kBlock, 0, c->stream>>>(
+          dt, ne, c->d_desc, c->d_state, ph, pl, nullptr, part);
+    }
   }
   if (any_tip)
     k_accel_tip32<<<grid, kBlock, 0, c->stream>>>(
@@ -1273,8 +1358,8 @@ int launch_accel(rd_ctx* c, const std::vector<Ent>& in, int precision, hipEvent_
   }
   if (k1) RD_HIP(hipEventRecord(k1, c->stream));
   k_accel_finish32<<<fin, 256, 0, c->stream>>>(dt, c->d_desc, c->d_state, part,
-                                                (const double*)c->acc.p, c->x, c->y, c->z, c->q,
-                                                c->m, c->ax, c->ay, c->az);
+                                                (const double*)c->acc.p, c->species, c->x, c->y,
+                                                c->z, c->q, c->m, c->ax, c->ay, c->az);
   RD_HIP(hipGetLastError());
   if (!tail.empty()) return tail_apply(c, ents, tail);
   return RD_OK;

@@ -347,7 +347,8 @@ int tail_decide(rd_ctx* c, std::vector<Ent>& ents, int precision,
   for (size_t k = 0; k < ents.size(); ++k) {
     const DevGeom& g = c->slots[ents[k].slot].desc.g;
     ents[k].nic_pair = g.nic;
-    if (tail_wanted(c, g, precision) && ents[k].n_ub > 0) cand.push_back((int)k);
+    if (tail_wanted(c, g, precision) && ents[k].n_ub > 0 && ents[k].tts == 0)
+      cand.push_back((int)k);
   }
   if (cand.empty()) return RD_OK;
   // exact particle counts (inside a step the host's may predate the emission)

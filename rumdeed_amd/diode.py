@@ -70,6 +70,11 @@ class Diode:
     def set_time_step(self, dt: float):
         check(self._lib.rd_set_time_step(self._h, c_double(dt)))
 
+    def set_two_time_step(self, atom_time_interval: int):
+        """two_time_step with atom_time_interval K (0: off): ions move every
+        K steps over K time steps (rd_set_two_time_step)."""
+        check(self._lib.rd_set_two_time_step(self._h, int(atom_time_interval)))
+
     def set_logs(self, on: bool = True):
         """Collect each step's emission and absorption particle records."""
         check(self._lib.rd_set_logs(self._h, int(bool(on))))

@@ -59,6 +59,9 @@ module mod_global
   double precision :: T_temp = T_ntp
   logical :: image_charge = .true.
   integer :: N_ic_max = 0
+  ! Two-time-step mode (:201, :363; set from atom_time_interval, main.F90:391-395)
+  integer :: atom_time_interval = 0
+  logical :: two_time_step = .false.
 
   ! Counters (:226-257)
   integer :: nrPart

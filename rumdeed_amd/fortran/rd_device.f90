@@ -75,6 +75,10 @@ contains
     g%box_top = box_dim(3)
     call rd_check(rd_set_geometry(rd_ctx, g), 'rd_set_geometry')
     call rd_check(rd_set_time_step(rd_ctx, time_step), 'rd_set_time_step')
+    ! two_time_step follows atom_time_interval (src/main.F90:391-395)
+    two_time_step = atom_time_interval /= 0
+    call rd_check(rd_set_two_time_step(rd_ctx, int(atom_time_interval, c_int)), &
+                  'rd_set_two_time_step')
     call rd_check(rd_set_seed(rd_ctx, rd_seed, rd_stream), 'rd_set_seed')
     nrPart = 0
     nrElec = 0

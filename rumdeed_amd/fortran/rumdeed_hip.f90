@@ -63,6 +63,12 @@ module rumdeed_hip
       real(c_double), value :: dt
       integer(c_int) :: r
     end function
+    function rd_set_two_time_step(ctx, k) bind(C, name='rd_set_two_time_step') result(r)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      integer(c_int) :: r
+    end function
     function rd_set_seed(ctx, seed, stream) bind(C, name='rd_set_seed') result(r)
       import :: c_int, c_ptr, c_int64_t
       type(c_ptr), value :: ctx

@@ -16,6 +16,10 @@ Fixtures (all SI units, float64):
   tip64.npz         hyperboloid tip, 64 electrons, accelerations and field
   beeman.npz        3 Beeman steps of one electron + removal/compaction scenario
   gtf.npz           Get_Kevin_Jgtf_v2 table
+  tts.npz           the two-time-step integrator (atom_time_interval = 3) over 10
+                    steps of electrons and ions, with removal
+
+    python tests/golden/make_golden.py tts     writes tts.npz only
 """
 from __future__ import annotations
 
@@ -174,8 +178,60 @@ def main():
             out[f"{k}_{s}"] = st[k]
     out["life_time"] = R.life_time()
     np.savez(os.path.join(HERE, "steps.npz"), **out)
+    two_time_step(R)
     print("golden fixtures written to", HERE)
 
 
+def two_time_step(R=None):
+    """Update_Position + Remove_Particles of the reference with two_time_step
+    on and atom_time_interval = 3 (src/mod_verlet.F90:182-193, 234-290,
+    396-419, 886-1151): 48 particles, every sixth an ion, the first one an
+    ion (so that ions also precede electrons in index order), some moving
+    through the cathode or the anode; planar, N_ic_max = 1."""
+    R = R or O.Ref.get()
+    d, V, dt, K = 1000 * NM, 2000.0, 0.25e-15, 3
+    R.setup_planar(d, V, dt=dt, image_charge=True, n_ic_max=1)
+    R.clear()
+    R.set_two_time_step(K)
+    rng = np.random.Generator(np.random.Philox(29))
+    n = 48
+    pos = np.stack([rng.uniform(-40, 40, n), rng.uniform(-40, 40, n),
+                    rng.uniform(5, 995, n)], axis=1) * NM
+    vel = rng.normal(0.0, 2.0e5, (n, 3))
+    pos[:8, 2] = rng.uniform(0.2, 2.0, 8) * NM      # close to the cathode, moving down
+    vel[:8, 2] = -rng.uniform(2e6, 8e6, 8)
+    pos[8:16, 2] = d - rng.uniform(0.2, 2.0, 8) * NM  # close to the anode, moving up
+    vel[8:16, 2] = rng.uniform(2e6, 8e6, 8)
+    perm = rng.permutation(n)
+    pos, vel = pos[perm], vel[perm]
+    species = np.where(np.arange(n) % 6 == 0, 2, 1).astype(np.int32)
+    birth = 1 + (np.arange(n) % 2)
+    for i in range(n):
+        R.add_particle(pos[i], vel[i], species=int(species[i]), step=int(birth[i]), emit=1, sec=1)
+    steps = 10
+    out = dict(d=d, V=V, dt=dt, K=K, pos0=pos, vel0=vel, species0=species,
+               birth0=birth.astype(np.int32), steps=steps)
+    for s in range(1, steps + 1):
+        R.update_position(s)
+        cnt = R.counts()
+        out[f"ramo_{s}"] = R.ramo()
+        out[f"removed_{s}"] = np.array([cnt["nrPart_remove"], cnt["nrPart_remove_top"],
+                                        cnt["nrPart_remove_bot"], cnt["nrElec_remove"],
+                                        cnt["nrIon_remove"]], dtype=np.int32)
+        R.remove_particles(s)
+        cnt = R.counts()
+        st = R.state(cnt["nrPart"])
+        for k in ("pos", "prev_pos", "vel", "acc", "acc_prev", "acc_prev2", "q", "m", "species",
+                  "step", "id"):
+            out[f"{k}_{s}"] = st[k]
+    out["life_time"] = R.life_time()
+    R.set_two_time_step(0)
+    np.savez(os.path.join(HERE, "tts.npz"), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["tts"]:
+        two_time_step()
+        print("tts.npz written to", HERE)
+    else:
+        main()
