@@ -174,7 +174,7 @@ E2E = {
 }
 
 
-def e2e_bench(name, steps, precision, graphs=False):
+def e2e_bench(name, steps, precision, graphs=False, quad_levels=0):
     from rumdeed_amd.diode import Diode
 
     cfg = dict(E2E[name])
@@ -191,11 +191,11 @@ def e2e_bench(name, steps, precision, graphs=False):
     dd.set_seed(cfg["seed"])
     if cfg["mode"] == 10:
         dd.set_emitter(10, 0, -cfg["side"] / 2, -cfg["side"] / 2, cfg["side"], cfg["side"],
-                       wf=np.array([[cfg["phi"]]]))
+                       wf=np.array([[cfg["phi"]]]), quad_levels=quad_levels)
     elif cfg["mode"] == 9:
         wf = np.where((np.arange(10)[:, None] + np.arange(10)[None, :]) % 2 == 0, 4.0, 4.5)
         dd.set_emitter(9, 0, -cfg["side"] / 2, -cfg["side"] / 2, cfg["side"], cfg["side"],
-                       T=cfg["T"], wf=wf)
+                       T=cfg["T"], wf=wf, quad_levels=quad_levels)
     else:
         dd.set_emitter(3, 0, w_tip=cfg["phi"])
     t0 = time.perf_counter()
@@ -221,11 +221,14 @@ def e2e_bench(name, steps, precision, graphs=False):
         force_err = {"n": int(dd.n), "max_rel": float(rel.max()),
                      "rms_rel": float(np.sqrt(np.mean(rel ** 2)))}
     out = {"workload": name, "precision": precision, "graphs": bool(graphs),
+           "supply_quad_levels": quad_levels,
            "steps": cfg["steps"],
            "wall_s": wall, "ms_per_step": wall / cfg["steps"] * 1e3,
            "force_pair_interactions_per_s": pairs / wall,
            "electrons_final": recs[-1].n, "electrons_max": max(r.n for r in recs),
            "emitted_total": sum(r.emitted for r in recs),
+           "supply_levels": {str(k): int(v) for k, v in
+                             zip(*np.unique([r.supply_level for r in recs], return_counts=True))},
            "absorbed_total": sum(r.absorbed for r in recs),
            "ramo_current_second_half_A": float(np.mean([r.ramo_current for r in half])),
            "final_accel_mixed_vs_fp64": force_err,
@@ -336,6 +339,8 @@ def main():
     ap.add_argument("--e2e", default="", help="end-to-end workload: " + ", ".join(E2E))
     ap.add_argument("--e2e-steps", type=int, default=0)
     ap.add_argument("--e2e-precision", default="mixed")
+    ap.add_argument("--e2e-quad-levels", type=int, default=0,
+                    help="refinement doublings of the planar supply rule (-1: the fixed rule)")
     ap.add_argument("--ensemble", default="",
                     help="ensemble-mean current, mixed vs FP64: " + ", ".join(E2E))
     ap.add_argument("--ensemble-seeds", type=int, default=8)
@@ -371,7 +376,7 @@ def main():
         return
     if args.e2e:
         print(json.dumps({"e2e": e2e_bench(args.e2e, args.e2e_steps, args.e2e_precision,
-                                           args.graphs)}), flush=True)
+                                           args.graphs, args.e2e_quad_levels)}), flush=True)
         return
     if args.ensemble:
         print(json.dumps({"ensemble": ensemble_bench(args.ensemble, args.ensemble_seeds,

@@ -135,6 +135,7 @@ __global__ void k_reset_counters(const Ent* tab, SimState* st) {
   s.emitted = 0;
   s.n_round = 0;
   s.rem_total = s.rem_top = s.rem_bot = s.rem_elec = s.rem_ion = s.rem_top_emit = 0;
+  s.first_dead = 0x7fffffff;
   s.pending = 0;
   s.start_rounds = 0;
   s.q_level = 0;
@@ -1519,11 +1520,12 @@ __global__ void k_add_commit(int slot, const SimDesc* desc, SimState* st, int n)
 // ---- Beeman position update + boundary marking ---------------------------
 __device__ __forceinline__ void mark_remove(SimState& S, int32_t* ALIVE, double* Q,
                                             const int32_t* SP, const int32_t* EMIT, int gi,
-                                            int reason, double* mark, double vx, double vy,
-                                            double vz) {
+                                            int li, int reason, double* mark, double vx,
+                                            double vy, double vz) {
   if (!ALIVE[gi]) return;
   ALIVE[gi] = 0;
   Q[gi] = 0.0;
+  atomicMin(&S.first_dead, li);
   if (mark) {  // the velocity at marking, for the absorption record
     mark[4 * (size_t)gi + 0] = vx;
     mark[4 * (size_t)gi + 1] = vy;
@@ -1607,12 +1609,12 @@ __global__ void k_position(const Ent* tab, const SimDesc* desc, SimState* st, do
     }
   }
   if (z < 0.0)
-    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2, mark, vx, vy, vz);
+    mark_remove(S, ALIVE, Q, SP, EMIT, gi, i, 2, mark, vx, vy, vz);
   else if (z > s.g.box_top)
-    mark_remove(S, ALIVE, Q, SP, EMIT, gi, 1, mark, vx, vy, vz);
+    mark_remove(S, ALIVE, Q, SP, EMIT, gi, i, 1, mark, vx, vy, vz);
   if (s.g.kind == RD_GEOM_TIP) {
     double eta = tip_eta(tip_surf(s.g), x, y, z);
-    if (eta < s.g.eta_1) mark_remove(S, ALIVE, Q, SP, EMIT, gi, 2, mark, vx, vy, vz);
+    if (eta < s.g.eta_1) mark_remove(S, ALIVE, Q, SP, EMIT, gi, i, 2, mark, vx, vy, vz);
   }
 }
 
@@ -1756,37 +1758,17 @@ __global__ __launch_bounds__(1024) void k_compact(
   if (S.rem_total == 0) return;
   const int n = S.n;
   const int base = e.base;
+  // everything before the chunk of the first marked particle stays in place
+  // (First_Dead_Index, src/mod_pair.F90:570-582)
+  const int start = (min(S.first_dead, n) / 1024) * 1024;
   if (threadIdx.x == 0) {
-    s_off = 0;
+    s_off = start;
     s_roff = 0;
   }
   __syncthreads();
-  for (int cb = 0; cb < n; cb += 1024) {
+  for (int cb = start; cb < n; cb += 1024) {
     const int i = cb + threadIdx.x;
-    int flag = 0;
-    double r[21];
-    int ri[5];
-    if (i < n) {
-      const int gi = base + i;
-      flag = ALIVE[gi] ? 1 : 0;
-      if (flag) {
-        r[0] = X[gi]; r[1] = Y[gi]; r[2] = Z[gi];
-        r[3] = PX[gi]; r[4] = PY[gi]; r[5] = PZ[gi];
-        r[6] = VX[gi]; r[7] = VY[gi]; r[8] = VZ[gi];
-        r[9] = AX[gi]; r[10] = AY[gi]; r[11] = AZ[gi];
-        r[12] = A1X[gi]; r[13] = A1Y[gi]; r[14] = A1Z[gi];
-        r[15] = A2X[gi]; r[16] = A2Y[gi]; r[17] = A2Z[gi];
-        r[18] = Q[gi]; r[19] = M[gi];
-        ri[0] = SP[gi]; ri[1] = STEP[gi]; ri[2] = EMIT[gi]; ri[3] = SEC[gi]; ri[4] = ID[gi];
-      } else {
-        // record_lifetime (src/mod_pair.F90:1135-1160)
-        int lt = e.step - STEP[gi];
-        if (lt <= 0) lt = 1;
-        if (lt > kLife) lt = kLife;
-        int spc = SP[gi];
-        if (spc >= 1 && spc <= 2) atomicAdd(&life[((size_t)e.slot * 2 + (spc - 1)) * kLife + (lt - 1)], 1);
-      }
-    }
+    const int flag = (i < n && ALIVE[base + i]) ? 1 : 0;
     // rank among the chunk's survivors: a ballot per wavefront, then the
     // survivors of the lower wavefronts
     const unsigned long long mask = __ballot(flag);
@@ -1810,6 +1792,32 @@ __global__ __launch_bounds__(1024) void k_compact(
       rtotal += rsum[u];
     }
     const int off = s_off, roff = s_roff;
+    // Up to the first removed particle nothing moves (First_Dead_Index,
+    // src/mod_pair.F90:570-582): such a chunk is only counted.  Uniform over
+    // the workgroup, like the barriers inside.
+    const bool moves = !(off == cb && total == min(1024, n - cb));
+    double r[21];
+    int ri[5];
+    if (moves && flag) {
+      const int gi = base + i;
+      r[0] = X[gi]; r[1] = Y[gi]; r[2] = Z[gi];
+      r[3] = PX[gi]; r[4] = PY[gi]; r[5] = PZ[gi];
+      r[6] = VX[gi]; r[7] = VY[gi]; r[8] = VZ[gi];
+      r[9] = AX[gi]; r[10] = AY[gi]; r[11] = AZ[gi];
+      r[12] = A1X[gi]; r[13] = A1Y[gi]; r[14] = A1Z[gi];
+      r[15] = A2X[gi]; r[16] = A2Y[gi]; r[17] = A2Z[gi];
+      r[18] = Q[gi]; r[19] = M[gi];
+      ri[0] = SP[gi]; ri[1] = STEP[gi]; ri[2] = EMIT[gi]; ri[3] = SEC[gi]; ri[4] = ID[gi];
+    }
+    if (gone) {
+      // record_lifetime (src/mod_pair.F90:1135-1160)
+      const int gi = base + i;
+      int lt = e.step - STEP[gi];
+      if (lt <= 0) lt = 1;
+      if (lt > kLife) lt = kLife;
+      int spc = SP[gi];
+      if (spc >= 1 && spc <= 2) atomicAdd(&life[((size_t)e.slot * 2 + (spc - 1)) * kLife + (lt - 1)], 1);
+    }
     if (gone && alog) {  // in index order, before any particle moves
       const int gi = base + i;
       rd_absorb_record a;
@@ -1826,7 +1834,7 @@ __global__ __launch_bounds__(1024) void k_compact(
       alog[base + roff + rrank] = a;
     }
     __syncthreads();  // every read of this chunk is done before any write
-    if (flag) {
+    if (moves && flag) {
       const int d = base + off + rank;
       X[d] = r[0]; Y[d] = r[1]; Z[d] = r[2];
       PX[d] = r[3]; PY[d] = r[4]; PZ[d] = r[5];

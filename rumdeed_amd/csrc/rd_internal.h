@@ -88,6 +88,7 @@ struct SimState {
   double ramo[3];   // Ramo current per species (A)
   double vsum[9];   // velocity sums: all, electrons, ions
   int plane_n[RD_MAX_PLANES];  // plane crossings of the last step
+  int first_dead;     // lowest index marked for removal since the counters' reset
   double qsum[2][4];  // supply sums (value, f_x, f_y, f_z) of levels of parity 0 / 1
   int q_final;        // the supply's last computed level is its result
   int q_level;        // that level

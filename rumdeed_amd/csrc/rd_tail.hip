@@ -160,21 +160,33 @@ static int build_plan(rd_ctx* c, double d_m, int nic) {
 }
 
 // Lateral bounding box of one simulation's particles: out = (xmin, xmax,
-// ymin, ymax) in m.
-__global__ __launch_bounds__(256) void k_tail_box(const double* __restrict__ X,
-                                                   const double* __restrict__ Y, int n,
-                                                   double* __restrict__ out) {
-  __shared__ double s[4][256];
+// ymin, ymax) in m.  One workgroup of 1024 lanes, eight independent loads in
+// flight per lane (min and max are exact, so the order does not matter).
+__global__ __launch_bounds__(1024) void k_tail_box(const double* __restrict__ X,
+                                                    const double* __restrict__ Y, int n,
+                                                    double* __restrict__ out) {
+  __shared__ double s[4][1024];
   double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
-  for (int i = threadIdx.x; i < n; i += 256) {
-    v[0] = fmin(v[0], X[i]);
-    v[1] = fmax(v[1], X[i]);
-    v[2] = fmin(v[2], Y[i]);
-    v[3] = fmax(v[3], Y[i]);
+  constexpr int kU = 4;
+  for (int i0 = threadIdx.x; i0 < n; i0 += 1024 * kU) {
+    double xs[kU], ys[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int i = i0 + 1024 * u;
+      xs[u] = i < n ? X[i] : X[i0];
+      ys[u] = i < n ? Y[i] : Y[i0];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      v[0] = fmin(v[0], xs[u]);
+      v[1] = fmax(v[1], xs[u]);
+      v[2] = fmin(v[2], ys[u]);
+      v[3] = fmax(v[3], ys[u]);
+    }
   }
   for (int k = 0; k < 4; ++k) s[k][threadIdx.x] = v[k];
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = 512; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][threadIdx.x + w]);
       s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][threadIdx.x + w]);
@@ -186,71 +198,116 @@ __global__ __launch_bounds__(256) void k_tail_box(const double* __restrict__ X,
   if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
-__device__ __forceinline__ double ipow(double x, int e) {
-  double r = 1.0;
-  for (int k = 0; k < e; ++k) r *= x;
-  return r;
+// Powers u^0 .. u^kTailDeg of the three coordinates of the block's particles,
+// column `lane` of pw (the block's first wavefront fills it).
+__device__ __forceinline__ void tail_powers(double (*pw)[kTailDeg + 1][kTailBlk], int lane,
+                                            double ux, double uy, double uz) {
+  double px = 1.0, py = 1.0, pz = 1.0;
+  for (int k = 0; k <= kTailDeg; ++k) {
+    pw[0][k][lane] = px;
+    pw[1][k][lane] = py;
+    pw[2][k][lane] = pz;
+    px *= ux;
+    py *= uy;
+    pz *= uz;
+  }
 }
 
 // Moments of every block of kTailBlk particles: Mblk[blk][t] =
-// sum_j q_j/q_0 (-w_j)^t, w_j = r_j - centre (nm).
+// sum_j q_j/q_0 (-w_j)^t, w_j = r_j - centre (nm), j in particle order.  The
+// block's powers sit in LDS; padding particles carry charge 0.
 __global__ __launch_bounds__(256) void k_tail_moments(
     const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
     const double* __restrict__ Q, int n, double cx, double cy, double cz,
     const int* __restrict__ terms, double* __restrict__ Mblk) {
-  const int t = blockIdx.y * 256 + threadIdx.x;
-  if (t >= kTailTerms) return;
-  const int pk = terms[t];
-  const int a = pk & 255, b = (pk >> 8) & 255, cc = (pk >> 16) & 255;
-  const int j0 = blockIdx.x * kTailBlk, j1 = min(n, j0 + kTailBlk);
-  double s = 0.0;
-  for (int j = j0; j < j1; ++j) {
-    const double wx = cx - X[j] * 1.0e9, wy = cy - Y[j] * 1.0e9, wz = cz - Z[j] * 1.0e9;
-    s += Q[j] / Consts::q_0 * ipow(wx, a) * ipow(wy, b) * ipow(wz, cc);
+  __shared__ double pw[3][kTailDeg + 1][kTailBlk];
+  __shared__ double sq[kTailBlk];
+  const int j0 = blockIdx.x * kTailBlk;
+  if (threadIdx.x < kTailBlk) {
+    const int j = j0 + threadIdx.x;
+    double wx = 0.0, wy = 0.0, wz = 0.0, qq = 0.0;
+    if (j < n) {
+      wx = cx - X[j] * 1.0e9;
+      wy = cy - Y[j] * 1.0e9;
+      wz = cz - Z[j] * 1.0e9;
+      qq = Q[j] / Consts::q_0;
+    }
+    tail_powers(pw, threadIdx.x, wx, wy, wz);
+    sq[threadIdx.x] = qq;
   }
-  Mblk[(size_t)blockIdx.x * kTailTerms + t] = s;
+  __syncthreads();
+  for (int t = threadIdx.x; t < kTailTerms; t += 256) {
+    const int pk = terms[t];
+    const int a = pk & 255, b = (pk >> 8) & 255, cc = (pk >> 16) & 255;
+    double s = 0.0;
+    for (int jj = 0; jj < kTailBlk; ++jj) s += sq[jj] * pw[0][a][jj] * pw[1][b][jj] * pw[2][cc][jj];
+    Mblk[(size_t)blockIdx.x * kTailTerms + t] = s;
+  }
 }
 
-// Ordered prefix over the blocks: Mpre[blk] = sum of the blocks before, in
-// block order; Mtot = sum of all.
-__global__ __launch_bounds__(256) void k_tail_scan(const double* __restrict__ Mblk, int nblk,
-                                                   double* __restrict__ Mpre,
-                                                   double* __restrict__ Mtot) {
+// Ordered prefix over the blocks, two levels: within groups of kScanGroup
+// blocks (Mpre[blk] = sum of the group's earlier blocks, Gsum[g] = the
+// group's sum) and over the groups (Gpre[g], Mtot); a block's prefix is
+// Gpre[g] + Mpre[blk], a fixed order.
+constexpr int kScanGroup = 32;
+__global__ __launch_bounds__(256) void k_tail_scan_groups(const double* __restrict__ Mblk,
+                                                          int nblk, double* __restrict__ Mpre,
+                                                          double* __restrict__ Gsum) {
+  const int t = blockIdx.y * 256 + threadIdx.x;
+  if (t >= kTailTerms) return;
+  const int g = blockIdx.x;
+  const int b0 = g * kScanGroup, b1 = min(nblk, b0 + kScanGroup);
+  double v[kScanGroup];
+#pragma unroll
+  for (int k = 0; k < kScanGroup; ++k)
+    v[k] = b0 + k < b1 ? Mblk[(size_t)(b0 + k) * kTailTerms + t] : 0.0;
+  double run = 0.0;
+#pragma unroll
+  for (int k = 0; k < kScanGroup; ++k) {
+    if (b0 + k < b1) Mpre[(size_t)(b0 + k) * kTailTerms + t] = run;
+    run += v[k];
+  }
+  Gsum[(size_t)g * kTailTerms + t] = run;
+}
+__global__ __launch_bounds__(256) void k_tail_scan_top(const double* __restrict__ Gsum,
+                                                       int ngroups, double* __restrict__ Gpre,
+                                                       double* __restrict__ Mtot) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= kTailTerms) return;
-  // 16 blocks' loads in flight at a time; the sum itself stays in block order
-  constexpr int kB = 16;
   double run = 0.0;
-  for (int b0 = 0; b0 < nblk; b0 += kB) {
-    double v[kB];
-#pragma unroll
-    for (int k = 0; k < kB; ++k)
-      v[k] = b0 + k < nblk ? Mblk[(size_t)(b0 + k) * kTailTerms + t] : 0.0;
-#pragma unroll
-    for (int k = 0; k < kB; ++k) {
-      if (b0 + k < nblk) Mpre[(size_t)(b0 + k) * kTailTerms + t] = run;
-      run += v[k];
-    }
+  for (int g = 0; g < ngroups; ++g) {
+    const double v = Gsum[(size_t)g * kTailTerms + t];
+    Gpre[(size_t)g * kTailTerms + t] = run;
+    run += v;
   }
   Mtot[t] = run;
 }
 
-// Local expansions: Lpre[blk][beta] from the block prefix (same-charge
-// kernel), L[beta] from all moments (both kernels).  Row blk = nblk is L.
+// Local expansions: Lpre[blk][beta] from the block's prefix moments
+// Gpre[blk / kScanGroup] + Mpre[blk] (same-charge kernel), L[beta] from all
+// moments (both kernels; column blk = nblk).  A wavefront takes one row beta
+// for 64 blocks, one per lane, so the operator's entries (v0, v1, col) are
+// the same for every lane of the row.
 __global__ __launch_bounds__(256) void k_tail_local(
     const double* __restrict__ v0, const double* __restrict__ v1, const int* __restrict__ rowp,
-    const int* __restrict__ col, const double* __restrict__ Mpre, const double* __restrict__ Mtot,
-    int nblk, double* __restrict__ Lpre, double* __restrict__ L) {
-  const int bt = blockIdx.y * 256 + threadIdx.x;
-  if (bt >= kTailTerms) return;
-  const int blk = blockIdx.x;
+    const int* __restrict__ col, const double* __restrict__ Mpre, const double* __restrict__ Gpre,
+    const double* __restrict__ Mtot, int nblk, double* __restrict__ Lpre, double* __restrict__ L) {
+  const int lane = threadIdx.x & 63;
+  const int bt = blockIdx.x * 4 + (threadIdx.x >> 6);  // the row of this wavefront
+  const int blk = blockIdx.y * 64 + lane;               // the block of this lane
+  if (bt >= kTailTerms || blk > nblk) return;
+  const int k0 = rowp[bt], k1 = rowp[bt + 1];
   double s = 0.0;
   if (blk < nblk) {
-    const double* M = Mpre + (size_t)blk * kTailTerms;
-    for (int k = rowp[bt]; k < rowp[bt + 1]; ++k) s += v0[k] * M[col[k]];
+    const double* Mi = Mpre + (size_t)blk * kTailTerms;
+    const double* Gi = Gpre + (size_t)(blk / kScanGroup) * kTailTerms;
+    for (int k = k0; k < k1; ++k) {
+      const int cc = col[k];
+      s += v0[k] * (Gi[cc] + Mi[cc]);
+    }
     Lpre[(size_t)blk * kTailTerms + bt] = s;
   } else {
-    for (int k = rowp[bt]; k < rowp[bt + 1]; ++k) s += (v0[k] - v1[k]) * Mtot[col[k]];
+    for (int k = k0; k < k1; ++k) s += (v0[k] - v1[k]) * Mtot[col[k]];
     L[bt] = s;
   }
 }
@@ -268,39 +325,38 @@ __device__ __forceinline__ double far_kz(double dx, double dy, double dz, double
   return s;
 }
 
-// One wavefront per block: the far part of every particle's acceleration,
-// added to AX, AY, AZ.
-__global__ __launch_bounds__(kTailBlk) void k_tail_apply(
+// The far part of every particle's acceleration, added to AX, AY, AZ: one
+// block of kTailBlk particles per workgroup of kTailWaves wavefronts; lane l
+// of every wavefront takes particle l, wavefront w the terms t = w mod
+// kTailWaves and the block's earlier particles j = w mod kTailWaves (the
+// reference's index-ordered part), and the wavefronts' sums are added in
+// wavefront order.
+constexpr int kTailWaves = 8;
+__global__ __launch_bounds__(kTailWaves * kTailBlk) void k_tail_apply(
     const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
     const double* __restrict__ Q, const double* __restrict__ Ms, int n, double cx, double cy,
     double cz, double dnm, int nic, double dfc, const int* __restrict__ terms,
     const double* __restrict__ L, const double* __restrict__ Lpre, double* __restrict__ AX,
     double* __restrict__ AY, double* __restrict__ AZ) {
   __shared__ double pw[3][kTailDeg + 1][kTailBlk];
-  const int lane = threadIdx.x;
+  __shared__ double red[5][kTailWaves][kTailBlk];
+  const int lane = threadIdx.x & (kTailBlk - 1);
+  const int w = threadIdx.x / kTailBlk;
   const int blk = blockIdx.x;
   const int i = blk * kTailBlk + lane;
   const bool act = i < n;
-  double ux = 0.0, uy = 0.0, uz = 0.0;
+  double xi = 0.0, yi = 0.0, zi = 0.0;
   if (act) {
-    ux = X[i] * 1.0e9 - cx;
-    uy = Y[i] * 1.0e9 - cy;
-    uz = Z[i] * 1.0e9 - cz;
+    xi = X[i] * 1.0e9;
+    yi = Y[i] * 1.0e9;
+    zi = Z[i] * 1.0e9;
   }
-  double px = 1.0, py = 1.0, pz = 1.0;
-  for (int k = 0; k <= kTailDeg; ++k) {
-    pw[0][k][lane] = px;
-    pw[1][k][lane] = py;
-    pw[2][k][lane] = pz;
-    px *= ux;
-    py *= uy;
-    pz *= uz;
-  }
+  if (w == 0) tail_powers(pw, lane, xi - cx, yi - cy, zi - cz);
   __syncthreads();
   // -grad of sum_t L_t u^t, and -d/dz of sum_t Lpre_t u^t
   double gx = 0.0, gy = 0.0, gz = 0.0, qz = 0.0;
   const double* Lp = Lpre + (size_t)blk * kTailTerms;
-  for (int t = 0; t < kTailTerms; ++t) {
+  for (int t = w; t < kTailTerms; t += kTailWaves) {
     const int pk = terms[t];
     const int a = pk & 255, b = (pk >> 8) & 255, c = (pk >> 16) & 255;
     const double xa = pw[0][a][lane], yb = pw[1][b][lane], zc = pw[2][c][lane];
@@ -313,18 +369,30 @@ __global__ __launch_bounds__(kTailBlk) void k_tail_apply(
       qz += Lp[t] * dzt;
     }
   }
-  if (!act) return;
-  const double two_d = 2.0 * dnm;
-  const double xi = X[i] * 1.0e9, yi = Y[i] * 1.0e9, zi = Z[i] * 1.0e9;
   // earlier particles of this block, directly
+  const double two_d = 2.0 * dnm;
   double qd = 0.0;
-  for (int j = blk * kTailBlk; j < i; ++j)
-    qd += Q[j] / Consts::q_0 * far_kz(xi - X[j] * 1.0e9, yi - Y[j] * 1.0e9, zi - Z[j] * 1.0e9,
-                                       two_d, nic);
+  if (act)
+    for (int j = blk * kTailBlk + w; j < i; j += kTailWaves)
+      qd += Q[j] / Consts::q_0 *
+            far_kz(xi - X[j] * 1.0e9, yi - Y[j] * 1.0e9, zi - Z[j] * 1.0e9, two_d, nic);
+  red[0][w][lane] = gx;
+  red[1][w][lane] = gy;
+  red[2][w][lane] = gz;
+  red[3][w][lane] = qz;
+  red[4][w][lane] = qd;
+  __syncthreads();
+  if (w != 0 || !act) return;
+  double sum[5];
+  for (int k = 0; k < 5; ++k) {
+    double v = red[k][0][lane];
+    for (int ww = 1; ww < kTailWaves; ++ww) v += red[k][ww][lane];
+    sum[k] = v;
+  }
   const double qi = Q[i] / Consts::q_0;
   const double self_z = qi * far_kz(0.0, 0.0, 2.0 * zi, two_d, nic);
-  const double tx = -gx, ty = -gy;
-  const double tz = -gz + self_z - 2.0 * (-qz + qd);
+  const double tx = -sum[0], ty = -sum[1];
+  const double tz = -sum[2] + self_z - 2.0 * (-sum[3] + sum[4]);
   // acceleration: dfc q_i q_0 / m_i times T (in (q/q_0) / nm^2 -> 1e18 / m^2)
   const double pre = dfc * Q[i] * Consts::q_0 * 1.0e18 / Ms[i];
   AX[i] += pre * tx;
@@ -361,7 +429,7 @@ int tail_decide(rd_ctx* c, std::vector<Ent>& ents, int precision,
     const Ent& e = ents[cand[k]];
     const int n = c->slots[e.slot].n;
     if (n < 1) continue;
-    k_tail_box<<<1, 256, 0, c->stream>>>(c->x + e.base, c->y + e.base, n, box + 4 * k);
+    k_tail_box<<<1, 1024, 0, c->stream>>>(c->x + e.base, c->y + e.base, n, box + 4 * k);
   }
   RD_HIP(hipGetLastError());
   std::vector<double> h(4 * cand.size(), 0.0);
@@ -407,24 +475,28 @@ int tail_apply(rd_ctx* c, const std::vector<Ent>& ents, const std::vector<TailUs
     const int* rowp = terms + T;
     const int* col = rowp + T + 1;
     const int nblk = (n + kTailBlk - 1) / kTailBlk;
-    r = ensure(c, c->tail_work, sizeof(double) * (size_t)T * (3 * (size_t)nblk + 2));
+    const int ngroups = (nblk + kScanGroup - 1) / kScanGroup;
+    r = ensure(c, c->tail_work,
+               sizeof(double) * (size_t)T * (3 * (size_t)nblk + 2 * (size_t)ngroups + 2));
     if (r) return r;
     double* Mblk = (double*)c->tail_work.p;
     double* Mpre = Mblk + (size_t)nblk * T;
     double* Lpre = Mpre + (size_t)nblk * T;
-    double* Mtot = Lpre + (size_t)nblk * T;
+    double* Gsum = Lpre + (size_t)nblk * T;
+    double* Gpre = Gsum + (size_t)ngroups * T;
+    double* Mtot = Gpre + (size_t)ngroups * T;
     double* L = Mtot + T;
     const int tb = (T + 255) / 256;
     const double* x = c->x + e.base;
     const double* y = c->y + e.base;
     const double* z = c->z + e.base;
     const double* q = c->q + e.base;
-    k_tail_moments<<<dim3(nblk, tb), 256, 0, c->stream>>>(x, y, z, q, n, u.cx, u.cy, u.cz, terms,
-                                                          Mblk);
-    k_tail_scan<<<tb, 256, 0, c->stream>>>(Mblk, nblk, Mpre, Mtot);
-    k_tail_local<<<dim3(nblk + 1, tb), 256, 0, c->stream>>>(v0, v1, rowp, col, Mpre, Mtot, nblk,
-                                                            Lpre, L);
-    k_tail_apply<<<nblk, kTailBlk, 0, c->stream>>>(
+    k_tail_moments<<<nblk, 256, 0, c->stream>>>(x, y, z, q, n, u.cx, u.cy, u.cz, terms, Mblk);
+    k_tail_scan_groups<<<dim3(ngroups, tb), 256, 0, c->stream>>>(Mblk, nblk, Mpre, Gsum);
+    k_tail_scan_top<<<tb, 256, 0, c->stream>>>(Gsum, ngroups, Gpre, Mtot);
+    k_tail_local<<<dim3((T + 3) / 4, (nblk + 1 + 63) / 64), 256, 0, c->stream>>>(
+        v0, v1, rowp, col, Mpre, Gpre, Mtot, nblk, Lpre, L);
+    k_tail_apply<<<nblk, kTailWaves * kTailBlk, 0, c->stream>>>(
         x, y, z, q, c->m + e.base, n, u.cx, u.cy, u.cz, g.d * 1.0e9, g.nic, g.dfc, terms, L, Lpre,
         c->ax + e.base, c->ay + e.base, c->az + e.base);
     RD_HIP(hipGetLastError());
