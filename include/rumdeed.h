@@ -277,6 +277,15 @@ int rd_set_graphs(rd_ctx* ctx, int32_t on);
  * threshold.  Library control, no reference counterpart. */
 int rd_set_jump_pairing(rd_ctx* ctx, int64_t min_workgroups);
 
+/* Mixed-precision jump loops of small simulations (on != 0, the default): a
+ * simulation whose chains x electrons per jump are few (by its own sizes
+ * only) runs all its Metropolis-Hastings jumps in one workgroup of one
+ * launch instead of two launches per jump.  The chain fields add the same
+ * FP32 pair terms in another fixed order, so the fields agree to rounding
+ * and the chains take the same sites unless an accept test falls within
+ * that rounding.  Library control, no reference counterpart. */
+int rd_set_solo_jumps(rd_ctx* ctx, int32_t on);
+
 /* Forms of the mixed-precision planar accelerations (flags, 0 = both on,
  * the default).  RD_FORCE_NO_SYMMETRIC: every simulation takes the gather
  * form (each ordered pair evaluated at its target) instead of the

@@ -136,6 +136,7 @@ _SIGS = {
     "rd_time_copy": (c_int32, [c_void_p, c_int64, c_int32, _dp]),
     "rd_set_graphs": (c_int32, [c_void_p, c_int32]),
     "rd_set_jump_pairing": (c_int32, [c_void_p, c_int64]),
+    "rd_set_solo_jumps": (c_int32, [c_void_p, c_int32]),
     "rd_set_logs": (c_int32, [c_void_p, c_int32]),
     "rd_get_logs": (c_int32, [c_void_p, c_int32, c_void_p, _ip, c_int32, c_void_p, _ip]),
     "rd_set_planes": (c_int32, [c_void_p, c_int32, c_void_p]),

@@ -385,7 +385,7 @@ int rd_ctx_destroy(rd_ctx* c) {
   Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pack_far, &c->pack_srf, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
                  &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo, &c->cnt,
                  &c->mark, &c->elog, &c->alog, &c->plog, &c->tail_plan, &c->tail_work,
-                 &c->tail_box};
+                 &c->tail_box, &c->acc};
   for (Buf* b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -395,6 +395,8 @@ int rd_ctx_destroy(rd_ctx* c) {
     if (c->ev_join[h]) hipEventDestroy(c->ev_join[h]);
     if (c->jstream[h]) hipStreamDestroy(c->jstream[h]);
   }
+  if (c->ev_solo) hipEventDestroy(c->ev_solo);
+  if (c->sstream) hipStreamDestroy(c->sstream);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return RD_OK;
@@ -1087,6 +1089,12 @@ int rd_set_jump_pairing(rd_ctx* c, int64_t min_workgroups) {
   if (!c) return fail(RD_ERR_ARG, "null context");
   if (min_workgroups < 0) return fail(RD_ERR_ARG, "negative workgroup threshold");
   c->jump_pair_wgs = min_workgroups;
+  return RD_OK;
+}
+
+int rd_set_solo_jumps(rd_ctx* c, int32_t on) {
+  if (!c) return fail(RD_ERR_ARG, "null context");
+  c->solo_jumps = on != 0;
   return RD_OK;
 }
 

@@ -1177,6 +1177,161 @@ __global__ void k_mh_width_last(const Ent* tab, const SimDesc* desc, SimState* s
   fe_width_update(st[e.slot], s.ndim, nint(s.ndim * 0.25), na, nr);
 }
 
+// ------------------------------------------------------------------------
+// Every jump of a small planar simulation in one workgroup (mixed precision).
+//
+// A lone small simulation (a few chains over a few hundred electrons) spends
+// two launches per jump on a few microseconds of pair terms: the launches,
+// not the terms, set its step time.  Here one workgroup of kSoloThreads lanes
+// takes all jumps 1..ndim of one simulation, with barriers between the
+// phases of a jump:
+//   1. lane k < m proposes for chain k (jump_proposal: the same random
+//      numbers, width and reflection as the two-launch path), records the
+//      proposal and packs its point into LDS;
+//   2. the lanes form mp columns (mp: the power of two >= m) and
+//      kSoloThreads / mp rows, lane t taking chain t % mp and the sources
+//      j = row, row + rows, ... of each staged block, with the FP32 pair terms
+//      of the two-launch path added in FP64 (on the cathode with N_ic_max = 0,
+//      FP32 blocks of at most kSurfBlock of the lane's own terms); the rows'
+//      sums are then added pairwise in a fixed tree;
+//   3. lane k < m forms the field (field_from_sums32) and runs the accept
+//      step with the batch width update of mh_accept_block.
+// The sources do not move during the jumps: up to kSoloStage of them stay in
+// LDS for all jumps, more are staged per jump.  Which simulations run here
+// depends on their own chain count, electron count and image series only
+// (solo_jumps), so a simulation gives the same records alone and in a pool.
+// Tip simulations start hundreds of chains or more and keep the two-launch
+// path.
+constexpr int kSoloThreads = 1024;
+constexpr int kSoloStage = 2048;     // sources held in LDS
+constexpr double kSoloWork = 2.0e6;  // most estimated VALU work per jump, in lane-ops
+
+// Estimated lane-ops of one jump's pair terms: chains x electrons x the
+// instructions of one term (surface form with N_ic_max = 0: 12, each image
+// partner pair about 10 more; without images: 25).  At 64 lane-ops per cycle
+// on one CU, kSoloWork is about 13 us per jump, against about 15 us for the
+// two launches of a lone simulation.
+__host__ __device__ inline double solo_work(int ic, int nic, int m, int n) {
+  const double per = ic ? 12.0 + 10.0 * nic : 25.0;
+  return per * (double)m * (double)(n > 1 ? n : 1);
+}
+__host__ __device__ inline bool solo_jumps(int kind, int ic, int nic, int m, int n) {
+  return kind == RD_GEOM_PLANAR && m >= 1 && m <= kSettle && solo_work(ic, nic, m, n) <= kSoloWork;
+}
+
+__global__ __launch_bounds__(kSoloThreads) void k_mh_solo32(const Ent* tab, const SimDesc* desc,
+                                                            SimState* st, const double* wf,
+                                                            Chains ch, JumpArgs a) {
+  __shared__ float4 shi[kSoloStage], slo[kSoloStage];
+  __shared__ float4 tgt[2][kSettle];
+  __shared__ double red[3][kSoloThreads];
+  __shared__ int ra[kSoloThreads / 64], rr[kSoloThreads / 64];
+  const Ent e = tab[blockIdx.x];
+  const SimDesc& s = desc[e.slot];
+  const DevGeom& g = s.g;
+  const int m = e.nch;
+  const int n = st[e.slot].n;
+  const int t = threadIdx.x;
+  int lmp = 0;
+  while ((1 << lmp) < m) ++lmp;
+  const int mp = 1 << lmp, rows = kSoloThreads >> lmp;
+  const int col = t & (mp - 1), row = t >> lmp;
+  const bool act = col < m;
+  const float4* hi = a.HI + e.base;
+  const float4* lo = a.LO + e.base;
+  const int ic = g.ic, nic = g.nic;
+  const float two_d = (float)(2.0 * g.d * 1e9);
+  const float dnm = 0.5f * two_d;
+  const bool resident = n <= kSoloStage;
+  if (resident) {
+    for (int j = t; j < n; j += kSoloThreads) {
+      shi[j] = hi[j];
+      slo[j] = lo[j];
+    }
+  }
+  for (int it = 1; it <= s.ndim; ++it) {
+    // 1. proposals
+    AcceptIn in{};
+    if (t < m) {
+      const Proposal P = jump_proposal(e, s, st, ch, it, t, a);
+      mh_record(e, s, ch, t, P, a.ptx, a.pty, a.ptz);
+      float4 h1, l1;
+      pack1(P.x, P.y, P.z, 0.0, g.d, h1, l1);
+      tgt[0][t] = h1;
+      tgt[1][t] = l1;
+      in = accept_inputs(e, s, ch, a.rlr + (size_t)(it - 1) * a.C, a.ptx, a.pty, a.ptz, t);
+    }
+    __syncthreads();
+    // 2. pair terms
+    float4 h1 = make_float4(0, 0, 0, 0), l1 = make_float4(0, 0, 0, 0);
+    if (act) {
+      h1 = tgt[0][col];
+      l1 = tgt[1][col];
+    }
+    const bool surf = wave_on_surface(ic, act, h1, l1);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int jb = 0; jb < n; jb += kSoloStage) {
+      if (!resident) {
+        __syncthreads();
+        for (int j = t; j < kSoloStage && jb + j < n; j += kSoloThreads) {
+          shi[j] = hi[jb + j];
+          slo[j] = lo[jb + j];
+        }
+        __syncthreads();
+      }
+      const int cnt = min(kSoloStage, n - jb);
+      if (!act) continue;
+      if (surf) {  // fz holds the sum without the factor -2 until the end
+        if (nic == 0) {
+          for (int j0 = row; j0 < cnt; j0 += rows * kSurfBlock) {
+            const int je = min(cnt, j0 + rows * kSurfBlock);
+            float acc = 0.f;
+            for (int jj = j0; jj < je; jj += rows)
+              acc = surface_acc32(h1, l1, shi[jj], slo[jj], acc);
+            fz += (double)acc;
+          }
+        } else {
+          for (int jj = row; jj < cnt; jj += rows)
+            fz += (double)surface_term32<false>(h1, l1, shi[jj], slo[jj], nic, dnm);
+        }
+      } else if (ic) {
+        for (int jj = row; jj < cnt; jj += rows)
+          mixed_pair(h1, l1, shi[jj], slo[jj], 1, nic, two_d, fx, fy, fz);
+      } else {
+        for (int jj = row; jj < cnt; jj += rows)
+          mixed_pair(h1, l1, shi[jj], slo[jj], 0, 0, two_d, fx, fy, fz);
+      }
+    }
+    if (surf) fz = -2.0 * fz;
+    red[0][t] = fx;
+    red[1][t] = fy;
+    red[2][t] = fz;
+    // rows added pairwise: row r takes row r + h for h = rows/2, rows/4, ..., 1
+    for (int h = rows >> 1; h > 0; h >>= 1) {
+      __syncthreads();
+      if (row < h) {
+        const int u = t + (h << lmp);
+        red[0][t] += red[0][u];
+        red[1][t] += red[1][u];
+        red[2][t] += red[2][u];
+      }
+    }
+    __syncthreads();
+    // 3. field and accept (lane t < m holds chain t: row 0)
+    double fxk = 0.0, fyk = 0.0, fzk = 0.0;
+    if (t < m) {
+      field_from_sums32(g, red[0][t], red[1][t], red[2][t], fxk, fyk, fzk);
+      const int p = e.pbase + t;
+      a.fx[p] = fxk;
+      a.fy[p] = fyk;
+      a.fz[p] = fzk;
+    }
+    mh_accept_block(e, s, st, wf, ch, it, in, fxk, fyk, fzk, ra, rr, t < m ? t : -1, nullptr);
+    // the width and the chains' states are read by the next jump's proposals
+    __syncthreads();
+  }
+}
+
 // Per-chain width updates (thermal-field, tip) in chain order and the escape
 // quantity of every chain.
 __global__ __launch_bounds__(256) void k_mh_finish(const Ent* tab, const SimDesc* desc,
@@ -2049,10 +2204,10 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
                      const Chains& V, int maxndim) {
   const bool fp64 = precision == RD_PREC_FP64;
   const int per = fp64 ? kBlock : kJumpTargets;  // chains per workgroup
-  std::vector<Ent> ents;
+  std::vector<Ent> all;
   for (const Ent& e : in)
-    if (e.nch > 0) ents.push_back(e);
-  if (ents.empty()) return RD_OK;
+    if (e.nch > 0) all.push_back(e);
+  if (all.empty()) return RD_OK;
   JumpArgs a;
   std::memset(&a, 0, sizeof(a));  // its bytes form part of the graph key
   double* ptx = (double*)c->pts.p;
@@ -2069,8 +2224,18 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   a.Q = c->q;
   int r = RD_OK;
   if (!fp64) {
-    r = launch_pack(c, ents, true);
+    r = launch_pack(c, all, true);
     if (r) return r;
+  }
+  // small simulations take all their jumps in one workgroup (k_mh_solo32),
+  // the others the two launches per jump below
+  std::vector<Ent> ents, solo;
+  for (const Ent& e : all) {
+    const DevGeom& g = c->slots[e.slot].desc.g;
+    if (!fp64 && c->solo_jumps && solo_jumps(g.kind, g.ic, g.nic, e.nch, e.n_ub))
+      solo.push_back(e);
+    else
+      ents.push_back(e);
   }
   a.HI = (const float4*)c->pack_hi.p;
   a.LO = (const float4*)c->pack_lo.p;
@@ -2163,15 +2328,23 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   const size_t nr = (size_t)maxndim * C;
   r = ensure(c, c->rnd, sizeof(double) * 3 * nr);
   if (r) return r;
-  r = ensure(c, c->cnt, sizeof(int) * 4 * (size_t)nsettle_all);
+  r = ensure(c, c->cnt, sizeof(int) * 4 * (size_t)std::max(nsettle_all, 1));
   if (r) return r;
   if (nh > 1 && !c->jstream[0]) {
-    RD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     for (int h = 0; h < kJumpStreams - 1; ++h) {
       RD_HIP(hipStreamCreateWithFlags(&c->jstream[h], hipStreamNonBlocking));
       RD_HIP(hipEventCreateWithFlags(&c->ev_join[h], hipEventDisableTiming));
     }
   }
+  // the solo launch overlaps the two-launch jumps on a stream of its own
+  // (one stream when the loop is captured)
+  const bool solo_own = !solo.empty() && nh > 0 && !c->graphs;
+  if (solo_own && !c->sstream) {
+    RD_HIP(hipStreamCreateWithFlags(&c->sstream, hipStreamNonBlocking));
+    RD_HIP(hipEventCreateWithFlags(&c->ev_solo, hipEventDisableTiming));
+  }
+  if ((nh > 1 || solo_own) && !c->ev_fork)
+    RD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   double* rb = (double*)c->rnd.p;
   a.ru1 = rb;
   a.ru2 = rb + nr;
@@ -2179,8 +2352,13 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   a.C = C;
   a.part = (double*)c->part.p;
   Ent* dall = nullptr;  // every entry, for the random numbers of all chains
-  if (nh > 1) {
-    r = upload_tab(c, ents, &dall);
+  Ent* dsolo = nullptr;
+  if (nh > 1 || !solo.empty()) {
+    r = upload_tab(c, all, &dall);
+    if (r) return r;
+  }
+  if (!solo.empty()) {
+    r = upload_tab(c, solo, &dsolo);
     if (r) return r;
   }
   int cnt_off = 0;
@@ -2194,8 +2372,8 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     cnt_off += 4 * H.nsettle;
     H.stream = h == 0 ? c->stream : c->jstream[h - 1];
   }
-  if (nh == 1) dall = hv[0].dt;
-  const int ne_all = (int)ents.size();
+  if (nh == 1 && solo.empty()) dall = hv[0].dt;
+  const int ne_all = (int)all.size();
   // Everything the launches below depend on besides device data.  With graphs
   // on, an unchanged key replays the captured loop (the kernels read the
   // launch table and the chains from device memory, so new contents under
@@ -2207,8 +2385,9 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     key.insert(key.end(), b, b + bytes);
   };
   const int ints[] = {(int)G.ents.size(), G.nwg, G.nsettle, (int)G.any_split,
-                      (int)G.any_planar, (int)G.any_tip, (int)fp64, maxndim, C};
-  const void* ptrs[] = {G.dt, dall, c->d_desc, c->d_state, c->d_wf};
+                      (int)G.any_planar, (int)G.any_tip, (int)fp64, maxndim, C,
+                      (int)solo.size(), nh};
+  const void* ptrs[] = {G.dt, dall, dsolo, c->d_desc, c->d_state, c->d_wf};
   put(ints, sizeof(ints));
   put(ptrs, sizeof(ptrs));
   put(&G.a, sizeof(G.a));
@@ -2226,9 +2405,16 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
   }
   k_mh_random<<<dim3((C + 255) / 256, maxndim), 256, 0, c->stream>>>(dall, ne_all, c->d_desc, C,
                                                                        rb, rb + nr, rb + 2 * nr);
-  if (nh > 1) {
+  if (nh > 1 || solo_own) {
     RD_HIP(hipEventRecord(c->ev_fork, c->stream));
     for (int h = 1; h < nh; ++h) RD_HIP(hipStreamWaitEvent(hv[h].stream, c->ev_fork, 0));
+    if (solo_own) RD_HIP(hipStreamWaitEvent(c->sstream, c->ev_fork, 0));
+  }
+  if (!solo.empty()) {
+    hipStream_t ss = solo_own ? c->sstream : c->stream;
+    k_mh_solo32<<<(int)solo.size(), kSoloThreads, 0, ss>>>(dsolo, c->d_desc, c->d_state,
+                                                          c->d_wf, V, a);
+    if (solo_own) RD_HIP(hipEventRecord(c->ev_solo, c->sstream));
   }
   for (int it = 1; it <= maxndim; ++it) {
     for (int h = 0; h < nh; ++h) {
@@ -2275,6 +2461,7 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     RD_HIP(hipEventRecord(c->ev_join[h - 1], hv[h].stream));
     RD_HIP(hipStreamWaitEvent(c->stream, c->ev_join[h - 1], 0));
   }
+  if (solo_own) RD_HIP(hipStreamWaitEvent(c->stream, c->ev_solo, 0));
   if (c->graphs) {
     hipGraph_t g = nullptr;
     const hipError_t le = hipGetLastError();

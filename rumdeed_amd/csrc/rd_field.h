@@ -200,23 +200,36 @@ __device__ __forceinline__ void ic_planar32(float zsum, float wsum, float dzab, 
   }
   // suffix sums of 1/r^3: p (a + t_n) and m (a - t_n), o(pposite) and s(ame)
   float po = 0.f, mo = 0.f, ps = 0.f, ms = 0.f, ro = 0.f, rs = 0.f;
-  float tn = (float)nic * two_d;
-  if (!NEAR)
-  for (int n = nic; n >= 2; --n) {
-    RD_PARTNER_SUM(zsum + tn, po);
-    RD_PARTNER_SUM(zsum - tn, mo);
-    RD_PARTNER_SUM(dzab + tn, ps);
-    RD_PARTNER_SUM(dzab - tn, ms);
+  if (NEAR) {
+    // n = 1 alone: the sums start at its terms (the same values as adding
+    // them to zero, without the additions: every term is positive)
+    auto r3 = [dxy2](float dz) {
+      const float ri = rsq(__builtin_fmaf(dz, dz, dxy2));
+      return ri * ri * ri;
+    };
+    po = r3(zsum + two_d);
+    ps = r3(dzab + two_d);
+    ms = r3(dzab - two_d);
+    ro = po;
+    rs = ps - ms;
+  } else {
+    float tn = (float)nic * two_d;
+    for (int n = nic; n >= 2; --n) {
+      RD_PARTNER_SUM(zsum + tn, po);
+      RD_PARTNER_SUM(zsum - tn, mo);
+      RD_PARTNER_SUM(dzab + tn, ps);
+      RD_PARTNER_SUM(dzab - tn, ms);
+      ro = (ro + po) - mo;
+      rs = (rs + ps) - ms;
+      tn -= two_d;
+    }
+    RD_PARTNER_SUM(zsum + two_d, po);
+    RD_PARTNER_SUM(dzab + two_d, ps);
+    RD_PARTNER_SUM(dzab - two_d, ms);
     ro = (ro + po) - mo;
     rs = (rs + ps) - ms;
-    tn -= two_d;
   }
-  RD_PARTNER_SUM(zsum + two_d, po);
-  RD_PARTNER_SUM(dzab + two_d, ps);
-  RD_PARTNER_SUM(dzab - two_d, ms);
-  ro = (ro + po) - mo;
-  rs = (rs + ps) - ms;
-  const float so = po + mo, ss = ps + ms;
+  const float so = NEAR ? po : po + mo, ss = ps + ms;
   wxy = ss - (wo + so);
   wz = __builtin_fmaf(dzab, ss, two_d * rs) - (__builtin_fmaf(zsum, so, wzo) + two_d * ro);
 }
@@ -568,16 +581,23 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
   if (surf) fz = -2.0 * fz;
 }
 
+// Field (V/m) of the planar geometry from the particle sums (sx, sy, sz) of a
+// mixed evaluation (nm^-2 per unit charge).
+__device__ __forceinline__ void field_from_sums32(const DevGeom& g, double sx, double sy,
+                                                  double sz, double& fx, double& fy, double& fz) {
+  double pre = g.dfc * Consts::q_0 * 1.0e18;
+  fx = pre * sx;
+  fy = pre * sy;
+  fz = g.Ez + pre * sz;
+}
+
 // Field (V/m) from the segment sums of a mixed evaluation, in segment order.
 __device__ __forceinline__ void field_finish32(const DevGeom& g, const double* __restrict__ P,
                                                size_t S, int m, int p, int segs, double& fx,
                                                double& fy, double& fz) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
   sum3_in_order(P + p, S, m, segs, sx, sy, sz);
-  double pre = g.dfc * Consts::q_0 * 1.0e18;
-  fx = pre * sx;
-  fy = pre * sy;
-  fz = g.Ez + pre * sz;
+  field_from_sums32(g, sx, sy, sz, fx, fy, fz);
 }
 
 // Field (V/m) of the tip geometry from the segment sums of a mixed

@@ -249,6 +249,9 @@ struct rd_ctx {
   // that fork them from and join them to `stream`
   hipStream_t jstream[rd::kJumpStreams - 1] = {};
   hipEvent_t ev_fork = nullptr, ev_join[rd::kJumpStreams - 1] = {};
+  bool solo_jumps = true;          // small simulations' jumps in one workgroup (mixed)
+  hipStream_t sstream = nullptr;  // one-workgroup jump loops of small simulations
+  hipEvent_t ev_solo = nullptr;
 };
 
 namespace rd {

@@ -556,9 +556,11 @@ constexpr int kSymWaves = 16;
 constexpr int kSymBlock = 64 * kSymWaves;
 constexpr int kSymMin = 16384;
 
-// One lane to the next: lane l takes lane l + 1's value (wave_rol:1).
+// One lane to the next: lane l takes lane l + 1's value (wave_rol:1).  Every
+// lane receives a value, so the old-value operand is v itself: the move can
+// write in place, with no register cleared for it first.
 __device__ __forceinline__ int wave_rotate(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x134, 0xF, 0xF, false);
+  return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false);
 }
 __device__ __forceinline__ double wave_rotate(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);

@@ -139,6 +139,11 @@ class Diode:
         partial sums (results unchanged; 0: always)."""
         check(self._lib.rd_set_jump_pairing(self._h, int(min_workgroups)))
 
+    def set_solo_jumps(self, on: bool):
+        """Mixed precision: small simulations run all their jumps in one
+        workgroup (default on; rd_set_solo_jumps)."""
+        check(self._lib.rd_set_solo_jumps(self._h, int(bool(on))))
+
     def set_seed(self, seed: int, stream: int = 0):
         check(self._lib.rd_set_seed(self._h, int(seed), int(stream)))
 
