@@ -93,11 +93,18 @@ typedef struct {
   int32_t absorbed, absorbed_top, absorbed_bot, dropped;
   int32_t start_rounds;
   int32_t supply_level;  /* planar supply: the level used (quad_n << level points per side) */
-  int32_t reserved1, reserved2;
+  int32_t supply_cells;  /* supply integral: cells of the level used (tip: 100 x 100); 0 when
+                            the step has no emission phase */
+  int32_t supply_fail;   /* planar supply: 1 when the last allowed level still differs from the
+                            one before by more than the tolerance, else 0 */
   double n_sup, ramo_current, mh_std, a_rate, df_avg;
   double f_avg[3];
   double ramo[3];
   double avg_elec_vel[3];
+  double supply_err;     /* planar supply: |I_L - I_(L-1)| of the last two levels (electrons);
+                            0 for the single level of the fixed rule.  With supply_cells and
+                            supply_fail: the line of integration.dt
+                            (src/mod_field_emission_v2.F90:654-655) */
 } rd_step_record;
 
 /* Particle records of one step for RUMDEED's density files (as
@@ -277,13 +284,15 @@ int rd_set_graphs(rd_ctx* ctx, int32_t on);
  * threshold.  Library control, no reference counterpart. */
 int rd_set_jump_pairing(rd_ctx* ctx, int64_t min_workgroups);
 
-/* Mixed-precision jump loops of small simulations (on != 0, the default): a
- * simulation whose chains x electrons per jump are few (by its own sizes
- * only) runs all its Metropolis-Hastings jumps in one workgroup of one
- * launch instead of two launches per jump.  The chain fields add the same
- * FP32 pair terms in another fixed order, so the fields agree to rounding
- * and the chains take the same sites unless an accept test falls within
- * that rounding.  Library control, no reference counterpart. */
+/* Mixed-precision jump loops of small simulations (on != 0, the default): in
+ * a context made for one simulation (rd_ctx_create), a planar simulation whose
+ * chains x electrons per jump are few (by its own sizes) runs all its
+ * Metropolis-Hastings jumps in one workgroup of one launch instead of two
+ * launches per jump; pools keep the two launches, which serve all their
+ * simulations at once.  The chain fields add the same FP32 pair terms in
+ * another fixed order, so the fields agree to rounding and the chains take
+ * the same sites unless an accept test falls within that rounding.  Library
+ * control, no reference counterpart. */
 int rd_set_solo_jumps(rd_ctx* ctx, int32_t on);
 
 /* Forms of the mixed-precision planar accelerations (flags, 0 = both on,

@@ -379,14 +379,14 @@ class Record(ctypes.Structure):
     _fields_ = [("step", c_int32), ("n", c_int32), ("emitted", c_int32), ("n_round", c_int32),
                 ("absorbed", c_int32), ("absorbed_top", c_int32), ("absorbed_bot", c_int32),
                 ("dropped", c_int32), ("start_rounds", c_int32), ("supply_level", c_int32),
-                ("r1", c_int32), ("r2", c_int32),
+                ("supply_cells", c_int32), ("supply_fail", c_int32),
                 ("n_sup", c_double), ("ramo_current", c_double), ("mh_std", c_double),
                 ("a_rate", c_double), ("df_avg", c_double), ("f_avg", c_double * 3),
-                ("ramo", c_double * 3), ("avg_elec_vel", c_double * 3)]
+                ("ramo", c_double * 3), ("avg_elec_vel", c_double * 3), ("supply_err", c_double)]
 
 
 def record_dict(r):
-    d = {k: getattr(r, k) for k, _ in Record._fields_ if not k.startswith("r") or k in ("ramo", "ramo_current")}
+    d = {k: getattr(r, k) for k, _ in Record._fields_}
     for k in ("f_avg", "ramo", "avg_elec_vel"):
         d[k] = list(getattr(r, k))
     return d

@@ -367,7 +367,8 @@ typedef struct {
   double residual, MH_std, a_rate, N_sup, df_avg, F_avg[3], ramo[3], vsum[9];
   int emitted, n_round, rem_total, rem_top, rem_bot, rem_elec, rem_ion, rem_top_emit, dropped;
   int start_rounds, n_elec;
-  int supply_level;
+  int supply_level, supply_cells, supply_fail;
+  double supply_err;
   int life[2][NLIFE];
   int nthreads;
   int serial; /* 1: the reference's serial samplers (emit_serial) */
@@ -403,11 +404,12 @@ typedef struct rdo_plane_rec {
 typedef struct {
   int32_t step, n, emitted, n_round;
   int32_t absorbed, absorbed_top, absorbed_bot, dropped;
-  int32_t start_rounds, supply_level, r1, r2;
+  int32_t start_rounds, supply_level, supply_cells, supply_fail;
   double n_sup, ramo_current, mh_std, a_rate, df_avg;
   double f_avg[3];
   double ramo[3];
   double avg_elec_vel[3];
+  double supply_err;
 } rdo_record;
 
 int rdo_quad_lmax(int g0, int levels);
@@ -695,6 +697,8 @@ static void supply_level(rdo_sim* s, int Gx, int Gy, double tot[4]) {
  * level is the result; *M receives its number of points. */
 static void supply_planar(rdo_sim* s, double tot[4], int* Mout) {
   s->supply_level = 0;
+  s->supply_fail = 0;
+  s->supply_err = 0.0;
   int Gx = s->qgx, Gy = s->qgy;
   supply_level(s, Gx, Gy, tot);
   double prev = tot[0] / ((double)Gx * (double)Gy);
@@ -705,14 +709,18 @@ static void supply_planar(rdo_sim* s, double tot[4], int* Mout) {
     const double cur = t2[0] / ((double)G2x * (double)G2y);
     double tol = 1.0e-3 * fabs(cur);
     if (tol < 0.5) tol = 0.5;
-    const int conv = fabs(cur - prev) <= tol;
+    const double err = fabs(cur - prev);
+    const int conv = err <= tol;
     for (int k = 0; k < 4; ++k) tot[k] = t2[k];
     Gx = G2x;
     Gy = G2y;
     prev = cur;
     s->supply_level = L;
+    s->supply_err = err;
+    s->supply_fail = !conv; /* final unless converged earlier: only the last level's counts */
     if (conv) break;
   }
+  s->supply_cells = Gx * Gy;
   *Mout = Gx * Gy;
 }
 
@@ -726,6 +734,7 @@ static void do_emission(rdo_sim* s, int step) {
   if (tip) {
     M = 100 * 100;
     supply_level(s, 0, 0, tot);
+    s->supply_cells = M;
   } else {
     supply_planar(s, tot, &M);
   }
@@ -1361,6 +1370,9 @@ static void reset_counters(rdo_sim* s) {
   s->rem_total = s->rem_top = s->rem_bot = s->rem_elec = s->rem_ion = s->rem_top_emit = 0;
   s->start_rounds = 0;
   s->supply_level = 0;
+  s->supply_cells = 0;
+  s->supply_fail = 0;
+  s->supply_err = 0.0;
   s->N_sup = 0.0;
   s->df_avg = 0.0;
   s->F_avg[0] = s->F_avg[1] = s->F_avg[2] = 0.0;
@@ -1393,6 +1405,9 @@ void rdo_sim_step(rdo_sim* s, int step, int phases, rdo_record* rec) {
     rec->dropped = s->dropped;
     rec->start_rounds = s->start_rounds;
     rec->supply_level = s->supply_level;
+    rec->supply_cells = s->supply_cells;
+    rec->supply_fail = s->supply_fail;
+    rec->supply_err = s->supply_err;
     rec->n_sup = s->N_sup;
     rec->ramo_current = s->ramo[0] + s->ramo[1] + s->ramo[2];
     rec->mh_std = s->MH_std;

@@ -65,16 +65,14 @@ class StepRecord(ctypes.Structure):
     _fields_ = [("step", c_int32), ("n", c_int32), ("emitted", c_int32), ("n_round", c_int32),
                 ("absorbed", c_int32), ("absorbed_top", c_int32), ("absorbed_bot", c_int32),
                 ("dropped", c_int32), ("start_rounds", c_int32), ("supply_level", c_int32),
-                ("reserved1", c_int32), ("reserved2", c_int32),
+                ("supply_cells", c_int32), ("supply_fail", c_int32),
                 ("n_sup", c_double), ("ramo_current", c_double), ("mh_std", c_double),
                 ("a_rate", c_double), ("df_avg", c_double), ("f_avg", c_double * 3),
-                ("ramo", c_double * 3), ("avg_elec_vel", c_double * 3)]
+                ("ramo", c_double * 3), ("avg_elec_vel", c_double * 3), ("supply_err", c_double)]
 
     def as_dict(self):
         d = {}
         for k, _ in self._fields_:
-            if k.startswith("reserved"):
-                continue
             v = getattr(self, k)
             d[k] = list(v) if k in ("f_avg", "ramo", "avg_elec_vel") else v
         return d

@@ -139,6 +139,9 @@ __global__ void k_reset_counters(const Ent* tab, SimState* st) {
   s.pending = 0;
   s.start_rounds = 0;
   s.q_level = 0;
+  s.q_cells = 0;
+  s.q_fail = 0;
+  s.q_err = 0.0;
   s.N_sup = 0.0;
   s.df_avg = 0.0;
   for (int k = 0; k < 3; ++k) s.F_avg[k] = 0.0;
@@ -295,16 +298,25 @@ __global__ __launch_bounds__(1024) void k_quad_close(const Ent* tab, const SimDe
   const double sum = cur[0];
   const double lambda = tip ? sum : sum / ((double)Gx * (double)Gy);
   if (threadIdx.x == 0) {
-    int fin = 1;
-    if (!tip && e.lev > 0 && e.lev < s.quad_lmax) {
+    int fin = 1, failed = 0;
+    double err = 0.0;
+    if (!tip && e.lev > 0) {
       const int Gpx = s.qgx << (e.lev - 1), Gpy = s.qgy << (e.lev - 1);
       const double prev = S.qsum[(e.lev - 1) & 1][0] / ((double)Gpx * (double)Gpy);
       double tol = 1.0e-3 * fabs(lambda);
       if (tol < 0.5) tol = 0.5;
-      fin = fabs(lambda - prev) <= tol;
+      err = fabs(lambda - prev);
+      const int conv = err <= tol;
+      if (e.lev < s.quad_lmax)
+        fin = conv;
+      else
+        failed = !conv;
     }
     S.q_final = fin;
     S.q_level = e.lev;
+    S.q_cells = tip ? 100 * 100 : Gx * Gy;
+    S.q_fail = failed;
+    S.q_err = err;
     s_final = fin;
   }
   __syncthreads();
@@ -1198,10 +1210,9 @@ __global__ void k_mh_width_last(const Ent* tab, const SimDesc* desc, SimState* s
 //      step with the batch width update of mh_accept_block.
 // The sources do not move during the jumps: up to kSoloStage of them stay in
 // LDS for all jumps, more are staged per jump.  Which simulations run here
-// depends on their own chain count, electron count and image series only
-// (solo_jumps), so a simulation gives the same records alone and in a pool.
-// Tip simulations start hundreds of chains or more and keep the two-launch
-// path.
+// depends on the kind of context (a lone simulation) and on the simulation's
+// own chain count, electron count and image series (solo_jumps).  Tip
+// simulations start hundreds of chains or more and keep the two-launch path.
 constexpr int kSoloThreads = 1024;
 constexpr int kSoloStage = 2048;     // sources held in LDS
 constexpr double kSoloWork = 2.0e6;  // most estimated VALU work per jump, in lane-ops
@@ -2227,12 +2238,17 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     r = launch_pack(c, all, true);
     if (r) return r;
   }
-  // small simulations take all their jumps in one workgroup (k_mh_solo32),
-  // the others the two launches per jump below
+  // A lone small simulation takes all its jumps in one workgroup
+  // (k_mh_solo32); the others, and every simulation of a pool, the two
+  // launches per jump below.  In a pool those launches serve every
+  // simulation at once, and a solo workgroup would hold a CU (and most of its
+  // LDS) for the whole jump phase: measured 7% fewer simulations per hour on
+  // the serving stream.
   std::vector<Ent> ents, solo;
+  const bool lone = c->nslots_max == 1;
   for (const Ent& e : all) {
     const DevGeom& g = c->slots[e.slot].desc.g;
-    if (!fp64 && c->solo_jumps && solo_jumps(g.kind, g.ic, g.nic, e.nch, e.n_ub))
+    if (!fp64 && lone && c->solo_jumps && solo_jumps(g.kind, g.ic, g.nic, e.nch, e.n_ub))
       solo.push_back(e);
     else
       ents.push_back(e);
@@ -2626,6 +2642,9 @@ static void fill_record(rd_ctx* c, int slot, int step, rd_step_record* rec) {
   rec->dropped = S.dropped;
   rec->start_rounds = S.start_rounds;
   rec->supply_level = S.q_level;
+  rec->supply_cells = S.q_cells;
+  rec->supply_fail = S.q_fail;
+  rec->supply_err = S.q_err;
   rec->n_sup = S.N_sup;
   rec->ramo_current = S.ramo[0] + S.ramo[1] + S.ramo[2];
   rec->mh_std = S.MH_std;

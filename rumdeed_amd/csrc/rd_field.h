@@ -349,7 +349,12 @@ __device__ __forceinline__ float surface_term32(const float4 h1, const float4 l1
 // sum to their FP64 total.  Every term of a block has the sign of the
 // charges (z_b >= 0, one species), so a block's FP32 sum is within
 // (kSurfBlock - 1) u = 15 * 2^-24 (< 1e-6) of its exact value; per source it
-// saves the conversion and the FP64 add.
+// saves the conversion and the FP64 add.  The same holds for the paired-image
+// sum with N_ic_max > 0: sum_{k=-nic..nic} u_k / (rho^2 + u_k^2)^(3/2),
+// u_k = z_b + 2kd, is positive for every z_b in (0, d), rho and N_ic_max
+// (the terms n and -n-1 pair off towards (2 nic + 1) z_b / rho^3 at large
+// rho; checked on a grid of z_b and rho for N_ic_max up to 20), so the
+// cathode field's far-series loop adds its terms in the same blocks.
 constexpr int kSurfBlock = 16;
 __device__ __forceinline__ float surface_acc32(const float4 h1, const float4 l1, const float4 h2,
                                                const float4 l2, float acc) {
@@ -530,18 +535,23 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
     const int cnt = min(kChunk, n - jb);
     if (act && surf) {  // fz holds the sum without the factor -2 until the end
       if (nic > 0 && far)
-        for (int jj = 0; jj < cnt; ++jj) {
-          const float4 h2 = shi[jj], l2 = slo[jj];
-          const float dx = (h1.x - h2.x) + (l1.x - l2.x);
-          const float dy = (h1.y - h2.y) + (l1.y - l2.y);
-          const float s = __builtin_fmaf(dy, dy, dx * dx);
-          const float ri = rsq(fmaxf(__builtin_fmaf(h2.z, h2.z, s), kR2Min));
-          float w = h2.z * (ri * ri * ri);
-          if (s <= slim)
-            w += surface_far_eval(s * inv_d2, sfar[2 * jj], sfar[2 * jj + 1]);
-          else
-            w = surface_far32(h2.z, l2.w, s, nic, dnm, w);
-          fz += (double)(h2.w * w);
+        for (int jb2 = 0; jb2 < cnt; jb2 += kSurfBlock) {
+          const int je2 = min(cnt, jb2 + kSurfBlock);
+          float acc = 0.f;
+          for (int jj = jb2; jj < je2; ++jj) {
+            const float4 h2 = shi[jj], l2 = slo[jj];
+            const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+            const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+            const float s = __builtin_fmaf(dy, dy, dx * dx);
+            const float ri = rsq(fmaxf(__builtin_fmaf(h2.z, h2.z, s), kR2Min));
+            float w = h2.z * (ri * ri * ri);
+            if (s <= slim)
+              w += surface_far_eval(s * inv_d2, sfar[2 * jj], sfar[2 * jj + 1]);
+            else
+              w = surface_far32(h2.z, l2.w, s, nic, dnm, w);
+            acc = __builtin_fmaf(h2.w, w, acc);
+          }
+          fz += (double)acc;
         }
       else if (nic == 0)
         for (int jb2 = 0; jb2 < cnt; jb2 += kSurfBlock) {

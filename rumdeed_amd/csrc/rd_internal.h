@@ -92,6 +92,9 @@ struct SimState {
   double qsum[2][4];  // supply sums (value, f_x, f_y, f_z) of levels of parity 0 / 1
   int q_final;        // the supply's last computed level is its result
   int q_level;        // that level
+  int q_cells;        // its cells
+  int q_fail;         // the last allowed level did not meet the tolerance
+  double q_err;       // |difference| of the last two levels (electrons)
 };
 
 // One launch-table entry (per active slot and launch).

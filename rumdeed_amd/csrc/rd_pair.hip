@@ -660,45 +660,81 @@ __global__ __launch_bounds__(kSymBlock) void k_accel_sym32(const Ent* tab, int n
     }
     return;
   }
-  for (int ph = 0; ph < kSymWaves; ++ph) {
-    const int tb = ((w + ph) & (kSymWaves - 1)) * 64;  // the tile of this wavefront
-    int jl = lane;  // the source whose reaction sum this lane holds
-    double gx = jsum[0][tb + lane], gy = jsum[1][tb + lane], gz = jsum[2][tb + lane];
-    for (int step = 0; step < 64; ++step) {
-      const float4 h2 = shi[tb + jl], l2 = slo[tb + jl];
-      float dx = (h1.x - h2.x) + (l1.x - l2.x);
-      float dy = (h1.y - h2.y) + (l1.y - l2.y);
-      float dz = (h1.z - h2.z) + (l1.z - l2.z);
-      float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
-      float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
-      float ri = rsq(r2);
-      float r3 = ri * ri * ri;
-      float cxy = r3, czi = dz * r3, czj = -czi;
-      if (ic) {
-        float wxy, wz;
-        ic_planar32<NEAR>(h1.z + h2.z, l1.w + l2.w, dz, dxy2, nic, two_d, wxy, wz);
-        cxy += wxy;
-        czi += wz;
-        czj += wz;
+  // Electrons only (every q/q_0 of both blocks exactly -1, uniform over the
+  // workgroup): a product with -1 is a sign, so each term is formed once for
+  // the pair and added to the target's sum and subtracted from the source's
+  // (or the reverse) -- the same roundings as multiplying by the charges.
+  // The image series' switches (ic; with NEAR, nic_pair in {0, 1}) are
+  // compile-time constants of each copy of the loop, so the loop holds no
+  // branch.
+  const bool uni = __syncthreads_and(h1.w == -1.0f && hs.w == -1.0f);
+  auto tiles = [&](auto uni_t, auto ic_t, auto n1_t) {
+    constexpr bool UNI = decltype(uni_t)::value;
+    constexpr bool IC = decltype(ic_t)::value;
+    constexpr bool N1 = decltype(n1_t)::value;  // NEAR: nic_pair == 1
+    const int nicv = NEAR ? (N1 ? 1 : 0) : nic;
+    for (int ph = 0; ph < kSymWaves; ++ph) {
+      const int tb = ((w + ph) & (kSymWaves - 1)) * 64;  // the tile of this wavefront
+      int jl = lane;  // the source whose reaction sum this lane holds
+      double gx = jsum[0][tb + lane], gy = jsum[1][tb + lane], gz = jsum[2][tb + lane];
+      for (int step = 0; step < 64; ++step) {
+        const float4 h2 = shi[tb + jl], l2 = slo[tb + jl];
+        float dx = (h1.x - h2.x) + (l1.x - l2.x);
+        float dy = (h1.y - h2.y) + (l1.y - l2.y);
+        float dz = (h1.z - h2.z) + (l1.z - l2.z);
+        float dxy2 = __builtin_fmaf(dy, dy, dx * dx);
+        float r2 = fmaxf(__builtin_fmaf(dz, dz, dxy2), kR2Min);
+        float ri = rsq(r2);
+        float r3 = ri * ri * ri;
+        float cxy = r3, czi = dz * r3, czj = -czi;
+        if (IC) {
+          float wxy, wz;
+          ic_planar32<NEAR>(h1.z + h2.z, l1.w + l2.w, dz, dxy2, nicv, two_d, wxy, wz);
+          cxy += wxy;
+          czi += wz;
+          czj += wz;
+        }
+        if (UNI) {
+          const double tx = (double)(cxy * dx), ty = (double)(cxy * dy);
+          fx -= tx;
+          fy -= ty;
+          fz -= (double)czi;
+          gx += tx;
+          gy += ty;
+          gz -= (double)czj;
+        } else {
+          const float ci = cxy * h2.w, cj = cxy * h1.w;
+          fx += (double)(ci * dx);
+          fy += (double)(ci * dy);
+          fz += (double)(czi * h2.w);
+          gx -= (double)(cj * dx);
+          gy -= (double)(cj * dy);
+          gz += (double)(czj * h1.w);
+        }
+        gx = wave_rotate(gx);
+        gy = wave_rotate(gy);
+        gz = wave_rotate(gz);
+        jl = wave_rotate(jl);
       }
-      const float ci = cxy * h2.w, cj = cxy * h1.w;
-      fx += (double)(ci * dx);
-      fy += (double)(ci * dy);
-      fz += (double)(czi * h2.w);
-      gx -= (double)(cj * dx);
-      gy -= (double)(cj * dy);
-      gz += (double)(czj * h1.w);
-      gx = wave_rotate(gx);
-      gy = wave_rotate(gy);
-      gz = wave_rotate(gz);
-      jl = wave_rotate(jl);
+      // 64 rotations: every lane holds its own source's sum again
+      jsum[0][tb + lane] = gx;
+      jsum[1][tb + lane] = gy;
+      jsum[2][tb + lane] = gz;
+      __syncthreads();
     }
-    // 64 rotations: every lane holds its own source's sum again
-    jsum[0][tb + lane] = gx;
-    jsum[1][tb + lane] = gy;
-    jsum[2][tb + lane] = gz;
-    __syncthreads();
-  }
+  };
+  auto by_series = [&](auto uni_t) {
+    if (!ic)
+      tiles(uni_t, std::false_type{}, std::false_type{});
+    else if (NEAR && nic >= 1)
+      tiles(uni_t, std::true_type{}, std::true_type{});
+    else
+      tiles(uni_t, std::true_type{}, std::false_type{});
+  };
+  if (uni)
+    by_series(std::true_type{});
+  else
+    by_series(std::false_type{});
   if (i < n) {
     const size_t o = (size_t)Y * e.d_ld + (i - i0);
     D[0 * DS + o] = fx;

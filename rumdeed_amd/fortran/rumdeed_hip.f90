@@ -29,9 +29,10 @@ module rumdeed_hip
   type, bind(C) :: rd_step_record
     integer(c_int) :: step, n, emitted, n_round
     integer(c_int) :: absorbed, absorbed_top, absorbed_bot, dropped
-    integer(c_int) :: start_rounds, supply_level, reserved1, reserved2
+    integer(c_int) :: start_rounds, supply_level, supply_cells, supply_fail
     real(c_double) :: n_sup, ramo_current, mh_std, a_rate, df_avg
     real(c_double) :: f_avg(3), ramo(3), avg_elec_vel(3)
+    real(c_double) :: supply_err
   end type rd_step_record
 
   interface

@@ -15,6 +15,16 @@ histogram), with the reference's Fortran edit descriptors:
 - ``ramo_current.dt``: Write_Ramo_Current (src/mod_pair.F90:812-837), with
   the mobility of Average_Velocities (src/mod_verlet.F90:425-446);
 - ``lifetime.dt``: Write_Life_Time (src/mod_pair.F90:774-784);
+- ``integration.dt``: Do_Surface_Integration_FE
+  (src/mod_field_emission_v2.F90:654-655), one line per planar FE emission
+  step: emitter, regions, integrand evaluations, fail flag, integral, error
+  estimate, probability.  Here the supply is the refined midpoint rule, so
+  the regions are the cells of the level used, the evaluations the points of
+  every level computed, the error the difference of the last two levels
+  (0 for the fixed rule), the fail flag set when the last allowed level still
+  misses the tolerance, and the probability (Divonne's chi-square) 0; the
+  file is opened for every mode (src/main.F90:647) and stays empty for the
+  others;
 - ``volt.dt``: Set_Voltage (src/mod_verlet.F90:2050-2057), step 0 at
   start-up (src/main.F90:172) and then every step; its format has three
   descriptors for four items, so the zero RF voltage goes on a record of
@@ -121,6 +131,24 @@ def fortran_ES(x: float, w: int, d: int) -> str:
     return s.rjust(w) if len(s) <= w else "*" * w
 
 
+def supply_evaluations(cells: int, level: int) -> int:
+    """Integrand evaluations of a planar supply whose level `level` has
+    `cells` cells: every level 0..level was computed, each with a quarter of
+    the next one's cells (level 0 alone for the fixed rule)."""
+    return sum(cells >> (2 * k) for k in range(level + 1))
+
+
+def integration_line(cells: int, level: int, fail: int, integral: float, error: float,
+                     emitter: int = 1) -> str:
+    """One integration.dt record, (i3, tr2, i8, tr2, i8, tr2, i4, tr2, ES12.4,
+    tr2, ES12.4, tr2, ES12.4): emitter, regions, evaluations, fail, integral,
+    error, probability (src/mod_field_emission_v2.F90:654-655)."""
+    return (fortran_I(emitter, 3) + "  " + fortran_I(cells, 8) + "  " +
+            fortran_I(supply_evaluations(cells, level), 8) + "  " + fortran_I(fail, 4) + "  " +
+            fortran_ES(integral, 12, 4) + "  " + fortran_ES(error, 12, 4) + "  " +
+            fortran_ES(0.0, 12, 4))
+
+
 def volt_lines(cur_time: float, step: int, V_d: float, V_rf: float) -> list[str]:
     """The records of one (ES12.4, tr2, i8, tr2, ES18.8) write of cur_time,
     step, V_d, V_rf: the fourth item reverts to the start of the format on a
@@ -152,7 +180,7 @@ class OutputFiles:
         self.V = V
         os.makedirs(directory, exist_ok=True)
         names = ["emitted", "absorbed", "absorbed_top", "absorbed_bot", "absorbed_recom",
-                 "ramo_current", "volt"]
+                 "ramo_current", "volt", "integration"]
         if mode in (EMIT_PLANAR_FE, EMIT_PLANAR_GTF):
             names.append("field")
         self.f = {k: open(os.path.join(directory, k + ".dt"), "w") for k in names}
@@ -218,6 +246,12 @@ class OutputFiles:
             vals = [*r["f_avg"], r["n_sup"], df_avg, r["a_rate"], r["mh_std"]]
             self.f["field"].write(fortran_I(step, 8) +
                                   "".join("  " + fortran_E(v, 16, 8) for v in vals) + "\n")
+        # integration.dt: (i3, tr2, i8, tr2, i8, tr2, i4, tr2, ES12.4, tr2, ES12.4, tr2, ES12.4)
+        cells = r.get("supply_cells", 0)
+        if self.mode == EMIT_PLANAR_FE and cells > 0:
+            self.f["integration"].write(integration_line(cells, r["supply_level"],
+                                                         r["supply_fail"], r["n_sup"],
+                                                         r["supply_err"]) + "\n")
         # absorbed*.dt: (ES12.4, *(tr2, i8))
         def absorbed(name, ints):
             self.f[name].write(fortran_ES(cur_time, 12, 4) +

@@ -1,6 +1,6 @@
 ! Prints sample values with the edit descriptors of RUMDEED's output files
 ! (emitted.dt, field.dt, absorbed*.dt, ramo_current.dt, lifetime.dt,
-! volt.dt), one
+! volt.dt, integration.dt), one
 ! record per line, so that tests/test_output_formats.py can compare the
 ! Python writer against a Fortran compiler's own formatting.
 program fortran_formats
@@ -16,5 +16,8 @@ program fortran_formats
     write (*, "(ES12.4, tr2, i6, tr2, i6)") x(i), i, 7*i
     ! volt.dt: four items for three descriptors (format reversion)
     write (*, "(ES12.4, tr2, i8, tr2, ES18.8)") x(i), i, 2.0d0*x(i) + 1.0d0, 0.0d0
+    ! integration.dt
+    write (*, "(i3, tr2, i8, tr2, i8, tr2, i4, tr2, ES12.4, tr2, ES12.4, tr2, ES12.4)") &
+      & i, 1024*i, 1280*i, mod(i, 2), x(i), -x(i), 0.0d0
   end do
 end program fortran_formats

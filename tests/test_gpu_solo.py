@@ -22,8 +22,8 @@ pytestmark = pytest.mark.gpu
 
 NM = 1e-9
 REC_INT = ("n", "emitted", "n_round", "absorbed", "absorbed_top", "absorbed_bot", "dropped",
-           "start_rounds", "supply_level")
-REC_DBL = ("n_sup", "ramo_current", "mh_std", "a_rate", "df_avg")
+           "start_rounds", "supply_level", "supply_cells", "supply_fail")
+REC_DBL = ("n_sup", "ramo_current", "mh_std", "a_rate", "df_avg", "supply_err")
 STATE = ("pos", "prev_pos", "vel", "acc", "acc_prev", "acc_prev2", "species", "step", "emitter",
          "section", "id")
 

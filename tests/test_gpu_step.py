@@ -13,8 +13,8 @@ pytestmark = pytest.mark.gpu
 NM = 1e-9
 
 REC_INT = ("n", "emitted", "n_round", "absorbed", "absorbed_top", "absorbed_bot", "dropped",
-           "start_rounds", "supply_level")
-REC_DBL = ("n_sup", "ramo_current", "mh_std", "a_rate", "df_avg")
+           "start_rounds", "supply_level", "supply_cells", "supply_fail")
+REC_DBL = ("n_sup", "ramo_current", "mh_std", "a_rate", "df_avg", "supply_err")
 
 
 def assert_records_equal(rg, ro, ctx=""):
@@ -173,6 +173,8 @@ def test_output_files_equal_oracle(Diode, oracle_mod, tmp_path):
     fo.close(so.life_time())
     names = sorted(os.listdir(tmp_path / "gpu"))
     assert names == sorted(os.listdir(tmp_path / "oracle")) and "ramo_current.dt" in names
+    # one supply line per emission step
+    assert len((tmp_path / "gpu" / "integration.dt").read_text().splitlines()) == 25
     for name in names:
         assert (tmp_path / "gpu" / name).read_bytes() == (tmp_path / "oracle" / name).read_bytes(), name
 

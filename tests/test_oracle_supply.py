@@ -122,3 +122,24 @@ def test_level_limit(oracle_mod):
     c = _sim(oracle_mod, 9, 4, 1, 6000.0, wf, pos=pos, **GTF)
     assert a.supply_level == 0 and c.supply_level == 1
     assert c.n_sup == b.n_sup and a.n_sup != b.n_sup
+
+
+def test_supply_diagnostics(oracle_mod):
+    """The record's integration.dt fields: cells of the level used, the
+    difference of the last two levels, and the fail flag, which is set only
+    when the last allowed level still misses the tolerance."""
+    pos = _space_charge()
+    wf = np.array([[4.0]])
+    runs = {lv: _sim(oracle_mod, 9, 4, lv, 6000.0, wf, pos=pos, **GTF) for lv in (-1, 1, 0)}
+    rules = {G: _sim(oracle_mod, 9, G, -1, 6000.0, wf, pos=pos, **GTF) for G in (4, 8)}
+    fixed, one, refined = runs[-1], runs[1], runs[0]
+    assert (fixed.supply_cells, fixed.supply_err, fixed.supply_fail) == (16, 0.0, 0)
+    # one doubling allowed: the 8 x 8 rule against the 4 x 4 rule, which are
+    # several tolerances apart (test_refinement_converges_to_the_fine_rule)
+    d = abs(rules[8].n_sup - rules[4].n_sup)
+    assert one.supply_cells == 64 and one.supply_err == d and d > _tol(one.n_sup)
+    assert one.supply_fail == 1
+    # refined until converged: the last difference within the tolerance
+    G = 4 << refined.supply_level
+    assert refined.supply_cells == G * G and refined.supply_fail == 0
+    assert 0.0 < refined.supply_err <= _tol(refined.n_sup)

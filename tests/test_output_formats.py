@@ -28,12 +28,16 @@ def python_lines():
                                                     for v in (i, 1000 * i, 0)))
         out.append(O.fortran_ES(x, 12, 4) + "  " + O.fortran_I(i, 6) + "  " + O.fortran_I(7 * i, 6))
         out.extend(O.volt_lines(x, i, 2.0 * x + 1.0, 0.0))
+        out.append(O.fortran_I(i, 3) + "  " + O.fortran_I(1024 * i, 8) + "  " +
+                   O.fortran_I(1280 * i, 8) + "  " + O.fortran_I(i % 2, 4) + "  " +
+                   O.fortran_ES(x, 12, 4) + "  " + O.fortran_ES(-x, 12, 4) + "  " +
+                   O.fortran_ES(0.0, 12, 4))
     return out
 
 
 @pytest.mark.skipif(not os.path.exists(FLANG), reason="no Fortran compiler in this image")
 def test_edit_descriptors_match_fortran(tmp_path):
-    """E14.6, E16.8, ES12.4, i6, i8 and tr fields as a Fortran compiler
+    """E14.6, E16.8, ES12.4, i3, i4, i6, i8 and tr fields as a Fortran compiler
     writes them (zero, negative zero, rounding carries, three-digit
     exponents)."""
     exe = tmp_path / "ff"
@@ -54,9 +58,10 @@ def test_known_lines():
 
 def _record(step, **kw):
     r = dict(step=step, n=100 + step, emitted=7, n_round=20, absorbed=3, absorbed_top=2,
-             absorbed_bot=1, dropped=0, start_rounds=1, n_sup=12.5, ramo_current=1e-6,
-             mh_std=0.05, a_rate=0.3, df_avg=0.01, f_avg=[1.0, -2.0, -3.0e9],
-             ramo=[1e-6, 0.0, 0.0], avg_elec_vel=[1.0, 2.0, 2.0e5])
+             absorbed_bot=1, dropped=0, start_rounds=1, supply_level=1, supply_cells=4096,
+             supply_fail=0, n_sup=12.5, ramo_current=1e-6, mh_std=0.05, a_rate=0.3,
+             df_avg=0.01, f_avg=[1.0, -2.0, -3.0e9], ramo=[1e-6, 0.0, 0.0],
+             avg_elec_vel=[1.0, 2.0, 2.0e5], supply_err=0.25)
     r.update(kw)
     return r
 
@@ -92,6 +97,10 @@ def test_run_files(tmp_path):
     assert len(vt) == 2 * 6  # step 0 and steps 1-5, two records each
     assert vt[2] == O.fortran_ES(dt * 1 / 1e-12, 12, 4) + "  " + O.fortran_I(1, 8) + "  " + \
         O.fortran_ES(V, 18, 8) and vt[3] == "  0.0000E+00"
+    it = (tmp_path / "integration.dt").read_text().splitlines()
+    # 64 x 64 cells at level 1: the 32 x 32 level 0 was evaluated too
+    assert it == ["  1      4096      5120     0    1.2500E+01    2.5000E-01    0.0000E+00"] * 5
+    assert O.supply_evaluations(64 * 64, 3) == 32 * 32 + 16 * 16 + 8 * 8 + 64 * 64
     lt = np.loadtxt(tmp_path / "lifetime.dt")
     assert lt.shape == (1000, 4) and lt[4, 2] == 9 and lt[4, 1] == 5
     init = np.fromfile(tmp_path / "init.bin", dtype=np.uint8)
