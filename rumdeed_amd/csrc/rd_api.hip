@@ -385,7 +385,7 @@ int rd_ctx_destroy(rd_ctx* c) {
   Buf* bufs[] = {&c->part, &c->pack_hi, &c->pack_lo, &c->pack_far, &c->pack_srf, &c->pts, &c->fld, &c->pts_hi, &c->pts_lo,
                  &c->stage, &c->img64, &c->tab, &c->red, &c->chains, &c->rnd, &c->img_hi, &c->img_lo, &c->cnt,
                  &c->mark, &c->elog, &c->alog, &c->plog, &c->tail_plan, &c->tail_work,
-                 &c->tail_box, &c->acc};
+                 &c->tail_box, &c->acc, &c->ctmp};
   for (Buf* b : bufs) free_buf(*b);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);

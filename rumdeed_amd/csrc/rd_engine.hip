@@ -179,37 +179,39 @@ __global__ void k_quad_points(const Ent* tab, const SimDesc* desc, double* __res
 }
 
 // Integrand per point and the sums of one level (value, f_x, f_y, f_z; the
-// tip: value, 0, 0, normal field) into S.qsum[lev & 1]: tiles of 1024
-// points, a fixed pairwise tree within a tile (zeros beyond the last point),
-// tile sums added in tile order.
-__global__ __launch_bounds__(1024) void k_quad_sum(const Ent* tab, const SimDesc* desc,
-                                                  SimState* st, const double* wf,
-                                                  const double* __restrict__ ptx,
-                                                  const double* __restrict__ pty,
-                                                  const double* __restrict__ ptz,
-                                                  const double* __restrict__ fx,
-                                                  const double* __restrict__ fy,
-                                                  const double* __restrict__ fz,
-                                                  double* __restrict__ qv,
-                                                  double* __restrict__ qn) {
+// tip: value, 0, 0, normal field): tiles of 1024 points, a fixed pairwise
+// tree within a tile (zeros beyond the last point), tile sums added in tile
+// order.  k_quad_tiles: a workgroup per tile (its integrand values and its
+// tree; the tile sums at qt + 4 (e.c_off + tile)); k_quad_tsum: the tile sums
+// in tile order into S.qsum[lev & 1].
+__global__ __launch_bounds__(1024) void k_quad_tiles(const Ent* tab, const SimDesc* desc,
+                                                    const double* wf,
+                                                    const double* __restrict__ ptx,
+                                                    const double* __restrict__ pty,
+                                                    const double* __restrict__ fx,
+                                                    const double* __restrict__ fy,
+                                                    const double* __restrict__ fz,
+                                                    double* __restrict__ qt) {
   constexpr int kTile = 1024;
   __shared__ double lds[4 * kTile];
   double *t0 = lds, *t1 = lds + kTile, *t2 = lds + 2 * kTile, *t3 = lds + 3 * kTile;
-  const Ent e = tab[blockIdx.x];
+  const Ent e = tab[blockIdx.y];
+  const int b = blockIdx.x * kTile;
+  if (b >= e.m) return;  // uniform
   const SimDesc& s = desc[e.slot];
   const EmitConsts ec = emit_consts();
   const bool tip = s.mode == RD_EMIT_TIP_FE;
-  for (int k = threadIdx.x; k < e.m; k += blockDim.x) {
-    const int p = e.pbase + k;
-    double val = 0.0, F;
+  const int k = threadIdx.x;  // blockDim.x == kTile
+  double val = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  if (b + k < e.m) {
+    const int kk = b + k;
+    const int p = e.pbase + kk;
     if (tip) {
-      F = field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]);
-      qn[p] = F;
-      if (F >= 0.0) {
-        val = 0.0;
-      } else {
+      const double F = field_normal(s.g, ptx[p], pty[p], fx[p], fy[p], fz[p]);
+      g2 = F;
+      if (F < 0.0) {
         const int nr = 100;
-        int i = k / nr + 1, j = k % nr + 1;
+        int i = kk / nr + 1, j = kk % nr + 1;
         double len_xi = (s.g.max_xi - 1.0) / nr;
         double len_phi = 2.0 * 3.141592653589793238462643383279502884197 / nr;
         double xi_1 = 1.0 + (i - 1.0) * len_xi, phi_1 = (j - 1.0) * len_phi;
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(1024) void k_quad_sum(const Ent* tab, const SimDesc
         val = A_f * ec.a_FN * (F * F) * s.dt / (ec.q0 * s.w_tip * (t * t));
       }
     } else {
-      F = fz[p];
+      const double F = fz[p];
       if (F < 0.0) {
         int sec;
         double w = work_at(s, wf, ptx[p], pty[p], sec);
@@ -230,52 +232,51 @@ __global__ __launch_bounds__(1024) void k_quad_sum(const Ent* tab, const SimDesc
           val = gtf_current(F, s.T, w) * s.dt_q0;
       }
       val = (s.elx * s.ely) * val;
+      g0 = fx[p];
+      g1 = fy[p];
+      g2 = F;
     }
-    qv[p] = val;
   }
+  t0[k] = val;
+  t1[k] = g0;
+  t2[k] = g1;
+  t3[k] = g2;
+  for (int w = kTile / 2; w > 0; w >>= 1) {
+    __syncthreads();
+    if (k < w) {
+      t0[k] = t0[k] + t0[k + w];
+      t1[k] = t1[k] + t1[k + w];
+      t2[k] = t2[k] + t2[k + w];
+      t3[k] = t3[k] + t3[k + w];
+    }
+  }
+  __syncthreads();
+  if (k == 0) {
+    double* o = qt + 4 * (e.c_off + blockIdx.x);
+    o[0] = t0[0];
+    o[1] = t1[0];
+    o[2] = t2[0];
+    o[3] = t3[0];
+  }
+}
+
+__global__ void k_quad_tsum(const Ent* tab, SimState* st, const double* __restrict__ qt) {
+  const Ent e = tab[blockIdx.x];
+  if (threadIdx.x != 0) return;
   double sum = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int b = 0; b < e.m; b += kTile) {
-    const int k = threadIdx.x;  // blockDim.x == kTile
-    __syncthreads();
-    if (b + k < e.m) {
-      const int p = e.pbase + b + k;
-      t0[k] = qv[p];
-      if (tip) {
-        t1[k] = 0.0;
-        t2[k] = 0.0;
-        t3[k] = qn[p];
-      } else {
-        t1[k] = fx[p];
-        t2[k] = fy[p];
-        t3[k] = fz[p];
-      }
-    } else {
-      t0[k] = t1[k] = t2[k] = t3[k] = 0.0;
-    }
-    for (int w = kTile / 2; w > 0; w >>= 1) {
-      __syncthreads();
-      if (k < w) {
-        t0[k] = t0[k] + t0[k + w];
-        t1[k] = t1[k] + t1[k + w];
-        t2[k] = t2[k] + t2[k + w];
-        t3[k] = t3[k] + t3[k + w];
-      }
-    }
-    __syncthreads();
-    if (k == 0) {
-      sum = sum + t0[0];
-      f0 = f0 + t1[0];
-      f1 = f1 + t2[0];
-      f2 = f2 + t3[0];
-    }
+  const int nt = (e.m + 1023) / 1024;
+  for (int b = 0; b < nt; ++b) {
+    const double* o = qt + 4 * (e.c_off + b);
+    sum = sum + o[0];
+    f0 = f0 + o[1];
+    f1 = f1 + o[2];
+    f2 = f2 + o[3];
   }
-  if (threadIdx.x == 0) {
-    double* q = st[e.slot].qsum[e.lev & 1];
-    q[0] = sum;
-    q[1] = f0;
-    q[2] = f1;
-    q[3] = f2;
-  }
+  double* q = st[e.slot].qsum[e.lev & 1];
+  q[0] = sum;
+  q[1] = f0;
+  q[2] = f1;
+  q[3] = f2;
 }
 
 // The supply's level e.lev is final when it is the tip's rule, the last level
@@ -2022,6 +2023,198 @@ __global__ __launch_bounds__(1024) void k_compact(
   if (threadIdx.x == 0) S.n = s_off;
 }
 
+// ------------------------------------------------------------------------
+// Compaction of a large simulation by chunks of 1024 particles, in three
+// launches with a workgroup per chunk (k_compact takes one workgroup per
+// simulation and walks its chunks in turn: about 5 us per chunk).  The result
+// is the same data movement: survivors in index order, absorption records in
+// index order, the same lifetime counts.
+//   k_compact_count: each chunk from the one holding the first marked
+//     particle counts its survivors and its removed particles;
+//   k_compact_move:  each chunk adds the counts of the chunks before it (its
+//     destination and record offsets), writes its absorption records and
+//     lifetimes, and stages its survivors in the staging arrays;
+//   k_compact_back:  the staged survivors return to the particle arrays from
+//     the first chunk's start on, and the new count is stored.
+// The staging keeps the moves free of overlaps between workgroups.  Header
+// slot (c_off): the particle count before compaction, written by pass 1 and
+// read by pass 3 (which stores the new count into the state).
+constexpr int kCompactSplit = 8 * 1024;  // particles from which a simulation is compacted by chunks
+
+struct CompactStage {
+  int2* cnt;        // headers and chunk counts (survivors, removed)
+  double* d;        // 20 staged double arrays of ts particles each
+  int32_t* i;       // 5 staged int arrays
+  long long ts;     // staged particles per array
+};
+
+// Survivors and removed particles of the block's chunk, and the ranks of this
+// thread's particle among each (every thread of the block calls it).
+__device__ __forceinline__ void chunk_ranks(int flag, int gone, int& rank, int& total, int& rrank,
+                                            int& rtotal) {
+  __shared__ int wsum[1024 / 64], rsum[1024 / 64];
+  const unsigned long long mask = __ballot(flag), gmask = __ballot(gone);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    wsum[wv] = __popcll(mask);
+    rsum[wv] = __popcll(gmask);
+  }
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  rank = __popcll(mask & below);
+  rrank = __popcll(gmask & below);
+  total = 0;
+  rtotal = 0;
+  for (int u = 0; u < 1024 / 64; ++u) {
+    if (u < wv) {
+      rank += wsum[u];
+      rrank += rsum[u];
+    }
+    total += wsum[u];
+    rtotal += rsum[u];
+  }
+  __syncthreads();
+}
+
+// Sum of the (survivors, removed) counts of chunks [c0, c1) over the block.
+__device__ __forceinline__ int2 chunk_prefix(const int2* cnt, int c0, int c1) {
+  __shared__ int pa[1024 / 64], pr[1024 / 64];
+  int a = 0, r = 0;
+  for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
+    a += cnt[c].x;
+    r += cnt[c].y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    r += __shfl_xor(r, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    pa[threadIdx.x >> 6] = a;
+    pr[threadIdx.x >> 6] = r;
+  }
+  __syncthreads();
+  a = 0;
+  r = 0;
+  for (int u = 0; u < (int)(blockDim.x >> 6); ++u) {
+    a += pa[u];
+    r += pr[u];
+  }
+  __syncthreads();
+  return make_int2(a, r);
+}
+
+__global__ __launch_bounds__(1024) void k_compact_count(const Ent* tab, const SimState* st,
+                                                       const int32_t* ALIVE, CompactStage cs) {
+  const Ent e = tab[blockIdx.y];
+  const SimState& S = st[e.slot];
+  const int n = S.n;
+  const int c = blockIdx.x, cb = c * 1024;
+  if (S.rem_total == 0 || cb >= n) return;
+  const int c0 = min(S.first_dead, n) / 1024;
+  if (c < c0) return;
+  const int i = cb + threadIdx.x;
+  const int flag = (i < n && ALIVE[e.base + i]) ? 1 : 0;
+  const int gone = (i < n && !flag) ? 1 : 0;
+  int rank, total, rrank, rtotal;
+  chunk_ranks(flag, gone, rank, total, rrank, rtotal);
+  if (threadIdx.x == 0) {
+    cs.cnt[e.c_off + 1 + c] = make_int2(total, rtotal);
+    if (c == c0) cs.cnt[e.c_off] = make_int2(n, 0);
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_compact_move(
+    const Ent* tab, const SimState* st, int32_t* life, const double* X, const double* Y,
+    const double* Z, const double* PX, const double* PY, const double* PZ, const double* VX,
+    const double* VY, const double* VZ, const double* AX, const double* AY, const double* AZ,
+    const double* A1X, const double* A1Y, const double* A1Z, const double* A2X, const double* A2Y,
+    const double* A2Z, const double* Q, const double* M, const int32_t* SP, const int32_t* STEP,
+    const int32_t* EMIT, const int32_t* SEC, const int32_t* ID, const int32_t* ALIVE,
+    const double* mark, rd_absorb_record* alog, CompactStage cs) {
+  const Ent e = tab[blockIdx.y];
+  const SimState& S = st[e.slot];
+  const int n = S.n;
+  const int c = blockIdx.x, cb = c * 1024;
+  if (S.rem_total == 0 || cb >= n) return;
+  const int c0 = min(S.first_dead, n) / 1024;
+  if (c < c0) return;
+  const int2* cnt = cs.cnt + e.c_off + 1;
+  const int2 pre = chunk_prefix(cnt, c0, c);  // survivors and records before this chunk
+  const int i = cb + threadIdx.x;
+  const int base = e.base;
+  const int flag = (i < n && ALIVE[base + i]) ? 1 : 0;
+  const int gone = (i < n && !flag) ? 1 : 0;
+  int rank, total, rrank, rtotal;
+  chunk_ranks(flag, gone, rank, total, rrank, rtotal);
+  const int gi = base + i;
+  if (gone) {
+    // record_lifetime (src/mod_pair.F90:1135-1160)
+    int lt = e.step - STEP[gi];
+    if (lt <= 0) lt = 1;
+    if (lt > kLife) lt = kLife;
+    const int spc = SP[gi];
+    if (spc >= 1 && spc <= 2) atomicAdd(&life[((size_t)e.slot * 2 + (spc - 1)) * kLife + (lt - 1)], 1);
+    if (alog) {
+      rd_absorb_record a;
+      a.x = X[gi];
+      a.y = Y[gi];
+      a.z = Z[gi];
+      a.vx = mark[4 * (size_t)gi + 0];
+      a.vy = mark[4 * (size_t)gi + 1];
+      a.vz = mark[4 * (size_t)gi + 2];
+      a.emitter = EMIT[gi];
+      a.section = SEC[gi];
+      a.id = ID[gi];
+      a.where = (int)mark[4 * (size_t)gi + 3];
+      alog[base + pre.y + rrank] = a;
+    }
+  }
+  if (flag) {
+    const long long k = e.t_off + pre.x + rank;  // staged position
+    const long long T = cs.ts;
+    const double v[20] = {X[gi],   Y[gi],   Z[gi],   PX[gi],  PY[gi],  PZ[gi],  VX[gi],
+                          VY[gi],  VZ[gi],  AX[gi],  AY[gi],  AZ[gi],  A1X[gi], A1Y[gi],
+                          A1Z[gi], A2X[gi], A2Y[gi], A2Z[gi], Q[gi],   M[gi]};
+#pragma unroll
+    for (int a = 0; a < 20; ++a) cs.d[a * T + k] = v[a];
+    const int32_t w[5] = {SP[gi], STEP[gi], EMIT[gi], SEC[gi], ID[gi]};
+#pragma unroll
+    for (int a = 0; a < 5; ++a) cs.i[a * T + k] = w[a];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_compact_back(
+    const Ent* tab, SimState* st, double* X, double* Y, double* Z, double* PX, double* PY,
+    double* PZ, double* VX, double* VY, double* VZ, double* AX, double* AY, double* AZ,
+    double* A1X, double* A1Y, double* A1Z, double* A2X, double* A2Y, double* A2Z, double* Q,
+    double* M, int32_t* SP, int32_t* STEP, int32_t* EMIT, int32_t* SEC, int32_t* ID,
+    int32_t* ALIVE, CompactStage cs) {
+  const Ent e = tab[blockIdx.y];
+  SimState& S = st[e.slot];
+  if (S.rem_total == 0) return;
+  const int n = cs.cnt[e.c_off].x;  // the count before compaction (pass 1)
+  const int c = blockIdx.x, cb = c * 1024;
+  if (cb >= n) return;
+  const int c0 = min(S.first_dead, n) / 1024;
+  if (c < c0) return;
+  const int2 all = chunk_prefix(cs.cnt + e.c_off + 1, c0, (n + 1023) / 1024);
+  const int start = c0 * 1024, nn = start + all.x;  // the new count
+  if (c == c0 && threadIdx.x == 0) S.n = nn;
+  const int d = cb + threadIdx.x;  // destination index
+  if (d >= nn) return;
+  const long long k = e.t_off + (d - start);
+  const long long T = cs.ts;
+  const int gd = e.base + d;
+  double* dst[20] = {X,  Y,  Z,  PX,  PY,  PZ,  VX,  VY,  VZ,  AX,
+                     AY, AZ, A1X, A1Y, A1Z, A2X, A2Y, A2Z, Q, M};
+#pragma unroll
+  for (int a = 0; a < 20; ++a) dst[a][gd] = cs.d[a * T + k];
+  int32_t* dsti[5] = {SP, STEP, EMIT, SEC, ID};
+#pragma unroll
+  for (int a = 0; a < 5; ++a) dsti[a][gd] = cs.i[a * T + k];
+  ALIVE[gd] = 1;
+}
+
 // ---- Check_Planes records ------------------------------------------------
 // One workgroup per simulation walks its particles in index order after the
 // move (removed ones included, as Check_Planes runs before removal) and lists
@@ -2185,9 +2378,57 @@ int engine_compact(rd_ctx* c, const std::vector<Ent>& ents) {
     k_planes<<<ents.size(), 1024, 0, c->stream>>>(
         dt, c->d_desc, c->d_state, c->x, c->y, c->z, c->pz, c->emitter, c->section, c->id,
         (const double*)c->mark.p, (rd_plane_record*)c->plog.p, (long long)c->cap);
-  k_compact<<<ents.size(), 1024, 0, c->stream>>>(
-      dt, c->d_desc, c->d_state, c->d_life, RD_ARRAYS, c->logs ? (const double*)c->mark.p : nullptr,
-      c->logs ? (rd_absorb_record*)c->alog.p : nullptr);
+  // simulations of up to kCompactSplit particles: one workgroup each; larger
+  // ones: a workgroup per chunk (the same data movement either way)
+  std::vector<Ent> small, large;
+  long long cslots = 0, ts = 0;
+  int maxch = 0;
+  for (Ent e : ents) {
+    if (e.n_ub < kCompactSplit) {
+      small.push_back(e);
+      continue;
+    }
+    const int nch = (e.n_ub + 1023) / 1024;
+    e.c_off = cslots;
+    e.t_off = ts;
+    cslots += 1 + nch;
+    ts += (long long)nch * 1024;
+    maxch = std::max(maxch, nch);
+    large.push_back(e);
+  }
+  const double* mark = c->logs ? (const double*)c->mark.p : nullptr;
+  rd_absorb_record* alog = c->logs ? (rd_absorb_record*)c->alog.p : nullptr;
+  if (!small.empty()) {
+    Ent* ds = dt;
+    if (!large.empty()) {
+      r = upload_tab(c, small, &ds);
+      if (r) return r;
+    }
+    k_compact<<<small.size(), 1024, 0, c->stream>>>(ds, c->d_desc, c->d_state, c->d_life,
+                                                     RD_ARRAYS, mark, alog);
+  }
+  if (!large.empty()) {
+    const size_t cnt_bytes = ((sizeof(int2) * (size_t)cslots + 255) / 256) * 256;
+    r = ensure(c, c->ctmp, cnt_bytes + (20 * sizeof(double) + 5 * sizeof(int32_t)) * (size_t)ts);
+    if (r) return r;
+    CompactStage cs;
+    cs.cnt = (int2*)c->ctmp.p;
+    cs.d = (double*)((char*)c->ctmp.p + cnt_bytes);
+    cs.i = (int32_t*)(cs.d + 20 * (size_t)ts);
+    cs.ts = ts;
+    Ent* dl;
+    r = upload_tab(c, large, &dl);
+    if (r) return r;
+    const dim3 g(maxch, large.size());
+    k_compact_count<<<g, 1024, 0, c->stream>>>(dl, c->d_state, c->alive, cs);
+    k_compact_move<<<g, 1024, 0, c->stream>>>(dl, c->d_state, c->d_life, RD_ARRAYS, mark, alog,
+                                               cs);
+    k_compact_back<<<g, 1024, 0, c->stream>>>(dl, c->d_state, c->x, c->y, c->z, c->px, c->py,
+                                               c->pz, c->vx, c->vy, c->vz, c->ax, c->ay, c->az,
+                                               c->a1x, c->a1y, c->a1z, c->a2x, c->a2y, c->a2z,
+                                               c->q, c->m, c->species, c->pstep, c->emitter,
+                                               c->section, c->id, c->alive, cs);
+  }
   RD_HIP(hipGetLastError());
   return RD_OK;
 }
@@ -2529,17 +2770,21 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
   if (q.empty()) return RD_OK;
   int r;
   while (!q.empty()) {
-    int mtot = 0, maxm = 0;
+    int mtot = 0, maxm = 0, ntiles = 0, maxt = 0;
     for (Ent& e : q) {
       const SimDesc& d = c->slots[e.slot].desc;
       e.pbase = mtot;
       e.m = (d.mode == RD_EMIT_TIP_FE) ? 100 * 100 : (d.qgx << e.lev) * (d.qgy << e.lev);
       mtot += e.m;
       maxm = std::max(maxm, e.m);
+      const int nt = (e.m + 1023) / 1024;  // tiles of the level's sums
+      e.c_off = ntiles;
+      ntiles += nt;
+      maxt = std::max(maxt, nt);
     }
     r = ensure(c, c->pts, sizeof(double) * 3 * (size_t)mtot);
     if (!r) r = ensure(c, c->fld, sizeof(double) * 3 * (size_t)mtot);
-    if (!r) r = ensure(c, c->stage, sizeof(double) * 2 * (size_t)mtot);
+    if (!r) r = ensure(c, c->stage, sizeof(double) * 4 * (size_t)ntiles);
     if (r) return r;
     double* ptx = (double*)c->pts.p;
     Ent* dq;
@@ -2550,10 +2795,9 @@ static int engine_emission(rd_ctx* c, const std::vector<Ent>& all, int precision
     r = launch_field(c, q, mtot, precision, nullptr, nullptr);
     if (r) return r;
     double* f = (double*)c->fld.p;
-    k_quad_sum<<<q.size(), 1024, 0, c->stream>>>(dq, c->d_desc, c->d_state, c->d_wf, ptx,
-                                                 ptx + mtot, ptx + 2 * mtot, f, f + mtot,
-                                                 f + 2 * mtot, (double*)c->stage.p,
-                                                 (double*)c->stage.p + mtot);
+    k_quad_tiles<<<dim3(maxt, q.size()), 1024, 0, c->stream>>>(
+        dq, c->d_desc, c->d_wf, ptx, ptx + mtot, f, f + mtot, f + 2 * mtot, (double*)c->stage.p);
+    k_quad_tsum<<<q.size(), 64, 0, c->stream>>>(dq, c->d_state, (const double*)c->stage.p);
     Ent* dtop;
     r = upload_tab(c, top, &dtop);
     if (r) return r;

@@ -122,6 +122,11 @@ struct Ent {
   int d_ld;          // leading dimension of the direct partials (pair-symmetric)
   long long acc_off; // running sums [3][targets] at rd_ctx::acc + acc_off
   long long d_off;   // direct partials [3][segs][d_ld] (pair-symmetric, banded)
+  // Compaction by chunks (k_compact_count / _move / _back): the entry's header
+  // and chunk counts from int2 slot c_off of rd_ctx::ctmp, its survivors staged
+  // from particle t_off of the staging arrays behind them.  Supply levels
+  // (k_quad_tiles / _tsum): c_off is the entry's first tile-sum slot.
+  long long c_off, t_off;
 };
 
 // Entry of a compact launch whose workgroups [wg0, next wg0) belong to one
@@ -221,6 +226,7 @@ struct rd_ctx {
   // running sums of banded launches and the partial-sum bound of one band
   // (rd_set_partial_budget)
   rd::Buf acc;
+  rd::Buf ctmp;  // survivors of the simulations compacted by chunks (kCompactSplit)
   size_t part_budget = kPartBudget;
   size_t work_bytes = 0;  // device bytes held by the work buffers (ensure)
   double tail_d = -1.0;

@@ -259,6 +259,45 @@ def test_particle_records_equal_oracle(Diode, oracle_mod, tmp_path):
         assert eg.tobytes() == eo.tobytes() and ag.tobytes() == ao.tobytes(), f"tip step {k}"
 
 
+def test_chunked_compaction_equals_oracle(Diode, oracle_mod):
+    """A simulation of more than 8192 particles is compacted by chunks
+    (k_compact_count / _move / _back, a workgroup per 1024 particles): 12 000
+    electrons spread over a 200 nm gap with fast vertical velocities, so that
+    over four 1 fs steps particles leave through both plates from every
+    chunk.  Records, absorption records (in index order), lifetimes and the
+    particle state equal the oracle's bit for bit."""
+    rng = np.random.default_rng(23)
+    n = 12000
+    pos = np.column_stack([rng.uniform(-100, 100, n), rng.uniform(-100, 100, n),
+                           rng.uniform(1, 199, n)]) * NM
+    vel = np.column_stack([rng.normal(0, 2e5, n), rng.normal(0, 2e5, n),
+                           rng.normal(0, 3e6, n)])
+    dd = Diode(16384)
+    so = oracle_mod.Sim(16384, nthreads=16)
+    for t in (dd, so):
+        if hasattr(t, "set_planar"):
+            t.set_planar(1600.0, 200 * NM, True, 0)
+        else:
+            t.set_geometry(oracle_mod.geom_planar(200 * NM, 1600.0, True, 0))
+        t.set_time_step(1e-15)
+    dd.add_particles(pos, vel)
+    so.add(pos, vel)
+    dd.set_logs(True)
+    removed = 0
+    for k in range(1, 5):
+        rg = dd.run(1, "fp64")[0]
+        ro = so.step(k)
+        print(f"chunked compaction step {k}: n={rg.n} absorbed={rg.absorbed}", flush=True)
+        assert_records_equal(rg, ro, f"step {k}")
+        _, ag = dd.step_logs()
+        _, ao = so.step_logs()
+        assert ag.tobytes() == ao.tobytes(), f"absorption records, step {k}"
+        removed += rg.absorbed
+    assert removed > 300 and rg.n > 8192  # every step compacted by chunks
+    assert_state_equal(dd.get_state(), so.state(), "chunked compaction")
+    assert np.array_equal(np.asarray(dd.life_time()).reshape(-1), np.asarray(so.life_time()).reshape(-1))
+
+
 def test_particle_records_mixed(Diode):
     """Mixed precision: the records agree with the step's counts, ids run
     on, and each absorbed particle has left the particle list."""
