@@ -358,7 +358,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-summary", type=int, default=64,
                     help="requests of the short serving cut in the default line (0: none)")
-    ap.add_argument("--serve-summary-steps", type=int, default=200)
+    ap.add_argument("--serve-summary-steps", type=int, default=2000,
+                    help="steps per request of the serving cut (the serving config's 2000)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

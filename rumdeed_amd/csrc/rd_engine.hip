@@ -897,15 +897,6 @@ constexpr int kJumpTargets = 64;
 constexpr int kJumpWaves = 16;
 constexpr int kJumpSlice = kChunk / kJumpWaves;
 
-// p[j] for a wavefront-uniform j of memory that no launch in flight writes:
-// read through the constant address space, so the load goes through the
-// scalar cache into SGPRs.
-__device__ __forceinline__ float4 load_uniform(const float4* p, int j) {
-  typedef const __attribute__((address_space(4))) float cfloat;
-  const cfloat* q = (const cfloat*)(const float*)(p + j);
-  return make_float4(q[0], q[1], q[2], q[3]);
-}
-
 // One mixed pair term (general series).
 __device__ __forceinline__ void mixed_pair(const float4 h1, const float4 l1, const float4 h2,
                                            const float4 l2, int ic, int nic, float two_d,

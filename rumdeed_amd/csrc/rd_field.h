@@ -502,6 +502,23 @@ __device__ __forceinline__ void field_finish64(const DevGeom& g, const double* _
   fz = ez + sz;
 }
 
+// p[j] for a wavefront-uniform j of memory that no launch in flight writes:
+// read through the constant address space, so the load goes through the
+// scalar cache into SGPRs.
+__device__ __forceinline__ float4 load_uniform(const float4* p, int j) {
+  typedef const __attribute__((address_space(4))) float cfloat;
+  const cfloat* q = (const cfloat*)(const float*)(p + j);
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+// v[q] = the minimum of v[q] over the wavefront's 64 lanes (butterfly).
+__device__ __forceinline__ void wave_min4(float v[4]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = fminf(v[q], __shfl_xor(v[q], o));
+}
+
 // Mixed (planar): FP64 sums, in nm^-2 per unit charge, of the packed sources
 // of chunks [c0, c1) at the packed target (h1, l1).
 __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h1, float4 l1,
@@ -517,24 +534,77 @@ __device__ __forceinline__ void field_seg32(const DevGeom& g, bool act, float4 h
   const float inv_d2 = (float)(1.0 / ((g.d * 1e9) * (g.d * 1e9)));
   const bool surf = wave_on_surface(ic, act, h1, l1);
   const int t = threadIdx.x;
+  // Lateral box of the wavefront's targets and of each chunk's sources (high
+  // parts): when every pair of a chunk lies within the far series' range,
+  // the wavefront takes the chunk without the per-pair test (the same choice
+  // the test makes for each pair, so the same result).
+  const bool boxes = far && nic > 0;
+  float tb[4] = {act ? h1.x : 3e38f, act ? -h1.x : 3e38f, act ? h1.y : 3e38f,
+                 act ? -h1.y : 3e38f};  // min x, -max x, min y, -max y
+  if (boxes) wave_min4(tb);
+  __shared__ float sbox[4][kBlock / 64];
   fx = 0.0;
   fy = 0.0;
   fz = 0.0;
   for (int c = c0; c < c1; ++c) {
     const int jb = c * kChunk;
     __syncthreads();
+    float sb[4] = {3e38f, 3e38f, 3e38f, 3e38f};
     if (jb + t < n) {
-      shi[t] = hi[jb + t];
+      const float4 hv = hi[jb + t];
+      shi[t] = hv;
       slo[t] = lo[jb + t];
       if (far) {
         sfar[2 * t] = far[2 * (jb + t)];
         sfar[2 * t + 1] = far[2 * (jb + t) + 1];
       }
+      sb[0] = hv.x;
+      sb[1] = -hv.x;
+      sb[2] = hv.y;
+      sb[3] = -hv.y;
+    }
+    if (boxes) {
+      wave_min4(sb);
+      if ((t & 63) == 0)
+        for (int q = 0; q < 4; ++q) sbox[q][t >> 6] = sb[q];
     }
     __syncthreads();
     const int cnt = min(kChunk, n - jb);
+    bool inside = false;
+    if (boxes) {
+      for (int q = 0; q < 4; ++q) {
+        sb[q] = sbox[q][0];
+        for (int u = 1; u < (int)(blockDim.x >> 6); ++u) sb[q] = fminf(sb[q], sbox[q][u]);
+      }
+      // largest |dx|, |dy| over the chunk's pairs, with a margin of 1e-3 nm for
+      // the low parts and the rounding of the differences
+      const float ex = fmaxf(-tb[1] - sb[0], -sb[1] - tb[0]) + 1e-3f;
+      const float ey = fmaxf(-tb[3] - sb[2], -sb[3] - tb[2]) + 1e-3f;
+      inside = fmaxf(ex, 0.f) * fmaxf(ex, 0.f) + fmaxf(ey, 0.f) * fmaxf(ey, 0.f) <=
+               0.999f * slim;
+    }
     if (act && surf) {  // fz holds the sum without the factor -2 until the end
-      if (nic > 0 && far)
+      if (nic > 0 && far && inside)
+        for (int jb2 = 0; jb2 < cnt; jb2 += kSurfBlock) {
+          const int je2 = min(cnt, jb2 + kSurfBlock);
+          float acc = 0.f;
+#pragma unroll 4
+          for (int jj = jb2; jj < je2; ++jj) {
+            const float4 h2 = shi[jj], l2 = slo[jj];
+            const float dx = (h1.x - h2.x) + (l1.x - l2.x);
+            const float dy = (h1.y - h2.y) + (l1.y - l2.y);
+            const float s = __builtin_fmaf(dy, dy, dx * dx);
+            const float ri = rsq(fmaxf(__builtin_fmaf(h2.z, h2.z, s), kR2Min));
+            float w = h2.z * (ri * ri * ri);
+            // the series coefficients through the scalar cache: the LDS
+            // reads of the sources are the kernel's limit
+            w += surface_far_eval(s * inv_d2, load_uniform(far, 2 * (jb + jj)),
+                                  load_uniform(far, 2 * (jb + jj) + 1));
+            acc = __builtin_fmaf(h2.w, w, acc);
+          }
+          fz += (double)acc;
+        }
+      else if (nic > 0 && far)
         for (int jb2 = 0; jb2 < cnt; jb2 += kSurfBlock) {
           const int je2 = min(cnt, jb2 + kSurfBlock);
           float acc = 0.f;
