@@ -48,11 +48,11 @@ COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per launch
 # workload and kernel form only.  The writes are the kernel's FP64 partial
 # sums, 3 x 64 blocks x N doubles = 100.66 MB; the reads are the packed
 # particles (32 B each) as the block pairs fetch them, mostly from L2.
-MEASURED_TRAFFIC = {("k_accel_sym32", 65536, 10): {"read": 27.614e6, "write": 100.663e6}}
+MEASURED_TRAFFIC = {("k_accel_sym32", 65536, 10): {"read": 27.530e6, "write": 100.663e6}}
 # VALU issue of the same dispatch (profiles/snapshot65536_nic10_valu_counters.txt):
 # SQ_INSTS_VALU over 1024 SIMDs x 2.4 GHz x duration / 4 cycles per wave64
 # instruction.
-MEASURED_VALU_ISSUE = {("k_accel_sym32", 65536, 10): 0.994}
+MEASURED_VALU_ISSUE = {("k_accel_sym32", 65536, 10): 0.976}
 
 
 def cpu_baseline(n_sample: int, nic: int, threads: int):
