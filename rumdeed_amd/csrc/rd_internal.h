@@ -156,7 +156,7 @@ __device__ __forceinline__ int find_entry(const Ent* tab, int ne, int b, bool se
 }
 
 constexpr int kChunk = 256;  // canonical FP64 summation chunk
-constexpr int kJumpStreams = 2;  // streams of a pool's jump loop (4 measured slower: launch-bound)
+constexpr int kJumpStreams = 2;  // streams of a pool's jump loop (3: the same makespan; 4: slower)
 constexpr int kBlock = 256;
 constexpr int kMaxSlots = 4096;
 constexpr int kLife = 1000;  // MAX_LIFE_TIME
