@@ -278,7 +278,7 @@ int rd_pool_step(rd_ctx* ctx, int32_t precision, rd_step_record* records);
 int rd_set_graphs(rd_ctx* ctx, int32_t on);
 
 /* Pool tuning: the mixed-precision jump launches of a step give each
- * workgroup two partial sums of the chain fields once they hold at least
+ * workgroup four partial sums of the chain fields once they hold at least
  * min_workgroups workgroups of one partial each (2048 by default; 0: always).
  * The partials keep their size and order, so results are identical for every
  * threshold.  Library control, no reference counterpart. */

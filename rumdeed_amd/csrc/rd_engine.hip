@@ -939,7 +939,9 @@ __global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* ta
                                                                 const SimState* st, Chains ch,
                                                                 int it, JumpArgs a) {
   __shared__ float4 shi[kChunk], slo[kChunk];
-  __shared__ double red[PPS][3][kJumpWaves][kJumpTargets];  // one buffer per partial
+  // the partials' sums alternate between two buffers: the barrier of partial
+  // q + 1 orders every read of partial q - 1's buffer before its reuse
+  __shared__ double red[PPS > 1 ? 2 : 1][3][kJumpWaves][kJumpTargets];
   __shared__ float4 tgt[4][kJumpTargets];  // wavefront 0's packed targets
   const Ent e = tab[find_entry(tab, ne, blockIdx.x)];
   const SimDesc& s = desc[e.slot];
@@ -1098,21 +1100,22 @@ __global__ __launch_bounds__(64 * kJumpWaves, 8) void k_mh_field32(const Ent* ta
       }
     }
     if (surf) fz = -2.0 * fz;
-    red[q][0][w][lane] = fx;
-    red[q][1][w][lane] = fy;
-    red[q][2][w][lane] = fz;
+    const int qb = q & 1;
+    red[qb][0][w][lane] = fx;
+    red[qb][1][w][lane] = fy;
+    red[qb][2][w][lane] = fz;
     __syncthreads();
     if (by_wave) {
       // lane q of wavefront w adds component q over the 64 lanes, in lane order
       if (lane < 3 && act) {
-        double v = red[q][lane][w][0];
-        for (int u = 1; u < 64; ++u) v += red[q][lane][w][u];
+        double v = red[qb][lane][w][0];
+        for (int u = 1; u < 64; ++u) v += red[qb][lane][w][u];
         part[lane * S + (size_t)sg * m + k] = v;
       }
     } else if (w < 3 && act) {
       // wavefront q adds component q over the wavefronts, in wavefront order
-      double v = red[q][w][0][lane];
-      for (int u = 1; u < kJumpWaves; ++u) v += red[q][w][u][lane];
+      double v = red[qb][w][0][lane];
+      for (int u = 1; u < kJumpWaves; ++u) v += red[qb][w][u][lane];
       part[w * S + (size_t)sg * m + k] = v;
     }
   }
@@ -2552,16 +2555,17 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
     }
     nsettle_all += H.nsettle;
   }
-  // Partials per mixed workgroup: one, or two when the launches are large
-  // enough to keep every CU busy with half as many workgroups (each then
-  // pays the proposal and set-up once for twice the sources).  The partials
-  // themselves keep their size, so the sums do not depend on this choice.
+  // Partials per mixed workgroup: one, or kJumpPairs when the launches are
+  // large enough to keep every CU busy with that many times fewer workgroups
+  // (each then pays the proposal and set-up once for all its sources).  The
+  // partials themselves keep their size, so the sums do not depend on this
+  // choice.
   int pps = 1;
   if (!fp64 && c->nslots_max > 1) {
     long long wg_all = 0;
     for (int h = 0; h < nh; ++h)
       for (const Ent& e : hv[h].ents) wg_all += (long long)jump_groups32(e.nch) * e.segs;
-    if (wg_all >= c->jump_pair_wgs) pps = 2;
+    if (wg_all >= c->jump_pair_wgs) pps = kJumpPairs;
   }
   for (int h = 0; h < nh; ++h) {
     Half& H = hv[h];
@@ -2679,16 +2683,16 @@ static int run_jumps(rd_ctx* c, const std::vector<Ent>& in, int C, int precision
             H.dt, ne, c->d_desc, c->d_state, c->d_wf, V, it, H.a);
       } else {
         if (H.any_planar) {
-          if (pps == 2)
-            k_mh_field32<RD_GEOM_PLANAR, 2><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+          if (pps == kJumpPairs)
+            k_mh_field32<RD_GEOM_PLANAR, kJumpPairs><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
                 H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
           else
             k_mh_field32<RD_GEOM_PLANAR, 1><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
                 H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
         }
         if (H.any_tip) {
-          if (pps == 2)
-            k_mh_field32<RD_GEOM_TIP, 2><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
+          if (pps == kJumpPairs)
+            k_mh_field32<RD_GEOM_TIP, kJumpPairs><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(
                 H.dt, ne, c->d_desc, c->d_state, V, it, H.a);
           else
             k_mh_field32<RD_GEOM_TIP, 1><<<H.nwg, 64 * kJumpWaves, 0, H.stream>>>(

@@ -135,8 +135,8 @@ class Diode:
         return v.value
 
     def set_jump_pairing(self, min_workgroups: int):
-        """Pool launches of at least min_workgroups jump workgroups pair their
-        partial sums (results unchanged; 0: always)."""
+        """Pool launches of at least min_workgroups jump workgroups give each
+        workgroup four of their partial sums (results unchanged; 0: always)."""
         check(self._lib.rd_set_jump_pairing(self._h, int(min_workgroups)))
 
     def set_solo_jumps(self, on: bool):
